@@ -1,0 +1,362 @@
+/*
+ * Small dense host kernels used by the GCG core: the gemm/symm-shaped
+ * products of the Rayleigh-Ritz step and of the in-subspace
+ * orthonormalisation, the Array* helpers, and a symmetric eigensolver.
+ *
+ * They are the table defaults for DenseMatDotDenseMat,
+ * DenseSymMatDotDenseMat, DenseMatEigenSolver and Array*, which the
+ * reference routes to BLAS/LAPACK (src/gcge_ops.c:401-646).  The matrices
+ * are at most (nev+20+2*block)^2, so plain column-major loops are enough
+ * here; the large tall-skinny products live in the device backend.
+ *
+ * Eigensolver: Householder reduction to tridiagonal form followed by the
+ * implicit-shift QL iteration (the Wilkinson-Reinsch method, Handbook for
+ * Automatic Computation vol. II, procedures tred2/tql2), then an ascending
+ * sort.  GCGE_Default_DenseMatEigenSolver keeps the reference dispatch
+ * (src/gcge_ops.c:401-432): the whole spectrum when il == 1 and iu == n is
+ * computed in place in `a` and then copied to evec as n*n contiguous
+ * doubles (which is what the reference's memcpy after dsyev does, also
+ * when lda != n); otherwise eigenpairs il..iu are written to evec with
+ * leading dimension lde.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "gcge.h"
+
+static inline int lsame(const char *c, char u) { return (*c == u) || (*c == (char)(u + 32)); }
+
+/* c = alpha op(a) op(b) + beta c, all column-major (Fortran dgemm contract:
+ * with beta == 0, c is not read). */
+void GCGE_Default_DenseMatDotDenseMat(char *transa, char *transb, GCGE_INT *nrows,
+                                      GCGE_INT *ncols, GCGE_INT *mid, GCGE_DOUBLE *alpha,
+                                      GCGE_DOUBLE *a, GCGE_INT *lda, GCGE_DOUBLE *b,
+                                      GCGE_INT *ldb, GCGE_DOUBLE *beta, GCGE_DOUBLE *c,
+                                      GCGE_INT *ldc)
+{
+    const int M = *nrows, N = *ncols, K = *mid;
+    const int ta = lsame(transa, 'T') || lsame(transa, 'C');
+    const int tb = lsame(transb, 'T') || lsame(transb, 'C');
+    const double al = *alpha, be = *beta;
+    const int LA = *lda, LB = *ldb, LC = *ldc;
+    if (M <= 0 || N <= 0) return;
+    for (int j = 0; j < N; ++j) {
+        double *cj = c + (size_t)j * LC;
+        if (be == 0.0) {
+            for (int i = 0; i < M; ++i) cj[i] = 0.0;
+        } else if (be != 1.0) {
+            for (int i = 0; i < M; ++i) cj[i] *= be;
+        }
+        if (al == 0.0 || K <= 0) continue;
+        if (!ta) {
+            /* column sweep: c(:,j) += alpha * a(:,l) * op(b)(l,j) */
+            for (int l = 0; l < K; ++l) {
+                double blj = tb ? b[j + (size_t)l * LB] : b[l + (size_t)j * LB];
+                double t = al * blj;
+                if (t == 0.0) continue;
+                const double *al_col = a + (size_t)l * LA;
+                for (int i = 0; i < M; ++i) cj[i] += t * al_col[i];
+            }
+        } else {
+            /* dot form: c(i,j) += alpha * a(:,i) . op(b)(:,j) */
+            for (int i = 0; i < M; ++i) {
+                const double *ai = a + (size_t)i * LA;
+                double s = 0.0;
+                if (!tb) {
+                    const double *bj = b + (size_t)j * LB;
+                    for (int l = 0; l < K; ++l) s += ai[l] * bj[l];
+                } else {
+                    for (int l = 0; l < K; ++l) s += ai[l] * b[j + (size_t)l * LB];
+                }
+                cj[i] += al * s;
+            }
+        }
+    }
+}
+
+/* c = alpha a b + beta c (side 'L') or alpha b a + beta c (side 'R'), with
+ * a symmetric and only its `uplo` triangle referenced. */
+void GCGE_Default_DenseSymMatDotDenseMat(char *side, char *uplo, GCGE_INT *nrows, GCGE_INT *ncols,
+                                         GCGE_DOUBLE *alpha, GCGE_DOUBLE *a, GCGE_INT *lda,
+                                         GCGE_DOUBLE *b, GCGE_INT *ldb, GCGE_DOUBLE *beta,
+                                         GCGE_DOUBLE *c, GCGE_INT *ldc)
+{
+    const int M = *nrows, N = *ncols, LA = *lda, LB = *ldb, LC = *ldc;
+    const int left = lsame(side, 'L'), upper = lsame(uplo, 'U');
+    const double al = *alpha, be = *beta;
+    if (M <= 0 || N <= 0) return;
+#define ASYM(i, k)                                                                           \
+    (upper ? ((i) <= (k) ? a[(i) + (size_t)(k) * LA] : a[(k) + (size_t)(i) * LA])          \
+           : ((i) >= (k) ? a[(i) + (size_t)(k) * LA] : a[(k) + (size_t)(i) * LA]))
+    for (int j = 0; j < N; ++j) {
+        for (int i = 0; i < M; ++i) {
+            double s = 0.0;
+            if (left) {
+                for (int k = 0; k < M; ++k) s += ASYM(i, k) * b[k + (size_t)j * LB];
+            } else {
+                for (int k = 0; k < N; ++k) s += b[i + (size_t)k * LB] * ASYM(k, j);
+            }
+            double *cij = c + i + (size_t)j * LC;
+            *cij = (be == 0.0 ? 0.0 : be * *cij) + al * s;
+        }
+    }
+#undef ASYM
+}
+
+GCGE_DOUBLE GCGE_Default_ArrayDotArray(GCGE_DOUBLE *x, GCGE_DOUBLE *y, GCGE_INT length)
+{
+    double s = 0.0;
+    for (int i = 0; i < length; ++i) s += x[i] * y[i];
+    return s;
+}
+
+/* Euclidean norm with scaling, so that very large or tiny entries neither
+ * overflow nor underflow. */
+GCGE_DOUBLE GCGE_Default_ArrayNorm(GCGE_DOUBLE *x, GCGE_INT length)
+{
+    double scale = 0.0, ssq = 1.0;
+    for (int i = 0; i < length; ++i) {
+        if (x[i] == 0.0) continue;
+        double ax = fabs(x[i]);
+        if (scale < ax) {
+            double r = scale / ax;
+            ssq = 1.0 + ssq * r * r;
+            scale = ax;
+        } else {
+            double r = ax / scale;
+            ssq += r * r;
+        }
+    }
+    return scale * sqrt(ssq);
+}
+
+/* y = a x + b y: y is first scaled by b (when b != 1), then a x is added,
+ * the order of the reference default (src/gcge_ops.c:608-622). */
+void GCGE_Default_ArrayAXPBY(GCGE_DOUBLE a, GCGE_DOUBLE *x, GCGE_DOUBLE b, GCGE_DOUBLE *y,
+                             GCGE_INT length)
+{
+    if (b != 1.0)
+        for (int i = 0; i < length; ++i) y[i] *= b;
+    if (a != 0.0)
+        for (int i = 0; i < length; ++i) y[i] += a * x[i];
+}
+
+void GCGE_Default_ArrayCopy(GCGE_DOUBLE *x, GCGE_DOUBLE *y, GCGE_INT length)
+{
+    if (length > 0) memmove(y, x, (size_t)length * sizeof(double));
+}
+
+void GCGE_Default_ArrayScale(GCGE_DOUBLE alpha, GCGE_DOUBLE *a, GCGE_INT length)
+{
+    for (int i = 0; i < length; ++i) a[i] *= alpha;
+}
+
+/* ---- symmetric eigensolver -------------------------------------------- */
+
+static double hypot_safe(double a, double b)
+{
+    double aa = fabs(a), bb = fabs(b);
+    if (aa > bb) {
+        double r = bb / aa;
+        return aa * sqrt(1.0 + r * r);
+    }
+    if (bb == 0.0) return 0.0;
+    double r = aa / bb;
+    return bb * sqrt(1.0 + r * r);
+}
+
+/* Householder tridiagonalisation of the symmetric n x n matrix z (row-major
+ * z[i*n+j], full storage).  On return d holds the diagonal, e[1..n-1] the
+ * sub-diagonal (e[0] = 0) and z the accumulated orthogonal transformation,
+ * whose column j is basis vector j. */
+static void tridiagonalize(int n, double *z, double *d, double *e)
+{
+#define Z(i, j) z[(size_t)(i) * n + (j)]
+    for (int i = n - 1; i > 0; --i) {
+        int l = i - 1;
+        double h = 0.0;
+        if (l > 0) {
+            double scale = 0.0;
+            for (int k = 0; k <= l; ++k) scale += fabs(Z(i, k));
+            if (scale == 0.0) {
+                e[i] = Z(i, l);
+            } else {
+                for (int k = 0; k <= l; ++k) {
+                    Z(i, k) /= scale;
+                    h += Z(i, k) * Z(i, k);
+                }
+                double f = Z(i, l);
+                double g = (f >= 0.0) ? -sqrt(h) : sqrt(h);
+                e[i] = scale * g;
+                h -= f * g;
+                Z(i, l) = f - g;
+                f = 0.0;
+                for (int j = 0; j <= l; ++j) {
+                    Z(j, i) = Z(i, j) / h;
+                    double gg = 0.0;
+                    for (int k = 0; k <= j; ++k) gg += Z(j, k) * Z(i, k);
+                    for (int k = j + 1; k <= l; ++k) gg += Z(k, j) * Z(i, k);
+                    e[j] = gg / h;
+                    f += e[j] * Z(i, j);
+                }
+                double hh = f / (h + h);
+                for (int j = 0; j <= l; ++j) {
+                    double fj = Z(i, j);
+                    double gj = e[j] - hh * fj;
+                    e[j] = gj;
+                    for (int k = 0; k <= j; ++k) Z(j, k) -= (fj * e[k] + gj * Z(i, k));
+                }
+            }
+        } else {
+            e[i] = Z(i, l);
+        }
+        d[i] = h;
+    }
+    d[0] = 0.0;
+    e[0] = 0.0;
+    for (int i = 0; i < n; ++i) {
+        int l = i - 1;
+        if (d[i] != 0.0) {
+            for (int j = 0; j <= l; ++j) {
+                double g = 0.0;
+                for (int k = 0; k <= l; ++k) g += Z(i, k) * Z(k, j);
+                for (int k = 0; k <= l; ++k) Z(k, j) -= g * Z(k, i);
+            }
+        }
+        d[i] = Z(i, i);
+        Z(i, i) = 1.0;
+        for (int j = 0; j <= l; ++j) Z(j, i) = Z(i, j) = 0.0;
+    }
+#undef Z
+}
+
+/* Implicit-shift QL on the tridiagonal (d, e), rotating the columns of z.
+ * Returns 0 on success, 1 when an eigenvalue needs more than 60 sweeps. */
+static int tridiagonal_ql(int n, double *d, double *e, double *z)
+{
+#define Z(i, j) z[(size_t)(i) * n + (j)]
+    for (int i = 1; i < n; ++i) e[i - 1] = e[i];
+    if (n > 0) e[n - 1] = 0.0;
+    for (int l = 0; l < n; ++l) {
+        int iter = 0, m;
+        do {
+            for (m = l; m < n - 1; ++m) {
+                double dd = fabs(d[m]) + fabs(d[m + 1]);
+                if (fabs(e[m]) <= 1.1102230246251565e-16 * dd) break;
+            }
+            if (m != l) {
+                if (iter++ == 60) return 1;
+                double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+                double r = hypot_safe(g, 1.0);
+                g = d[m] - d[l] + e[l] / (g + (g >= 0.0 ? fabs(r) : -fabs(r)));
+                double s = 1.0, c = 1.0, p = 0.0;
+                int i;
+                for (i = m - 1; i >= l; --i) {
+                    double f = s * e[i];
+                    double b = c * e[i];
+                    r = hypot_safe(f, g);
+                    e[i + 1] = r;
+                    if (r == 0.0) {
+                        d[i + 1] -= p;
+                        e[m] = 0.0;
+                        break;
+                    }
+                    s = f / r;
+                    c = g / r;
+                    g = d[i + 1] - p;
+                    r = (d[i] - g) * s + 2.0 * c * b;
+                    p = s * r;
+                    d[i + 1] = g + p;
+                    g = c * r - b;
+                    for (int k = 0; k < n; ++k) {
+                        double zf = Z(k, i + 1);
+                        Z(k, i + 1) = s * Z(k, i) + c * zf;
+                        Z(k, i) = c * Z(k, i) - s * zf;
+                    }
+                }
+                if (r == 0.0 && i >= l) continue;
+                d[l] -= p;
+                e[l] = g;
+                e[m] = 0.0;
+            }
+        } while (m != l);
+    }
+    return 0;
+#undef Z
+}
+
+/* All eigenpairs of the symmetric n x n matrix whose upper triangle is in
+ * a (column-major, lda).  Eigenvalues ascending in eval; eigenvector j in
+ * evec[j*lde .. j*lde+n-1].  work needs n*n + 2n doubles. */
+void GCGE_Dense_SymEig(GCGE_DOUBLE *a, GCGE_INT lda, GCGE_INT n, GCGE_DOUBLE *eval,
+                       GCGE_DOUBLE *evec, GCGE_INT lde, GCGE_DOUBLE *work)
+{
+    if (n <= 0) return;
+    double *z = work, *d = work + (size_t)n * n, *e = d + n;
+    /* full symmetric copy from the upper triangle: z(i,j) = a(min,max) */
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i <= j; ++i) {
+            double v = a[i + (size_t)j * lda];
+            z[(size_t)i * n + j] = v;
+            z[(size_t)j * n + i] = v;
+        }
+    tridiagonalize(n, z, d, e);
+    tridiagonal_ql(n, d, e, z);
+    /* selection sort of the eigenvalues, permuting vector columns */
+    for (int i = 0; i < n - 1; ++i) {
+        int k = i;
+        double p = d[i];
+        for (int j = i + 1; j < n; ++j)
+            if (d[j] < p) {
+                k = j;
+                p = d[j];
+            }
+        if (k != i) {
+            d[k] = d[i];
+            d[i] = p;
+            for (int r = 0; r < n; ++r) {
+                double t = z[(size_t)r * n + i];
+                z[(size_t)r * n + i] = z[(size_t)r * n + k];
+                z[(size_t)r * n + k] = t;
+            }
+        }
+    }
+    for (int j = 0; j < n; ++j) {
+        eval[j] = d[j];
+        for (int r = 0; r < n; ++r) evec[r + (size_t)j * lde] = z[(size_t)r * n + j];
+    }
+}
+
+void GCGE_Default_DenseMatEigenSolver(GCGE_DOUBLE *a, GCGE_INT lda, GCGE_INT nrows,
+                                      GCGE_DOUBLE *eval, GCGE_DOUBLE *evec, GCGE_INT lde,
+                                      GCGE_INT il, GCGE_INT iu, GCGE_INT *iwork,
+                                      GCGE_DOUBLE *dwork)
+{
+    (void)iwork;
+    (void)dwork;
+    const int n = nrows;
+    if (n <= 0) return;
+    double *vals = (double *)malloc((size_t)n * sizeof(double));
+    double *vecs = (double *)malloc((size_t)n * n * sizeof(double));
+    double *work = (double *)malloc(((size_t)n * n + 2 * (size_t)n) * sizeof(double));
+    GCGE_Dense_SymEig(a, lda, n, vals, vecs, n, work);
+    if (il == 1 && iu == n) {
+        /* whole spectrum: vectors replace a (leading dimension lda), then n*n
+         * contiguous doubles are copied from a to evec */
+        for (int j = 0; j < n; ++j) {
+            eval[j] = vals[j];
+            memcpy(a + (size_t)j * lda, vecs + (size_t)j * n, (size_t)n * sizeof(double));
+        }
+        memcpy(evec, a, (size_t)n * n * sizeof(double));
+    } else {
+        for (int j = il - 1; j < iu; ++j) {
+            eval[j - (il - 1)] = vals[j];
+            memcpy(evec + (size_t)(j - (il - 1)) * lde, vecs + (size_t)j * n,
+                   (size_t)n * sizeof(double));
+        }
+    }
+    free(vals);
+    free(vecs);
+    free(work);
+}

@@ -1,0 +1,179 @@
+/*
+ * Ritz vectors X, inner-solve directions W and conjugate directions P.
+ *
+ *   GCGE_ComputeX  reference src/gcge_xpw.c:47-76
+ *       X(:, nsl:nev) = V(:, nsl:dim_v) * C(nsl:dim_v, nsl:nev)
+ *   GCGE_ComputeW  reference src/gcge_xpw.c:118-260
+ *       rhs_i = (lambda_i + shift) B x_i with initial guess x_i when
+ *       |lambda_i| <= 1, rhs_i = B x_i with guess x_i / (lambda_i + shift)
+ *       otherwise; then MultiLinearSolver / LinearSolver / block CG / CG.
+ *   GCGE_ComputeP  reference src/gcge_xpw.c:329-478
+ *       P coefficients from the unconverged Ritz coefficients with their X
+ *       rows zeroed, orthonormalised in the small space, then one linear
+ *       combination of V into V_tmp and a swap into V.
+ * When the backend publishes BlockCG it replaces the column-by-column
+ * GCGE_BCG with the same argument list and arithmetic.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "gcge.h"
+
+void GCGE_ComputeX(void **V, GCGE_DOUBLE *subspace_evec, void **X, GCGE_INT size_X,
+                   GCGE_INT size_V, GCGE_OPS *ops, GCGE_WORKSPACE *ws)
+{
+    GCGE_INT sx = ws->num_soft_locked_in_last_iter;
+    GCGE_INT start[2] = {sx, sx};
+    GCGE_INT end[2] = {size_V, size_X};
+    ops->MultiVecLinearComb(V, X, start, end, subspace_evec + (size_t)sx * size_V + sx, size_V, 0,
+                            1.0, 0.0, ops);
+}
+
+void GCGE_ComputeW(void *A, void *B, void **V, GCGE_DOUBLE *eval, GCGE_OPS *ops,
+                   GCGE_PARA *para, GCGE_WORKSPACE *ws)
+{
+    const GCGE_INT w_start = ws->dim_xp;
+    const GCGE_INT w_length = ws->unconv_bs;
+    GCGE_INT *unlock = ws->unlock;
+    void **V_tmp = ws->V_tmp;
+    void *rhs, *xc, *wc;
+    GCGE_INT mv_s[5], mv_e[5];
+
+    for (GCGE_INT idx = 0; idx < w_length; ++idx) {
+        GCGE_INT xi = unlock[idx], wi = w_start + idx;
+        ops->GetVecFromMultiVec(V_tmp, idx, &rhs, ops);
+        ops->GetVecFromMultiVec(V, xi, &xc, ops);
+        ops->GetVecFromMultiVec(V, wi, &wc, ops);
+        if (B == NULL)
+            ops->VecAxpby(1.0, xc, 0.0, rhs, ops);
+        else
+            ops->MatDotVec(B, xc, rhs, ops);
+        if (fabs(eval[xi]) <= 1.0) {
+            ops->VecAxpby(0.0, rhs, eval[xi] + para->shift, rhs, ops);
+            ops->VecAxpby(1.0, xc, 0.0, wc, ops);
+        } else {
+            ops->VecAxpby(1.0 / (eval[xi] + para->shift), xc, 0.0, wc, ops);
+        }
+        ops->RestoreVecForMultiVec(V, xi, &xc, ops);
+        ops->RestoreVecForMultiVec(V, wi, &wc, ops);
+        ops->RestoreVecForMultiVec(V_tmp, idx, &rhs, ops);
+    }
+
+    /* ranges: [0] rhs in V_tmp, [1] solution in V, [2] CG_p, [3] scratch, [4] spare */
+    mv_s[0] = 0; mv_e[0] = w_length;
+    mv_s[1] = w_start; mv_e[1] = w_start + w_length;
+    mv_s[2] = 0; mv_e[2] = w_length;
+    mv_s[3] = 0; mv_e[3] = w_length;
+    mv_s[4] = 0; mv_e[4] = w_length;
+
+    if (ops->MultiLinearSolver) {
+        ops->MultiLinearSolver(A, V_tmp, V, mv_s, mv_e, ops);
+        return;
+    }
+    if (ops->LinearSolver) {
+        for (GCGE_INT idx = 0; idx < w_length; ++idx) {
+            ops->GetVecFromMultiVec(V_tmp, idx, &rhs, ops);
+            ops->GetVecFromMultiVec(V, w_start + idx, &wc, ops);
+            ops->LinearSolver(A, rhs, wc, ops);
+            ops->RestoreVecForMultiVec(V, w_start + idx, &wc, ops);
+            ops->RestoreVecForMultiVec(V_tmp, idx, &rhs, ops);
+        }
+        return;
+    }
+    if (para->if_use_bcg == 1) {
+        if (para->if_shift == 1) para->use_bcg_continuous = 0;
+        if (para->use_bcg_continuous) {
+            fprintf(stderr, "gcge: use_bcg_continuous is not available in this build\n");
+            exit(1);
+        }
+        GCGE_OPS_EXT *ext = GCGE_OPS_GetExt(ops);
+        if (para->use_ops_ext && ext && ext->BlockCG) {
+            ext->BlockCG(A, para->if_shift, para->shift, B, V_tmp, V, mv_s, mv_e, para->cg_max_it,
+                         para->cg_rate, ops, ws->CG_p, ws->evec, ws->subspace_dtmp,
+                         ws->subspace_itmp);
+        } else {
+            GCGE_BCG(A, para->if_shift, para->shift, B, V_tmp, V, mv_s, mv_e, para->cg_max_it,
+                     para->cg_rate, ops, ws->CG_p, ws->evec, NULL, ws->subspace_dtmp,
+                     ws->subspace_itmp);
+        }
+    } else {
+        for (GCGE_INT idx = 0; idx < w_length; ++idx) {
+            ops->GetVecFromMultiVec(V_tmp, idx, &rhs, ops);
+            ops->GetVecFromMultiVec(V, w_start + idx, &wc, ops);
+            GCGE_CG(A, rhs, wc, ops, para, ws->V_tmp, ws->evec);
+            ops->RestoreVecForMultiVec(V, w_start + idx, &wc, ops);
+            ops->RestoreVecForMultiVec(V_tmp, idx, &rhs, ops);
+        }
+    }
+}
+
+void GCGE_ComputeP(GCGE_DOUBLE *subspace_evec, void **V, GCGE_OPS *ops, GCGE_PARA *para,
+                   GCGE_WORKSPACE *ws)
+{
+    GCGE_DOUBLE t1 = GCGE_GetTime(), t2;
+    GCGE_DOUBLE *coef = subspace_evec;
+    const GCGE_INT ldc = ws->dim_xpw;
+    GCGE_INT *unlock = ws->unlock;
+    GCGE_INT p_start = ws->dim_x;
+    GCGE_INT p_ncols = ws->unconv_bs;
+    const GCGE_INT xx_nrows = ws->last_dim_x;
+    void **V_tmp = ws->V_tmp;
+    GCGE_INT mv_s[2], mv_e[2];
+
+    /* coefficient blocks (columns: new X | new P; rows: old X, P, W):
+     *   XX  PX        PX := 0
+     *   XP  PP        PP, PW := rows below the old X of unconverged X columns
+     *   XW  PW                                                             */
+    GCGE_DOUBLE *px = coef + (size_t)p_start * ldc;
+    GCGE_DOUBLE *pp = px + xx_nrows;
+    GCGE_DOUBLE *xp = coef + xx_nrows;
+    memset(px, 0, (size_t)p_ncols * ldc * sizeof(GCGE_DOUBLE));
+    for (GCGE_INT cp = 0; cp < p_ncols; ++cp)
+        memcpy(pp + (size_t)cp * ldc, xp + (size_t)unlock[cp] * ldc,
+               (size_t)(ldc - xx_nrows) * sizeof(GCGE_DOUBLE));
+
+    /* orthonormalise the trailing block (from the first unlocked row/column) */
+    const GCGE_INT orth_start = ws->num_soft_locked_in_last_iter;
+    const GCGE_INT orth_length = ldc - orth_start;
+    GCGE_DOUBLE *sub = coef + (size_t)orth_start * ldc + orth_start;
+    p_start -= orth_start;
+    GCGE_INT p_end = p_start + p_ncols;
+    if (strcmp(para->p_orth_type, "multi") == 0) {
+        GCGE_StableMultiOrthonormalizationInSubspace(sub, ldc, orth_length, p_start, &p_end, NULL,
+                                                     -1, para->orth_para, ws, ops);
+    } else if (strcmp(para->p_orth_type, "scbgs") == 0 || strcmp(para->p_orth_type, "bgs") == 0) {
+        fprintf(stderr, "gcge: p_orth_type \"%s\" is not available in this build\n",
+                para->p_orth_type);
+        exit(1);
+    } else {
+        GCGE_OrthonormalizationInSubspace(sub, ldc, orth_length, p_start, &p_end, NULL, -1,
+                                          para->orth_para, ops);
+    }
+    p_ncols = p_end - p_start;
+    p_end += orth_start;
+    t2 = GCGE_GetTime();
+    para->stat_para->part_time_one_iter->p_orth_time = t2 - t1;
+    para->stat_para->part_time_total->p_orth_time += t2 - t1;
+
+    /* V_tmp(:, 0:p_end-nev) = V(:, orth_start:ldc) * coef(orth_start:ldc, nev:p_end),
+     * then swap into V(:, nev:p_end) */
+    const GCGE_INT nev = para->nev;
+    const GCGE_INT p_end_tmp = p_end - nev;
+    mv_s[0] = orth_start; mv_e[0] = ldc;
+    mv_s[1] = 0; mv_e[1] = p_end_tmp;
+    ops->MultiVecLinearComb(V, V_tmp, mv_s, mv_e, coef + (size_t)nev * ldc + orth_start, ldc, 0,
+                            1.0, 0.0, ops);
+    mv_s[0] = nev; mv_e[0] = p_end;
+    mv_s[1] = 0; mv_e[1] = p_end_tmp;
+    ops->MultiVecSwap(V, V_tmp, mv_s, mv_e, ops);
+    ws->dim_xp = p_end;
+
+    GCGE_OPS_EXT *ext = GCGE_OPS_GetExt(ops);
+    if (ext && ext->Synchronize) ext->Synchronize(ops);
+    t1 = GCGE_GetTime();
+    para->stat_para->part_time_one_iter->p_axpy_time = t1 - t2;
+    para->stat_para->part_time_total->p_axpy_time += t1 - t2;
+    (void)p_ncols;
+}
