@@ -1,0 +1,181 @@
+"""Benchmark: seconds to nev eigenpairs of the Q1 FEM pencil on one MI355X.
+
+Workload (one "step" = one complete GCG solve): trilinear Q1 hexahedral
+elements on the uniform 64^3 mesh of the unit cube, Dirichlet nodes
+eliminated (n = 63^3 = 250047), stiffness K and consistent mass M with
+kappa = 1, nev = 50 smallest eigenpairs of K x = lambda M x, float64
+throughout, converged to ||Kx - lambda Mx|| / (|lambda| ||Mx||) <= 1e-8.
+Matrices are uploaded before timing; the initial block is drawn on the
+device inside each solve from the seed.
+
+The JSON line carries:
+  value          mean seconds per solve over the timed steps (lower is better)
+  roofline       the block SpMM (dominant kernel): algorithmic bytes over its
+                 HIP-event time summed over the timed region, against 8 TB/s
+  cpu_baseline   the reference CSR path built from its own sources (or the
+                 oracle restatement when that binary is absent), one core,
+                 timed on a bounded number of GCG iterations of the same
+                 workload and scaled to the device run's iteration count
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "s to nev eigpairs @resid 1e-8: FP32-inner vs FP64 on GPU; SpMM HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--mesh", type=int, default=64, help="elements per axis")
+    p.add_argument("--nev", type=int, default=50)
+    p.add_argument("--tol", type=float, default=1e-8)
+    p.add_argument("--cpu-iters", type=int, default=3,
+                   help="GCG iterations timed for the CPU baseline (0: skip)")
+    p.add_argument("--no-profile", action="store_true",
+                   help="time without per-kernel events (no roofline numbers)")
+    return p.parse_args()
+
+
+def cpu_baseline(K, M, nev, tol, iters, dev_iters):
+    """Time `iters` GCG iterations of the reference CSR path on one core and
+    scale per-iteration cost to the device run's iteration count."""
+    import oracle
+
+    flags = ["-gcge_ev_max_it", str(iters), "-gcge_ev_tol", str(tol)]
+    if oracle.ref_available():
+        kind = "reference"
+        r = oracle.ref_solve(K, M, nev, flags)
+        secs, done = r["time"], r["num_iter"]
+    else:
+        kind = "port"
+        r = oracle.solve(K, M, nev, flags + ["-gcge_conv_type", "RB"])
+        secs, done = r["time"], r["num_iter"]
+    per_iter = secs / max(done, 1)
+    return {
+        "value": per_iter * dev_iters,
+        "unit": "s",
+        "cores": 1,
+        "kind": kind,
+        "sample": f"{done} GCG iterations of the same workload in {secs:.1f} s on one host core "
+                  f"({per_iter:.2f} s/iteration), scaled to the device run's {dev_iters} iterations",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the row-partitioned multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import gcge_old_amd as gcge
+    from gcge_old_amd import _native, problems
+
+    gcge.init(0)
+    lib = _native.hip()
+    E = args.mesh
+    t0 = time.time()
+    K, M = problems.q1_fem_3d(E)
+    assemble_s = time.time() - t0
+    dK, dM = gcge.DeviceMatrix(K), gcge.DeviceMatrix(M)
+    flags = ["-gcge_conv_type", "RB", "-gcge_ev_tol", str(args.tol), "-gcge_ev_max_it", "500"]
+    print(f"# workload: Q1 {E}^3 n={K.nrows} nnz(K)={K.nnz} nnz(M)={M.nnz} nev={args.nev} "
+          f"(assembly {assemble_s:.1f}s)", file=sys.stderr, flush=True)
+
+    for w in range(args.warmup):
+        out = gcge.solve(dK, dM, args.nev, flags, seed=100 + w)
+        print(f"# warmup {w}: {out['time']:.4f}s iters={out['num_iter']} conv={out['num_conv']}",
+              file=sys.stderr, flush=True)
+
+    lib.GCGE_HIP_ProfileReset()
+    lib.GCGE_HIP_ProfileEnable(0 if args.no_profile else 1)
+    lib.GCGE_HIP_Synchronize()
+    times, iters, outs = [], [], []
+    t_start = time.perf_counter()
+    for s in range(args.steps):
+        out = gcge.solve(dK, dM, args.nev, flags, seed=1 + s)
+        times.append(out["time"])
+        iters.append(out["num_iter"])
+        outs.append(out)
+        print(f"# step {s}: {out['time']:.4f}s iters={out['num_iter']} conv={out['num_conv']}",
+              file=sys.stderr, flush=True)
+    lib.GCGE_HIP_Synchronize()
+    wall = time.perf_counter() - t_start
+    lib.GCGE_HIP_ProfileEnable(0)
+
+    import ctypes as C
+
+    kernels = {}
+    for name in ("spmm", "gram", "gram_reduce", "lincomb", "axpby", "swap", "coldots", "bcg_p",
+                 "bcg_xr", "residual", "random"):
+        ms, cnt, byt = C.c_double(0.0), C.c_int(0), C.c_double(0.0)
+        lib.GCGE_HIP_KernelStats(name.encode(), C.byref(ms), C.byref(cnt), C.byref(byt))
+        if cnt.value:
+            kernels[name] = {"ms": ms.value, "launches": cnt.value, "bytes": byt.value}
+
+    value = float(np.mean(times))
+    roofline = None
+    if "spmm" in kernels and kernels["spmm"]["ms"] > 0:
+        sp = kernels["spmm"]
+        achieved = sp["bytes"] / (sp["ms"] * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "kernel": "spmm", "launches": sp["launches"],
+                    "avg_launch_us": round(sp["ms"] * 1e3 / sp["launches"], 2),
+                    "algorithmic_bytes_per_launch": round(sp["bytes"] / sp["launches"])}
+
+    # accuracy of the last solve against the closed form
+    exact = problems.q1_eigs(E, args.nev)
+    last = outs[-1]
+    eig_err = float(np.max(np.abs(np.sort(last["eval"]) - exact) / exact))
+
+    cpu = None
+    if args.cpu_iters > 0:
+        try:
+            cpu = cpu_baseline(K, M, args.nev, args.tol, args.cpu_iters, int(np.mean(iters)))
+        except Exception as e:  # the baseline is reported, never required
+            print(f"# cpu baseline skipped: {e}", file=sys.stderr)
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 5),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(value * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"Q1 FEM {E}^3 unit cube, kappa=1, K x = lambda M x, nev={args.nev}, "
+                               f"FP64, ||r||/(|lambda| ||Mx||) <= {args.tol:g}",
+                   "n": K.nrows, "nnz_K": K.nnz, "nnz_M": M.nnz, "nev": args.nev,
+                   "block_size": last["block_size"], "parallelism": "1 GPU"},
+        "iterations": iters,
+        "max_rel_eig_err_vs_closed_form": eig_err,
+        "wall_s_timed_region": round(wall, 4),
+        "roofline": roofline,
+        "kernels": {k: {"ms": round(v["ms"], 3), "launches": v["launches"]}
+                    for k, v in kernels.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -83,6 +83,10 @@ def _declare(name: str, lib: C.CDLL) -> None:
         lib.GCGE_HIP_Synchronize.argtypes = []
         lib.GCGE_HIP_KernelTime.argtypes = [C.c_char_p, dptr, iptr]
         lib.GCGE_HIP_KernelTime.restype = C.c_int
+        lib.GCGE_HIP_KernelStats.argtypes = [C.c_char_p, dptr, iptr, dptr]
+        lib.GCGE_HIP_KernelStats.restype = C.c_int
+        lib.GCGE_HIP_ProfileEnable.argtypes = [C.c_int]
+        lib.GCGE_HIP_ProfileReset.argtypes = []
 
 
 def as_dptr(a):

@@ -1,0 +1,99 @@
+"""In-tree build of the native libraries.
+
+    python -m gcge_old_amd.build
+
+libgcge_amd.so   GCG core, host C (gcc)
+libgcge_hip.so   MI355X backend, HIP C++ for gfx950 (hipcc)
+
+Every HIP translation unit is compiled with packed-FP32 instructions
+disabled (-Xclang -target-feature -Xclang -packed-fp32-ops).  Objects are
+compiled in parallel and linked into gcge_old_amd/lib/.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import glob
+import os
+import shutil
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "gcge_old_amd")
+CSRC = os.path.join(PKG, "csrc")
+LIB = os.path.join(PKG, "lib")
+OBJ = os.path.join(PKG, "lib", "obj")
+INCLUDE = os.path.join(ROOT, "include")
+ARCH = os.environ.get("GCGE_HIP_ARCH", "gfx950")
+
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+CC = shutil.which("gcc") or "cc"
+
+CORE_FLAGS = ["-O2", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter", "-I" + INCLUDE]
+HIP_FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-I" + INCLUDE,
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wno-unused-result"]
+
+
+def _run(cmd: list[str]) -> None:
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError(f"build failed: {' '.join(cmd)}\n{p.stdout}\n{p.stderr}")
+
+
+def _compile(job):
+    cmd, = job
+    _run(cmd)
+
+
+def build_core() -> str:
+    os.makedirs(OBJ, exist_ok=True)
+    srcs = sorted(glob.glob(os.path.join(CSRC, "core", "*.c")))
+    objs, jobs = [], []
+    for s in srcs:
+        o = os.path.join(OBJ, "core_" + os.path.basename(s) + ".o")
+        objs.append(o)
+        jobs.append(([CC] + CORE_FLAGS + ["-c", s, "-o", o],))
+    with cf.ThreadPoolExecutor(max_workers=8) as ex:
+        list(ex.map(_compile, jobs))
+    out = os.path.join(LIB, "libgcge_amd.so")
+    _run([CC, "-shared", "-o", out] + objs + ["-lm"])
+    return out
+
+
+def build_hip() -> str:
+    os.makedirs(OBJ, exist_ok=True)
+    srcs = sorted(glob.glob(os.path.join(CSRC, "hip", "*.hip")))
+    objs, jobs = [], []
+    for s in srcs:
+        o = os.path.join(OBJ, "hip_" + os.path.basename(s) + ".o")
+        objs.append(o)
+        jobs.append(([HIPCC] + HIP_FLAGS + ["-c", s, "-o", o],))
+    with cf.ThreadPoolExecutor(max_workers=8) as ex:
+        list(ex.map(_compile, jobs))
+    out = os.path.join(LIB, "libgcge_hip.so")
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", out] + objs)
+    return out
+
+
+def build_oracle(with_reference: bool = True) -> None:
+    """The checker: the CPU restatement, and the reference built from its own
+    sources when the reference tree is present (this container only)."""
+    odir = os.path.join(ROOT, "oracle")
+    _run(["make", "-s", "-C", odir])
+    if with_reference and os.path.isdir("/root/reference/src"):
+        _run(["make", "-s", "-j8", "-C", odir, "ref"])
+
+
+def build_all(verbose: bool = True) -> None:
+    with cf.ThreadPoolExecutor(max_workers=3) as ex:
+        futs = [ex.submit(build_core), ex.submit(build_hip), ex.submit(build_oracle)]
+        for f in futs:
+            r = f.result()
+            if verbose and r:
+                print("built", r)
+
+
+if __name__ == "__main__":
+    build_all()
+    sys.exit(0)

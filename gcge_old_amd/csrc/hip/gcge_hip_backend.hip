@@ -1,0 +1,1010 @@
+// The MI355X backend of the GCG eigensolver: device matrices and multivecs,
+// the operations table, the block extensions and the C ABI of gcge_hip.h.
+//
+// Operation semantics follow the reference CSR backend and the reference
+// column-loop defaults they replace (app/csr/gcge_app_csr.c:284-396,
+// src/gcge_ops.c:102-398); the differences are in how the work is done:
+//   - a multivec is one column-major device block; column views are Vec
+//     structs pointing into it (no per-vector allocations);
+//   - MultiVecSwap copies the two column ranges instead of exchanging
+//     handles (the SLEPc backend of the reference does the same,
+//     app/slepc/gcge_app_slepc.c:365-392);
+//   - inner products, linear combinations and SpMM are single block kernels;
+//   - MultiLinearSolver / BlockCG run the reference block CG (src/gcge_cg.c:
+//     302-501) with all columns of one CG step in one launch per kernel and
+//     one host synchronisation per step.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "gcge.h"
+#include "gcge_hip.h"
+#include "gcge_hip_internal.h"
+
+namespace gcge_hip {
+
+// ---- errors -----------------------------------------------------------------------
+static std::string g_last_error;
+
+void fail(const char *what, hipError_t err, const char *file, int line)
+{
+    char buf[512];
+    snprintf(buf, sizeof(buf), "gcge_hip: %s failed: %s (%s:%d)", what, hipGetErrorString(err),
+             file, line);
+    if (g_last_error.empty()) g_last_error = buf;
+    fprintf(stderr, "%s\n", buf);
+    fflush(stderr);
+    std::exit(3);
+}
+
+void fail_msg(const char *msg)
+{
+    if (g_last_error.empty()) g_last_error = msg;
+    fprintf(stderr, "gcge_hip: %s\n", msg);
+    fflush(stderr);
+    std::exit(3);
+}
+
+// ---- state and buffers ----------------------------------------------------------------
+State &state()
+{
+    static State s;
+    return s;
+}
+
+int ensure_init()
+{
+    State &s = state();
+    if (s.stream) return 0;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+        if (g_last_error.empty()) g_last_error = "no HIP device";
+        return 1;
+    }
+    if (s.device < 0) s.device = 0;
+    HIP_CHECK(hipSetDevice(s.device));
+    HIP_CHECK(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+    for (int i = 0; i < State::kRing; ++i) HIP_CHECK(hipEventCreateWithFlags(&s.ring_event[i], hipEventDisableTiming));
+    return 0;
+}
+
+static void require_init()
+{
+    if (ensure_init() != 0) fail_msg("no usable GPU (GCGE_HIP_Init failed)");
+}
+
+template <typename T>
+static T *grow_device(T *&buf, size_t &cap, size_t need)
+{
+    if (need <= cap && buf) return buf;
+    State &s = state();
+    if (buf) {
+        HIP_CHECK(hipStreamSynchronize(s.stream));
+        HIP_CHECK(hipFree(buf));
+    }
+    size_t n = std::max<size_t>(need, 1024);
+    n = std::max(n, cap * 3 / 2);
+    HIP_CHECK(hipMalloc((void **)&buf, n * sizeof(T)));
+    cap = n;
+    return buf;
+}
+
+double *partial_buffer(size_t d) { return grow_device(state().d_partial, state().partial_cap, d); }
+double *small_buffer(size_t d) { return grow_device(state().d_small, state().small_cap, d); }
+double *small_buffer2(size_t d) { return grow_device(state().d_small2, state().small2_cap, d); }
+int *index_buffer(size_t d) { return grow_device(state().d_index, state().index_cap, d); }
+
+double *result_buffer(size_t d)
+{
+    State &s = state();
+    if (d > s.result_cap || !s.h_result) {
+        if (s.h_result) {
+            HIP_CHECK(hipStreamSynchronize(s.stream));
+            HIP_CHECK(hipHostFree(s.h_result));
+        }
+        size_t n = std::max<size_t>(d, 4096);
+        HIP_CHECK(hipHostMalloc((void **)&s.h_result, n * sizeof(double), hipHostMallocDefault));
+        s.result_cap = n;
+    }
+    return s.h_result;
+}
+
+void upload_async(void *dst, const void *src, size_t bytes)
+{
+    if (bytes == 0) return;
+    State &s = state();
+    const int slot = s.ring_next;
+    s.ring_next = (s.ring_next + 1) % State::kRing;
+    HIP_CHECK(hipEventSynchronize(s.ring_event[slot]));  // slot's previous copy finished
+    const size_t need = (bytes + sizeof(double) - 1) / sizeof(double);
+    if (need > s.ring_cap[slot] || !s.h_ring[slot]) {
+        if (s.h_ring[slot]) HIP_CHECK(hipHostFree(s.h_ring[slot]));
+        size_t n = std::max<size_t>(need, 8192);
+        HIP_CHECK(hipHostMalloc((void **)&s.h_ring[slot], n * sizeof(double), hipHostMallocDefault));
+        s.ring_cap[slot] = n;
+    }
+    memcpy(s.h_ring[slot], src, bytes);
+    HIP_CHECK(hipMemcpyAsync(dst, s.h_ring[slot], bytes, hipMemcpyHostToDevice, s.stream));
+    HIP_CHECK(hipEventRecord(s.ring_event[slot], s.stream));
+}
+
+// ---- profiling ------------------------------------------------------------------------------
+struct ProfRecord {
+    const char *name;
+    hipEvent_t start, stop;
+    double bytes;
+};
+static std::vector<ProfRecord> g_prof;
+static std::vector<hipEvent_t> g_event_pool;  // events of past records, reused
+
+static hipEvent_t pooled_event()
+{
+    if (!g_event_pool.empty()) {
+        hipEvent_t e = g_event_pool.back();
+        g_event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    HIP_CHECK(hipEventCreate(&e));
+    return e;
+}
+
+ProfScope::ProfScope(const char *name, double bytes) : slot(-1)
+{
+    State &s = state();
+    if (!s.profiling) return;
+    ProfRecord r{name, pooled_event(), pooled_event(), bytes};
+    HIP_CHECK(hipEventRecord(r.start, s.stream));
+    slot = (int)g_prof.size();
+    g_prof.push_back(r);
+}
+
+ProfScope::~ProfScope()
+{
+    if (slot < 0) return;
+    HIP_CHECK(hipEventRecord(g_prof[slot].stop, state().stream));
+}
+
+// ---- matrices ---------------------------------------------------------------------------------
+struct MatHost {
+    std::vector<int> nz_count;  // entries with |a_ij| > 1e-20 per row (Dirichlet test)
+};
+static std::vector<std::pair<Mat *, MatHost *>> g_mat_host;
+
+static MatHost *mat_host(const Mat *m)
+{
+    for (auto &p : g_mat_host)
+        if (p.first == m) return p.second;
+    return nullptr;
+}
+
+Mat *create_mat(int nrows, int ncols, int nnz, const int *rowptr, const int *colidx,
+                const double *vals)
+{
+    require_init();
+    if (nrows < 0 || ncols < 0 || nnz < 0 || rowptr[0] != 0 || rowptr[nrows] != nnz) {
+        fail_msg("GCGE_HIP_MatCreateCSR: inconsistent CSR arrays");
+    }
+    for (int i = 0; i < nrows; ++i)
+        if (rowptr[i + 1] < rowptr[i]) fail_msg("GCGE_HIP_MatCreateCSR: row pointers decrease");
+    for (int e = 0; e < nnz; ++e)
+        if (colidx[e] < 0 || colidx[e] >= ncols) fail_msg("GCGE_HIP_MatCreateCSR: column index out of range");
+    Mat *m = (Mat *)calloc(1, sizeof(Mat));
+    m->magic = kMatMagic;
+    m->nrows = nrows;
+    m->ncols = ncols;
+    m->nnz = nnz;
+    const int ns = (nrows + 63) / 64;
+    m->nslices = ns;
+    std::vector<int> sptr(ns + 1, 0), sw(ns, 0);
+    for (int s = 0; s < ns; ++s) {
+        int w = 0;
+        for (int r = s * 64; r < std::min(nrows, s * 64 + 64); ++r) w = std::max(w, rowptr[r + 1] - rowptr[r]);
+        sw[s] = w;
+        sptr[s + 1] = sptr[s] + w * 64;
+    }
+    const size_t len = (size_t)sptr[ns];
+    m->sell_len = len;
+    std::vector<int> scol(std::max<size_t>(len, 1));
+    std::vector<double> sval(std::max<size_t>(len, 1));
+    for (int s = 0; s < ns; ++s) {
+        for (int lane = 0; lane < 64; ++lane) {
+            const int r = s * 64 + lane;
+            const int len_r = r < nrows ? rowptr[r + 1] - rowptr[r] : 0;
+            for (int j = 0; j < sw[s]; ++j) {
+                const size_t pos = (size_t)sptr[s] + (size_t)j * 64 + lane;
+                if (j < len_r) {
+                    scol[pos] = colidx[rowptr[r] + j];
+                    sval[pos] = vals[rowptr[r] + j];
+                } else {  // padding: a zero entry in the row's own column
+                    scol[pos] = r < ncols ? r : 0;
+                    sval[pos] = 0.0;
+                }
+            }
+        }
+    }
+    State &st = state();
+    HIP_CHECK(hipMalloc((void **)&m->d_slice_ptr, (ns + 1) * sizeof(int)));
+    HIP_CHECK(hipMalloc((void **)&m->d_slice_w, std::max(ns, 1) * sizeof(int)));
+    HIP_CHECK(hipMalloc((void **)&m->d_col, scol.size() * sizeof(int)));
+    HIP_CHECK(hipMalloc((void **)&m->d_val, sval.size() * sizeof(double)));
+    HIP_CHECK(hipMemcpyAsync(m->d_slice_ptr, sptr.data(), (ns + 1) * sizeof(int), hipMemcpyHostToDevice, st.stream));
+    if (ns) HIP_CHECK(hipMemcpyAsync(m->d_slice_w, sw.data(), ns * sizeof(int), hipMemcpyHostToDevice, st.stream));
+    HIP_CHECK(hipMemcpyAsync(m->d_col, scol.data(), scol.size() * sizeof(int), hipMemcpyHostToDevice, st.stream));
+    HIP_CHECK(hipMemcpyAsync(m->d_val, sval.data(), sval.size() * sizeof(double), hipMemcpyHostToDevice, st.stream));
+    HIP_CHECK(hipStreamSynchronize(st.stream));
+    MatHost *mh = new MatHost;
+    mh->nz_count.resize(nrows);
+    for (int i = 0; i < nrows; ++i) {
+        int c = 0;
+        for (int e = rowptr[i]; e < rowptr[i + 1]; ++e)
+            if (std::fabs(vals[e]) > 1e-20) ++c;
+        mh->nz_count[i] = c;
+    }
+    g_mat_host.push_back({m, mh});
+    return m;
+}
+
+static Mat *as_mat(void *h)
+{
+    Mat *m = (Mat *)h;
+    if (!m || m->magic != kMatMagic) fail_msg("not a GCGE_HIP matrix handle");
+    return m;
+}
+
+// ---- vectors and multivecs ----------------------------------------------------------------------
+MultiVec *as_mv(void **handle)
+{
+    MultiVec *mv = (MultiVec *)handle;
+    if (!mv || mv->magic != kMultiVecMagic) fail_msg("not a GCGE_HIP multivec handle");
+    return mv;
+}
+
+Vec *as_vec(void *handle)
+{
+    if (!handle) fail_msg("null GCGE_HIP vector handle");
+    return (Vec *)handle;
+}
+
+static MultiVec *create_mv(int n, int ncols)
+{
+    require_init();
+    MultiVec *mv = (MultiVec *)calloc(1, sizeof(MultiVec));
+    mv->magic = kMultiVecMagic;
+    mv->n = n;
+    mv->ncols = ncols;
+    mv->ld = padded_ld(n);
+    if (ncols > 0) {
+        HIP_CHECK(hipMalloc((void **)&mv->data, (size_t)mv->ld * ncols * sizeof(double)));
+        HIP_CHECK(hipMemsetAsync(mv->data, 0, (size_t)mv->ld * ncols * sizeof(double), state().stream));
+    }
+    mv->views = (Vec *)calloc((size_t)std::max(ncols, 1), sizeof(Vec));
+    for (int j = 0; j < ncols; ++j) {
+        mv->views[j].data = mv->data + (size_t)j * mv->ld;
+        mv->views[j].n = n;
+        mv->views[j].owner = 0;
+        mv->views[j].parent = mv;
+        mv->views[j].col = j;
+    }
+    return mv;
+}
+
+static void destroy_mv(MultiVec *mv)
+{
+    if (!mv) return;
+    HIP_CHECK(hipStreamSynchronize(state().stream));
+    if (mv->data) HIP_CHECK(hipFree(mv->data));
+    free(mv->views);
+    mv->magic = 0;
+    free(mv);
+}
+
+static Vec *create_vec(int n)
+{
+    require_init();
+    Vec *v = (Vec *)calloc(1, sizeof(Vec));
+    v->n = n;
+    v->owner = 1;
+    HIP_CHECK(hipMalloc((void **)&v->data, (size_t)padded_ld(n) * sizeof(double)));
+    HIP_CHECK(hipMemsetAsync(v->data, 0, (size_t)padded_ld(n) * sizeof(double), state().stream));
+    return v;
+}
+
+static void check_range(const MultiVec *mv, int start, int k, const char *who)
+{
+    if (k < 0 || start < 0 || start + k > mv->ncols) {
+        char buf[256];
+        snprintf(buf, sizeof(buf), "%s: columns [%d, %d) outside a multivec of %d columns", who,
+                 start, start + k, mv->ncols);
+        fail_msg(buf);
+    }
+}
+
+static inline double *col_ptr(MultiVec *mv, int c) { return mv->data + (size_t)c * mv->ld; }
+
+// Backend scratch multivecs (residual check, block CG without a workspace)
+struct Scratch {
+    double *data = nullptr;
+    long ld = 0;
+    size_t cap = 0;
+};
+static Scratch g_scratch[4];
+
+static double *scratch(int which, int n, int k, long *ld)
+{
+    Scratch &s = g_scratch[which];
+    const long l = padded_ld(n);
+    const size_t need = (size_t)l * std::max(k, 1);
+    if (need > s.cap || !s.data) {
+        if (s.data) {
+            HIP_CHECK(hipStreamSynchronize(state().stream));
+            HIP_CHECK(hipFree(s.data));
+        }
+        HIP_CHECK(hipMalloc((void **)&s.data, need * sizeof(double)));
+        s.cap = need;
+    }
+    s.ld = l;
+    *ld = l;
+    return s.data;
+}
+
+static double dot_dev(int n, const double *x, const double *y)
+{
+    double *d = small_buffer2(1);
+    coldots(n, x, n, nullptr, y, n, nullptr, 1, d);
+    double *h = result_buffer(1);
+    HIP_CHECK(hipMemcpyAsync(h, d, sizeof(double), hipMemcpyDeviceToHost, state().stream));
+    HIP_CHECK(hipStreamSynchronize(state().stream));
+    return h[0];
+}
+
+// ---- operations table: single vectors ------------------------------------------------------------
+static void op_vec_random(void *vec, GCGE_OPS *)
+{
+    Vec *v = as_vec(vec);
+    State &s = state();
+    random_fill(v->n, v->data, padded_ld(v->n), 0, 1, s.seed, s.draw++);
+}
+
+static void op_mat_dot_vec(void *mat, void *x, void *r, GCGE_OPS *)
+{
+    Mat *A = as_mat(mat);
+    Vec *xv = as_vec(x), *rv = as_vec(r);
+    if (xv->n != A->ncols || rv->n != A->nrows) fail_msg("MatDotVec: matrix and vector sizes differ");
+    spmm(A, xv->data, padded_ld(A->ncols), rv->data, padded_ld(A->nrows), 1);
+}
+
+static void op_vec_axpby(double a, void *x, double b, void *y, GCGE_OPS *)
+{
+    Vec *yv = as_vec(y);
+    axpby(yv->n, a, as_vec(x)->data, yv->n, b, yv->data, yv->n, 1);
+}
+
+static void op_vec_inner(void *x, void *y, double *v, GCGE_OPS *)
+{
+    Vec *xv = as_vec(x);
+    *v = dot_dev(xv->n, xv->data, as_vec(y)->data);
+}
+
+static void op_vec_create_by_vec(void **d, void *s, GCGE_OPS *) { *d = create_vec(as_vec(s)->n); }
+static void op_vec_create_by_mat(void **v, void *mat, GCGE_OPS *) { *v = create_vec(as_mat(mat)->nrows); }
+
+static void op_vec_destroy(void **v, GCGE_OPS *)
+{
+    if (!*v) return;
+    Vec *vv = (Vec *)*v;
+    if (vv->owner) {
+        HIP_CHECK(hipStreamSynchronize(state().stream));
+        HIP_CHECK(hipFree(vv->data));
+        free(vv);
+    }
+    *v = nullptr;
+}
+
+static void op_get(void **V, GCGE_INT j, void **x, GCGE_OPS *)
+{
+    MultiVec *mv = as_mv(V);
+    check_range(mv, j, 1, "GetVecFromMultiVec");
+    *x = &mv->views[j];
+}
+
+static void op_restore(void **, GCGE_INT, void **, GCGE_OPS *) {}
+
+// ---- operations table: multivecs ----------------------------------------------------------------------
+static void op_mv_create_by_mat(void ***mv, GCGE_INT n, void *mat, GCGE_OPS *)
+{
+    *mv = (void **)create_mv(as_mat(mat)->nrows, n);
+}
+
+static void op_mv_create_by_vec(void ***mv, GCGE_INT n, void *vec, GCGE_OPS *)
+{
+    *mv = (void **)create_mv(as_vec(vec)->n, n);
+}
+
+static void op_mv_create_by_mv(void ***mv, GCGE_INT n, void **init, GCGE_OPS *)
+{
+    *mv = (void **)create_mv(as_mv(init)->n, n);
+}
+
+static void op_mv_destroy(void ***mv, GCGE_INT, GCGE_OPS *)
+{
+    if (!*mv) return;
+    destroy_mv(as_mv(*mv));
+    *mv = nullptr;
+}
+
+static void op_mv_random(void **mv, GCGE_INT start, GCGE_INT k, GCGE_OPS *)
+{
+    MultiVec *m = as_mv(mv);
+    check_range(m, start, k, "MultiVecSetRandomValue");
+    State &s = state();
+    random_fill(m->n, col_ptr(m, start), m->ld, start, k, s.seed, s.draw++);
+}
+
+static void op_mat_dot_mv(void *mat, void **x, void **y, GCGE_INT *start, GCGE_INT *end, GCGE_OPS *)
+{
+    Mat *A = as_mat(mat);
+    MultiVec *X = as_mv(x), *Y = as_mv(y);
+    const int k = end[0] - start[0];
+    check_range(X, start[0], k, "MatDotMultiVec");
+    check_range(Y, start[1], k, "MatDotMultiVec");
+    if (X->n != A->ncols || Y->n != A->nrows) fail_msg("MatDotMultiVec: matrix and multivec sizes differ");
+    spmm(A, col_ptr(X, start[0]), X->ld, col_ptr(Y, start[1]), Y->ld, k);
+}
+
+static void op_mv_axpby(double a, void **x, double b, void **y, GCGE_INT *start, GCGE_INT *end,
+                        GCGE_OPS *)
+{
+    MultiVec *X = as_mv(x), *Y = as_mv(y);
+    const int k = end[0] - start[0];
+    check_range(X, start[0], k, "MultiVecAxpby");
+    check_range(Y, start[1], k, "MultiVecAxpby");
+    axpby(Y->n, a, col_ptr(X, start[0]), X->ld, b, col_ptr(Y, start[1]), Y->ld, k);
+}
+
+static void op_mv_axpby_column(double a, void **x, GCGE_INT cx, double b, void **y, GCGE_INT cy,
+                               GCGE_OPS *)
+{
+    MultiVec *X = as_mv(x), *Y = as_mv(y);
+    check_range(X, cx, 1, "MultiVecAxpbyColumn");
+    check_range(Y, cy, 1, "MultiVecAxpbyColumn");
+    axpby(Y->n, a, col_ptr(X, cx), X->ld, b, col_ptr(Y, cy), Y->ld, 1);
+}
+
+static void op_mv_lincomb(void **x, void **y, GCGE_INT *start, GCGE_INT *end, double *a,
+                          GCGE_INT lda, GCGE_INT if_Vec, double alpha, double beta, GCGE_OPS *)
+{
+    (void)alpha;
+    (void)if_Vec;
+    MultiVec *X = as_mv(x), *Y = as_mv(y);
+    const int m = end[0] - start[0], k = end[1] - start[1];
+    if (k <= 0) return;
+    check_range(X, start[0], std::max(m, 0), "MultiVecLinearComb");
+    check_range(Y, start[1], k, "MultiVecLinearComb");
+    if (m <= 0) fail_msg("MultiVecLinearComb: empty input range");
+    std::vector<double> packed((size_t)m * k);
+    for (int j = 0; j < k; ++j) memcpy(&packed[(size_t)j * m], a + (size_t)j * lda, (size_t)m * sizeof(double));
+    double *d_C = small_buffer(packed.size());
+    upload_async(d_C, packed.data(), packed.size() * sizeof(double));
+    lincomb_dev(Y->n, col_ptr(X, start[0]), X->ld, m, d_C, beta, col_ptr(Y, start[1]), Y->ld, k);
+}
+
+static void op_mv_inner(void **V, void **W, double *a, char *is_sym, GCGE_INT *start,
+                        GCGE_INT *end, GCGE_INT lda, GCGE_INT if_Vec, GCGE_OPS *)
+{
+    (void)if_Vec;
+    MultiVec *Vm = as_mv(V), *Wm = as_mv(W);
+    const bool sym = strcmp(is_sym, "sym") == 0;
+    const int k = end[1] - start[1];
+    const int m = sym ? k : end[0] - start[0];
+    if (m <= 0 || k <= 0) return;
+    check_range(Vm, start[0], m, "MultiVecInnerProd");
+    check_range(Wm, start[1], k, "MultiVecInnerProd");
+    std::vector<double> g((size_t)m * k);
+    gram(Vm->n, col_ptr(Vm, start[0]), Vm->ld, m, col_ptr(Wm, start[1]), Wm->ld, k, g.data(), m);
+    if (!sym) {
+        for (int j = 0; j < k; ++j) memcpy(a + (size_t)j * lda, &g[(size_t)j * m], (size_t)m * sizeof(double));
+    } else {
+        // upper triangle (i <= j) computed, lower mirrored from it
+        for (int j = 0; j < k; ++j)
+            for (int i = 0; i <= j; ++i) a[(size_t)j * lda + i] = g[(size_t)j * m + i];
+        for (int j = 0; j < k; ++j)
+            for (int i = j + 1; i < k; ++i) a[(size_t)j * lda + i] = a[(size_t)i * lda + j];
+    }
+}
+
+static void op_mv_swap(void **V1, void **V2, GCGE_INT *start, GCGE_INT *end, GCGE_OPS *)
+{
+    MultiVec *A = as_mv(V1), *B = as_mv(V2);
+    const int k = end[0] - start[0];
+    check_range(A, start[0], k, "MultiVecSwap");
+    check_range(B, start[1], k, "MultiVecSwap");
+    if (A == B && start[0] == start[1]) return;
+    if (A == B && std::abs(start[0] - start[1]) < k) fail_msg("MultiVecSwap: overlapping ranges");
+    swap_cols(A->n, col_ptr(A, start[0]), A->ld, col_ptr(B, start[1]), B->ld, k);
+}
+
+static void op_mv_print(void **x, GCGE_INT n, GCGE_OPS *)
+{
+    MultiVec *X = as_mv(x);
+    std::vector<double> h((size_t)X->n);
+    for (int c = 0; c < std::min(n, X->ncols); ++c) {
+        HIP_CHECK(hipMemcpy(h.data(), col_ptr(X, c), (size_t)X->n * sizeof(double), hipMemcpyDeviceToHost));
+        for (int i = 0; i < X->n; ++i) printf("%0.4f\t", h[i]);
+        printf("\n");
+    }
+}
+
+__global__ void k_zero_rows(int nrows_list, const int *rows, double *X, long ldx)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < nrows_list) X[(long)blockIdx.y * ldx + rows[e]] = 0.0;
+}
+
+// Rows where B has no entry above 1e-20 and A exactly one (reference
+// app/csr/gcge_app_csr.c:130-183) are zeroed in the first nev columns.
+static void op_dirichlet(void **vecs, GCGE_INT nev, void *A, void *B)
+{
+    if (!A || !B) return;
+    MatHost *ha = mat_host(as_mat(A)), *hb = mat_host(as_mat(B));
+    if (!ha || !hb) return;
+    std::vector<int> rows;
+    for (size_t i = 0; i < ha->nz_count.size(); ++i)
+        if (hb->nz_count[i] == 0 && ha->nz_count[i] == 1) rows.push_back((int)i);
+    if (rows.empty() || nev <= 0) return;
+    MultiVec *X = as_mv(vecs);
+    int *d_rows = index_buffer(rows.size());
+    upload_async(d_rows, rows.data(), rows.size() * sizeof(int));
+    hipLaunchKernelGGL(k_zero_rows, dim3(((int)rows.size() + 255) / 256, nev), dim3(256), 0,
+                       state().stream, (int)rows.size(), d_rows, X->data, X->ld);
+    HIP_CHECK(hipGetLastError());
+}
+
+// ---- block CG (reference src/gcge_cg.c:302-501 arithmetic) --------------------------------------------
+struct CGParams {
+    int max_it = 20;
+    double rate = 1e-2;
+    int if_shift = 0;
+    double shift = 0.0;
+    void *B = nullptr;
+};
+static CGParams g_cg;
+
+static void block_cg(void *Amat, int if_shift, double shift, void *Bmat, void **RHS, void **V,
+                     int *start, int *end, int max_it, double rate, void **CG_P, void **CG_W,
+                     double *dtmp, int *itmp)
+{
+    Mat *A = as_mat(Amat);
+    Mat *B = Bmat ? as_mat(Bmat) : nullptr;
+    MultiVec *R = as_mv(RHS), *X = as_mv(V);
+    const int n = X->n;
+    const int len = end[1] - start[1];
+    if (len <= 0) return;
+    check_range(R, start[0], len, "BlockCG rhs");
+    check_range(X, start[1], len, "BlockCG solution");
+    // directions and A p: caller's blocks when given, else backend scratch
+    double *Pd, *Wd;
+    long ldp, ldw;
+    int p0 = start[2], w0 = start[3];
+    if (CG_P) {
+        MultiVec *P = as_mv(CG_P);
+        check_range(P, start[2], len, "BlockCG directions");
+        Pd = P->data;
+        ldp = P->ld;
+    } else {
+        Pd = scratch(0, n, len, &ldp);
+        p0 = 0;
+    }
+    if (CG_W) {
+        MultiVec *W = as_mv(CG_W);
+        check_range(W, start[3], len, "BlockCG A*p");
+        Wd = W->data;
+        ldw = W->ld;
+    } else {
+        Wd = scratch(1, n, len, &ldw);
+        w0 = 0;
+    }
+    long ldt = 0;
+    double *T = (if_shift == 1 && B) ? scratch(2, n, len, &ldt) : nullptr;
+    std::vector<double> host_scalars;
+    double *rho1, *rho2, *err, *last_err;
+    if (dtmp) {
+        rho1 = dtmp;
+    } else {
+        host_scalars.resize((size_t)5 * len);
+        rho1 = host_scalars.data();
+    }
+    rho2 = rho1 + len;
+    err = rho2 + len;
+    last_err = err + len;
+    std::vector<int> host_unlock;
+    int *unlock = itmp;
+    if (!unlock) {
+        host_unlock.resize(len);
+        unlock = host_unlock.data();
+    }
+    State &s = state();
+    double *Rd = R->data, *Xd = X->data;
+    const long ldr = R->ld, ldx = X->ld;
+
+    // r = A x (+ shift B x) into W, b -= r, rho2 = b.b   (all columns)
+    spmm(A, Xd + (long)start[1] * ldx, ldx, Wd + (long)w0 * ldw, ldw, len);
+    if (if_shift == 1) {
+        if (!B) {
+            axpby(n, shift, Xd + (long)start[1] * ldx, ldx, 1.0, Wd + (long)w0 * ldw, ldw, len);
+        } else {
+            spmm(B, Xd + (long)start[1] * ldx, ldx, T, ldt, len);
+            axpby(n, shift, T, ldt, 1.0, Wd + (long)w0 * ldw, ldw, len);
+        }
+    }
+    axpby(n, -1.0, Wd + (long)w0 * ldw, ldw, 1.0, Rd + (long)start[0] * ldr, ldr, len);
+    double *d_scal = small_buffer2((size_t)4 * len + 8);
+    coldots(n, Rd + (long)start[0] * ldr, ldr, nullptr, Rd + (long)start[0] * ldr, ldr, nullptr, len, d_scal);
+    double *h = result_buffer((size_t)len);
+    HIP_CHECK(hipMemcpyAsync(h, d_scal, len * sizeof(double), hipMemcpyDeviceToHost, s.stream));
+    HIP_CHECK(hipStreamSynchronize(s.stream));
+    int num_unlock = 0;
+    for (int i = 0; i < len; ++i) {
+        rho2[i] = h[i];
+        err[i] = std::sqrt(rho2[i]);
+        last_err[i] = err[i];
+        if (err[i] >= 1e-15) unlock[num_unlock++] = i;
+    }
+    // device scratch: maps (4 x len ints), beta, rho2 at position, ptw, new r.r
+    int *d_maps = index_buffer((size_t)4 * len);
+    double *d_beta = d_scal, *d_rho = d_scal + len, *d_ptw = d_scal + 2 * len, *d_rr = d_scal + 3 * len;
+    std::vector<int> maps((size_t)4 * len);
+    std::vector<double> coef((size_t)2 * len);
+    int niter = 1;
+    while (num_unlock > 0 && niter < max_it) {
+        const int nu = num_unlock;
+        for (int id = 0; id < nu; ++id) {
+            const int idx = unlock[id];
+            maps[id] = p0 + idx;                // p
+            maps[len + id] = start[0] + idx;    // r
+            maps[2 * len + id] = start[1] + idx;  // x
+            maps[3 * len + id] = w0 + idx;      // w
+            coef[id] = (niter == 1) ? 0.0 : rho2[idx] / rho1[idx];  // beta, indexed by column
+            coef[len + id] = rho2[id];                              // alpha numerator, by position
+        }
+        upload_async(d_maps, maps.data(), maps.size() * sizeof(int));
+        upload_async(d_beta, coef.data(), coef.size() * sizeof(double));
+        const int *pm = d_maps, *rm = d_maps + len, *xm = d_maps + 2 * len, *wm = d_maps + 3 * len;
+        bcg_p_update(n, Pd, ldp, pm, Rd, ldr, rm, d_beta, niter == 1, nu);
+        spmm(A, Pd, ldp, Wd, ldw, nu, pm, wm);
+        if (if_shift == 1) {
+            std::vector<double> sh(nu, shift);
+            double *d_sh = small_buffer((size_t)nu);
+            upload_async(d_sh, sh.data(), nu * sizeof(double));
+            if (!B) {
+                axpy_cols(n, d_sh, Pd, ldp, pm, Wd, ldw, wm, nu);
+            } else {
+                spmm(B, Pd, ldp, T, ldt, nu, pm, nullptr);
+                axpy_cols(n, d_sh, T, ldt, nullptr, Wd, ldw, wm, nu);
+            }
+        }
+        coldots(n, Pd, ldp, pm, Wd, ldw, wm, nu, d_ptw);
+        bcg_xr_update(n, Xd, ldx, xm, Rd, ldr, rm, Pd, ldp, pm, Wd, ldw, wm, d_rho, d_ptw, nu, d_rr);
+        HIP_CHECK(hipMemcpyAsync(h, d_rr, nu * sizeof(double), hipMemcpyDeviceToHost, s.stream));
+        HIP_CHECK(hipStreamSynchronize(s.stream));
+        for (int id = 0; id < nu; ++id) {
+            rho1[id] = rho2[id];
+            rho2[id] = h[id];
+        }
+        for (int id = 0; id < nu; ++id) last_err[unlock[id]] = std::sqrt(std::fabs(rho2[id]));
+        num_unlock = 0;
+        for (int id = 0; id < nu; ++id) {
+            const int idx = unlock[id];
+            if (last_err[idx] >= rate * err[idx] && last_err[idx] >= 1e-15) unlock[num_unlock++] = idx;
+        }
+        niter++;
+    }
+}
+
+static void ext_block_cg(void *A, GCGE_INT if_shift, GCGE_DOUBLE shift, void *B, void **RHS,
+                         void **V, GCGE_INT *start, GCGE_INT *end, GCGE_INT max_it,
+                         GCGE_DOUBLE rate, GCGE_OPS *, void **CG_P, void **CG_W,
+                         GCGE_DOUBLE *dtmp, GCGE_INT *itmp)
+{
+    block_cg(A, if_shift, shift, B, RHS, V, start, end, max_it, rate, CG_P, CG_W, dtmp, itmp);
+}
+
+static void op_multi_linear_solver(void *A, void **b, void **x, int *start, int *end, GCGE_OPS *)
+{
+    block_cg(A, g_cg.if_shift, g_cg.shift, g_cg.B, b, x, start, end, g_cg.max_it, g_cg.rate,
+             nullptr, nullptr, nullptr, nullptr);
+}
+
+// ---- residual norms of a column block --------------------------------------------------------------------
+static void ext_residual_norms(void *Amat, void *Bmat, void **Xh, GCGE_INT start, GCGE_INT end,
+                               const GCGE_DOUBLE *eval, GCGE_DOUBLE *rnorm, GCGE_DOUBLE *xnorm,
+                               GCGE_DOUBLE *bxnorm, GCGE_OPS *)
+{
+    Mat *A = as_mat(Amat);
+    MultiVec *X = as_mv(Xh);
+    const int k = end - start;
+    if (k <= 0) return;
+    check_range(X, start, k, "MultiVecResidualNorms");
+    const int n = X->n;
+    long ldax, ldbx;
+    double *AX = scratch(0, n, k, &ldax);
+    const double *Xc = col_ptr(X, start);
+    spmm(A, Xc, X->ld, AX, ldax, k);
+    const double *BX = Xc;
+    ldbx = X->ld;
+    if (Bmat) {
+        double *t = scratch(1, n, k, &ldbx);
+        spmm(as_mat(Bmat), Xc, X->ld, t, ldbx, k);
+        BX = t;
+    }
+    double *d = small_buffer2((size_t)4 * k);
+    upload_async(d, eval, (size_t)k * sizeof(double));
+    residual_sq(n, AX, ldax, BX, ldbx, Xc, X->ld, d, k, d + k);
+    double *h = result_buffer((size_t)3 * k);
+    HIP_CHECK(hipMemcpyAsync(h, d + k, 3 * k * sizeof(double), hipMemcpyDeviceToHost, state().stream));
+    HIP_CHECK(hipStreamSynchronize(state().stream));
+    for (int c = 0; c < k; ++c) {
+        rnorm[c] = std::sqrt(h[c]);
+        xnorm[c] = std::sqrt(h[k + c]);
+        if (bxnorm) bxnorm[c] = std::sqrt(h[2 * k + c]);
+    }
+}
+
+static void ext_sync(GCGE_OPS *) { HIP_CHECK(hipStreamSynchronize(state().stream)); }
+
+static GCGE_OPS_EXT g_ext = {GCGE_OPS_EXT_MAGIC, ext_residual_norms, ext_block_cg, ext_sync, nullptr};
+
+// ---- host copies ----------------------------------------------------------------------------------------------
+static void mv_set_host(void **mv, int start, int ncols, const double *host, int ld)
+{
+    MultiVec *X = as_mv(mv);
+    check_range(X, start, ncols, "MultiVecSetHost");
+    if (ncols <= 0) return;
+    HIP_CHECK(hipMemcpy2DAsync(col_ptr(X, start), X->ld * sizeof(double), host, (size_t)ld * sizeof(double),
+                               (size_t)X->n * sizeof(double), ncols, hipMemcpyHostToDevice, state().stream));
+    HIP_CHECK(hipStreamSynchronize(state().stream));
+}
+
+static void mv_get_host(void **mv, int start, int ncols, double *host, int ld)
+{
+    MultiVec *X = as_mv(mv);
+    check_range(X, start, ncols, "MultiVecGetHost");
+    if (ncols <= 0) return;
+    HIP_CHECK(hipMemcpy2DAsync(host, (size_t)ld * sizeof(double), col_ptr(X, start), X->ld * sizeof(double),
+                               (size_t)X->n * sizeof(double), ncols, hipMemcpyDeviceToHost, state().stream));
+    HIP_CHECK(hipStreamSynchronize(state().stream));
+}
+
+static void set_ops(GCGE_OPS *ops)
+{
+    require_init();
+    ops->ops = &g_ext;
+    ops->VecSetRandomValue = op_vec_random;
+    ops->MatDotVec = op_mat_dot_vec;
+    ops->VecAxpby = op_vec_axpby;
+    ops->VecInnerProd = op_vec_inner;
+    ops->VecLocalInnerProd = op_vec_inner;
+    ops->VecCreateByVec = op_vec_create_by_vec;
+    ops->VecCreateByMat = op_vec_create_by_mat;
+    ops->VecDestroy = op_vec_destroy;
+    ops->GetVecFromMultiVec = op_get;
+    ops->RestoreVecForMultiVec = op_restore;
+    ops->MultiVecCreateByVec = op_mv_create_by_vec;
+    ops->MultiVecCreateByMat = op_mv_create_by_mat;
+    ops->MultiVecCreateByMultiVec = op_mv_create_by_mv;
+    ops->MultiVecDestroy = op_mv_destroy;
+    ops->MultiVecSetRandomValue = op_mv_random;
+    ops->MatDotMultiVec = op_mat_dot_mv;
+    ops->MultiVecAxpby = op_mv_axpby;
+    ops->MultiVecAxpbyColumn = op_mv_axpby_column;
+    ops->MultiVecLinearComb = op_mv_lincomb;
+    ops->MultiVecInnerProd = op_mv_inner;
+    ops->MultiVecInnerProdLocal = op_mv_inner;
+    ops->MultiVecSwap = op_mv_swap;
+    ops->MultiVecPrint = op_mv_print;
+    ops->SetDirichletBoundary = op_dirichlet;
+    ops->MultiLinearSolver = op_multi_linear_solver;
+}
+
+struct BackendDesc {
+    void (*SetOps)(GCGE_OPS *);
+    void (*SetHost)(void **, GCGE_INT, GCGE_INT, const GCGE_DOUBLE *, GCGE_INT);
+    void (*GetHost)(void **, GCGE_INT, GCGE_INT, GCGE_DOUBLE *, GCGE_INT);
+};
+
+static void set_ops_core(GCGE_OPS *ops)
+{
+    // For the in-tree core: the block CG extension is used directly, so the
+    // table leaves MultiLinearSolver empty and the core passes its own
+    // workspace and parameters.
+    set_ops(ops);
+    ops->MultiLinearSolver = nullptr;
+}
+
+static const BackendDesc g_backend = {set_ops_core, mv_set_host, mv_get_host};
+
+}  // namespace gcge_hip
+
+// ==== C ABI ===========================================================================================================
+using namespace gcge_hip;
+
+extern "C" {
+
+int GCGE_HIP_Init(int device)
+{
+    State &s = state();
+    if (s.stream) return 0;
+    s.device = device;
+    return ensure_init();
+}
+
+void GCGE_HIP_Finalize(void)
+{
+    State &s = state();
+    if (!s.stream) return;
+    HIP_CHECK(hipStreamSynchronize(s.stream));
+    for (int i = 0; i < 4; ++i)
+        if (g_scratch[i].data) {
+            HIP_CHECK(hipFree(g_scratch[i].data));
+            g_scratch[i] = Scratch();
+        }
+    if (s.d_partial) HIP_CHECK(hipFree(s.d_partial));
+    if (s.d_small) HIP_CHECK(hipFree(s.d_small));
+    if (s.d_small2) HIP_CHECK(hipFree(s.d_small2));
+    if (s.d_index) HIP_CHECK(hipFree(s.d_index));
+    if (s.h_result) HIP_CHECK(hipHostFree(s.h_result));
+    for (int i = 0; i < State::kRing; ++i) {
+        if (s.h_ring[i]) HIP_CHECK(hipHostFree(s.h_ring[i]));
+        HIP_CHECK(hipEventDestroy(s.ring_event[i]));
+    }
+    for (auto &r : g_prof) {
+        HIP_CHECK(hipEventDestroy(r.start));
+        HIP_CHECK(hipEventDestroy(r.stop));
+    }
+    g_prof.clear();
+    for (hipEvent_t e : g_event_pool) HIP_CHECK(hipEventDestroy(e));
+    g_event_pool.clear();
+    HIP_CHECK(hipStreamDestroy(s.stream));
+    s = State();
+}
+
+const char *GCGE_HIP_LastError(void) { return g_last_error.c_str(); }
+
+void GCGE_HIP_Synchronize(void)
+{
+    if (state().stream) HIP_CHECK(hipStreamSynchronize(state().stream));
+}
+
+void GCGE_HIP_SetSeed(uint64_t seed)
+{
+    state().seed = seed;
+    state().draw = 0;
+}
+
+void *GCGE_HIP_MatCreateCSR(int nrows, int ncols, int nnz, const int *rowptr, const int *colidx,
+                            const double *vals)
+{
+    return create_mat(nrows, ncols, nnz, rowptr, colidx, vals);
+}
+
+void GCGE_HIP_MatDestroy(void *mat)
+{
+    if (!mat) return;
+    Mat *m = as_mat(mat);
+    HIP_CHECK(hipStreamSynchronize(state().stream));
+    HIP_CHECK(hipFree(m->d_slice_ptr));
+    HIP_CHECK(hipFree(m->d_slice_w));
+    HIP_CHECK(hipFree(m->d_col));
+    HIP_CHECK(hipFree(m->d_val));
+    if (m->d_val32) HIP_CHECK(hipFree(m->d_val32));
+    for (size_t i = 0; i < g_mat_host.size(); ++i)
+        if (g_mat_host[i].first == m) {
+            delete g_mat_host[i].second;
+            g_mat_host.erase(g_mat_host.begin() + (long)i);
+            break;
+        }
+    m->magic = 0;
+    free(m);
+}
+
+void GCGE_HIP_MatSetPrecision(void *mat, int fp32) { as_mat(mat)->fp32 = fp32 ? 1 : 0; }
+
+void **GCGE_HIP_MultiVecCreate(int n, int ncols) { return (void **)create_mv(n, ncols); }
+void GCGE_HIP_MultiVecDestroy(void **mv)
+{
+    if (mv) destroy_mv(as_mv(mv));
+}
+void GCGE_HIP_MultiVecSetHost(void **mv, int start, int ncols, const double *host, int ld)
+{
+    mv_set_host(mv, start, ncols, host, ld);
+}
+void GCGE_HIP_MultiVecGetHost(void **mv, int start, int ncols, double *host, int ld)
+{
+    mv_get_host(mv, start, ncols, host, ld);
+}
+
+void GCGE_HIP_SpMM(void *mat, void **X, int xs, void **Y, int ys, int k)
+{
+    GCGE_INT s[2] = {xs, ys}, e[2] = {xs + k, ys + k};
+    op_mat_dot_mv(mat, X, Y, s, e, nullptr);
+}
+
+void GCGE_HIP_Gram(void **V, int vs, int m, void **W, int ws, int k, double *G, int ldg)
+{
+    GCGE_INT s[2] = {vs, ws}, e[2] = {vs + m, ws + k};
+    char ns[] = "ns";
+    op_mv_inner(V, W, G, ns, s, e, ldg, 0, nullptr);
+}
+
+void GCGE_HIP_LinearComb(void **V, int vs, int m, const double *C, int ldc, double beta, void **Y,
+                         int ys, int k)
+{
+    GCGE_INT s[2] = {vs, ys}, e[2] = {vs + m, ys + k};
+    op_mv_lincomb(V, Y, s, e, (double *)C, ldc, 0, 1.0, beta, nullptr);
+}
+
+void GCGE_HIP_Axpby(double a, void **X, int xs, double b, void **Y, int ys, int k)
+{
+    GCGE_INT s[2] = {xs, ys}, e[2] = {xs + k, ys + k};
+    op_mv_axpby(a, X, b, Y, s, e, nullptr);
+}
+
+void GCGE_HIP_RandomFill(void **X, int start, int k) { op_mv_random(X, start, k, nullptr); }
+
+void GCGE_HIP_SetOps(GCGE_OPS *ops) { set_ops(ops); }
+
+void GCGE_HIP_SetLinearSolverParams(int max_it, double rate, int if_shift, double shift, void *B)
+{
+    g_cg.max_it = max_it;
+    g_cg.rate = rate;
+    g_cg.if_shift = if_shift;
+    g_cg.shift = shift;
+    g_cg.B = B;
+}
+
+const void *GCGE_HIP_Backend(void) { return &g_backend; }
+
+void GCGE_HIP_ProfileEnable(int on) { state().profiling = on != 0; }
+
+void GCGE_HIP_ProfileReset(void)
+{
+    if (state().stream) HIP_CHECK(hipStreamSynchronize(state().stream));
+    for (auto &r : g_prof) {
+        g_event_pool.push_back(r.start);
+        g_event_pool.push_back(r.stop);
+    }
+    g_prof.clear();
+}
+
+int GCGE_HIP_KernelStats(const char *name, double *ms_total, int *count, double *bytes_total)
+{
+    if (state().stream) HIP_CHECK(hipStreamSynchronize(state().stream));
+    double total = 0.0, bytes = 0.0;
+    int c = 0;
+    for (auto &r : g_prof) {
+        if (strcmp(r.name, name) != 0) continue;
+        float ms = 0.0f;
+        HIP_CHECK(hipEventElapsedTime(&ms, r.start, r.stop));
+        total += ms;
+        bytes += r.bytes;
+        ++c;
+    }
+    if (ms_total) *ms_total = total;
+    if (count) *count = c;
+    if (bytes_total) *bytes_total = bytes;
+    return c;
+}
+
+int GCGE_HIP_KernelTime(const char *name, double *ms_total, int *count)
+{
+    return GCGE_HIP_KernelStats(name, ms_total, count, nullptr);
+}
+
+}  // extern "C"

@@ -1,0 +1,133 @@
+// Internal declarations shared by the device kernels and the backend glue.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace gcge_hip {
+
+// ---- error handling --------------------------------------------------------
+void fail(const char *what, hipError_t err, const char *file, int line);
+#define HIP_CHECK(expr)                                                       \
+    do {                                                                      \
+        hipError_t e_ = (expr);                                               \
+        if (e_ != hipSuccess) ::gcge_hip::fail(#expr, e_, __FILE__, __LINE__); \
+    } while (0)
+void fail_msg(const char *msg);
+
+// ---- process-wide state ------------------------------------------------------
+struct State {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    // device scratch, grown on demand
+    double *d_partial = nullptr;  // reduction partials
+    size_t partial_cap = 0;       // doubles
+    double *d_small = nullptr;    // small results / coefficient matrices
+    size_t small_cap = 0;         // doubles
+    double *d_small2 = nullptr;
+    size_t small2_cap = 0;
+    int *d_index = nullptr;       // column index lists
+    size_t index_cap = 0;         // ints
+    // pinned host staging
+    double *h_result = nullptr;
+    size_t result_cap = 0;
+    static const int kRing = 8;
+    double *h_ring[kRing] = {};
+    size_t ring_cap[kRing] = {};
+    hipEvent_t ring_event[kRing] = {};
+    int ring_next = 0;
+    uint64_t seed = 0x5eed;
+    uint64_t draw = 0;
+    bool profiling = false;
+};
+State &state();
+int ensure_init();
+
+double *partial_buffer(size_t doubles);
+double *small_buffer(size_t doubles);
+double *small_buffer2(size_t doubles);
+int *index_buffer(size_t ints);
+double *result_buffer(size_t doubles);
+// copy host data into a device buffer through the pinned ring (async, safe
+// against reuse of the staging slot)
+void upload_async(void *dst, const void *src, size_t bytes);
+
+// ---- profiling -----------------------------------------------------------------
+// Brackets one launch with HIP events on the backend stream while profiling
+// is enabled; `bytes` is the launch's algorithmic HBM traffic.
+struct ProfScope {
+    int slot;
+    explicit ProfScope(const char *name, double bytes = 0.0);
+    ~ProfScope();
+};
+
+// ---- matrices --------------------------------------------------------------------
+struct Mat {
+    uint32_t magic;
+    int nrows, ncols, nnz;
+    int nslices;
+    int *d_slice_ptr;   // nslices + 1
+    int *d_slice_w;     // nslices
+    int *d_col;         // sliced ELLPACK column indices
+    double *d_val;      // float64 values
+    float *d_val32;     // float32 copy (mixed precision), or null
+    size_t sell_len;    // padded entries
+    int fp32;           // use d_val32 in the inner solves
+};
+constexpr uint32_t kMatMagic = 0x4d415431u;  // "MAT1"
+
+// ---- vectors and multivecs ---------------------------------------------------------
+struct MultiVec;
+struct Vec {
+    double *data;
+    int n;
+    int owner;          // 1: data allocated by this vector
+    MultiVec *parent;   // for column views
+    int col;
+};
+struct MultiVec {
+    uint32_t magic;
+    int n, ncols;
+    long ld;
+    double *data;
+    Vec *views;
+};
+constexpr uint32_t kMultiVecMagic = 0x4d564531u;  // "MVE1"
+constexpr int kAlign = 64;
+
+inline long padded_ld(int n) { return ((long)n + kAlign - 1) / kAlign * kAlign; }
+MultiVec *as_mv(void **handle);
+Vec *as_vec(void *handle);
+
+// ---- kernel launchers (stream-ordered, asynchronous) ---------------------------------
+// Y(:, ymap[c]) = A X(:, xmap[c]) for c < k; maps may be null (identity).
+void spmm(const Mat *A, const double *X, long ldx, double *Y, long ldy, int k,
+          const int *d_xmap = nullptr, const int *d_ymap = nullptr);
+// host G(m x k, ldg) = V^T W
+void gram(int n, const double *V, long ldv, int m, const double *W, long ldw, int k, double *G,
+          int ldg, bool sync_result = true);
+// Y = beta Y + V C with C on the device (m x k, ldc = m)
+void lincomb_dev(int n, const double *V, long ldv, int m, const double *d_C, double beta,
+                 double *Y, long ldy, int k);
+void axpby(int n, double a, const double *X, long ldx, double b, double *Y, long ldy, int k);
+void swap_cols(int n, double *X, long ldx, double *Y, long ldy, int k);
+void random_fill(int n, double *X, long ldx, int col0, int k, uint64_t seed, uint64_t draw);
+// per-column dots out[c] = A(:, amap[c]) . B(:, bmap[c]) into device memory d_out
+void coldots(int n, const double *A, long lda, const int *d_amap, const double *B, long ldb,
+             const int *d_bmap, int k, double *d_out);
+// residual norms of columns c < k: |AX - lam BX|^2, |X|^2, |BX|^2 -> d_out[3k]
+void residual_sq(int n, const double *AX, long ldax, const double *BX, long ldbx, const double *X,
+                 long ldx, const double *d_lam, int k, double *d_out);
+// block CG pieces (column lists on the device)
+void bcg_p_update(int n, double *P, long ldp, const int *d_pmap, const double *R, long ldr,
+                  const int *d_rmap, const double *d_beta, int first, int k);
+void bcg_xr_update(int n, double *X, long ldx, const int *d_xmap, double *R, long ldr,
+                   const int *d_rmap, const double *P, long ldp, const int *d_pmap,
+                   const double *W, long ldw, const int *d_wmap, const double *d_rho2,
+                   const double *d_ptw, int k, double *d_rr_out);
+void axpy_cols(int n, const double *d_alpha, const double *X, long ldx, const int *d_xmap,
+               double *Y, long ldy, const int *d_ymap, int k);
+
+}  // namespace gcge_hip

@@ -1,0 +1,594 @@
+// Device kernels of the GCG backend for gfx950 (CDNA4, 64-wide wavefronts).
+//
+//   spmm        sliced-ELLPACK (64-row slices) times a column block; one
+//               wavefront per slice, each lane one row, register blocking
+//               over 4 columns; entries of a slice are stored column-major so
+//               that every entry load is one 512-byte coalesced access, and
+//               workgroups are remapped so each XCD sweeps a contiguous band
+//               of rows (its L2 keeps the band of X that the stencil reuses).
+//               HBM-bound: nnz*(8+4) + n*k*8*2 bytes per launch.
+//   gram        V^T W for tall-skinny column-major blocks: 64x64 (or 16-wide)
+//               output tiles, 32-row steps staged through LDS (transposed),
+//               4x4 register blocks per thread, split over row chunks with a
+//               deterministic second-pass reduction.
+//   lincomb     Y = beta Y + V C: one lane per row, coefficient chunks in LDS
+//               (broadcast reads), 8 output columns per pass.
+//   axpby/swap/random/coldots/residual/bcg_*: streaming kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+
+#include "gcge_hip_internal.h"
+
+namespace gcge_hip {
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+constexpr int kXcds = 8;
+
+// Workgroup b is dispatched to XCD b % 8.  Renumber so that the workgroups of
+// one XCD cover one contiguous range of work items.
+__device__ __forceinline__ int xcd_remap(int bid, int nb)
+{
+    const int q = nb / kXcds, r = nb % kXcds;
+    const int xcd = bid % kXcds, loc = bid / kXcds;
+    return (xcd < r) ? xcd * (q + 1) + loc : r * (q + 1) + (xcd - r) * q + loc;
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
+    return v;
+}
+
+// sum over a 256-thread block; result valid in thread 0
+__device__ __forceinline__ double block_sum(double v, double *lds)
+{
+    v = wave_sum(v);
+    const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+    if (lane == 0) lds[wid] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < kBlock / kWave; ++w) s += lds[w];
+    __syncthreads();
+    return s;
+}
+
+// ---- SpMM ------------------------------------------------------------------------
+// One chunk of CK consecutive output columns for the lane's row; the slice's
+// entries are re-read per chunk (from L1/L2 after the first chunk).
+template <int CK, typename VT>
+__device__ __forceinline__ void sell_chunk(int row, int n, int w, const int *__restrict__ cp,
+                                           const VT *__restrict__ vp,
+                                           const double *__restrict__ X, long ldx,
+                                           double *__restrict__ Y, long ldy, int c0,
+                                           const int *__restrict__ xmap,
+                                           const int *__restrict__ ymap)
+{
+    const double *xc[CK];
+#pragma unroll
+    for (int cc = 0; cc < CK; ++cc) xc[cc] = X + (long)(xmap ? xmap[c0 + cc] : c0 + cc) * ldx;
+    double acc[CK];
+#pragma unroll
+    for (int cc = 0; cc < CK; ++cc) acc[cc] = 0.0;
+    for (int j = 0; j < w; ++j) {
+        const int col = cp[(long)j * kWave];
+        const double v = (double)vp[(long)j * kWave];
+#pragma unroll
+        for (int cc = 0; cc < CK; ++cc) acc[cc] += v * xc[cc][col];
+    }
+    if (row < n) {
+#pragma unroll
+        for (int cc = 0; cc < CK; ++cc) Y[(long)(ymap ? ymap[c0 + cc] : c0 + cc) * ldy + row] = acc[cc];
+    }
+}
+
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_sell_spmm(int n, int nslices,
+                                                     const int *__restrict__ sptr,
+                                                     const int *__restrict__ sw,
+                                                     const int *__restrict__ scol,
+                                                     const VT *__restrict__ sval,
+                                                     const double *__restrict__ X, long ldx,
+                                                     double *__restrict__ Y, long ldy, int k,
+                                                     const int *__restrict__ xmap,
+                                                     const int *__restrict__ ymap)
+{
+    const int blk = xcd_remap(blockIdx.x, gridDim.x);
+    const int slice = blk * (kBlock / kWave) + threadIdx.x / kWave;
+    if (slice >= nslices) return;
+    const int lane = threadIdx.x % kWave;
+    const int row = slice * kWave + lane;
+    const int off = sptr[slice];
+    const int w = sw[slice];
+    const int *cp = scol + off + lane;
+    const VT *vp = sval + off + lane;
+    int c0 = 0;
+    for (; c0 + 8 <= k; c0 += 8) sell_chunk<8>(row, n, w, cp, vp, X, ldx, Y, ldy, c0, xmap, ymap);
+    if (c0 + 4 <= k) {
+        sell_chunk<4>(row, n, w, cp, vp, X, ldx, Y, ldy, c0, xmap, ymap);
+        c0 += 4;
+    }
+    if (c0 + 2 <= k) {
+        sell_chunk<2>(row, n, w, cp, vp, X, ldx, Y, ldy, c0, xmap, ymap);
+        c0 += 2;
+    }
+    if (c0 < k) sell_chunk<1>(row, n, w, cp, vp, X, ldx, Y, ldy, c0, xmap, ymap);
+}
+
+// ---- Gram -----------------------------------------------------------------------
+constexpr int kGramRows = 32;
+
+template <int TM, int TN>
+__global__ __launch_bounds__(kBlock) void k_gram(int n, int rows_per_block,
+                                                const double *__restrict__ V, long ldv, int m,
+                                                const double *__restrict__ W, long ldw, int k,
+                                                double *__restrict__ partial)
+{
+    constexpr int RM = TM / 16, RN = TN / 16;
+    __shared__ double Vs[kGramRows][TM + 1];
+    __shared__ double Ws[kGramRows][TN + 1];
+    const int bx = blockIdx.x;
+    const int i0 = blockIdx.y * TM, j0 = blockIdx.z * TN;
+    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+    const int r_begin = bx * rows_per_block;
+    const int r_end = min(n, r_begin + rows_per_block);
+    double acc[RM][RN];
+#pragma unroll
+    for (int a = 0; a < RM; ++a)
+#pragma unroll
+        for (int b = 0; b < RN; ++b) acc[a][b] = 0.0;
+
+    for (int r0 = r_begin; r0 < r_end; r0 += kGramRows) {
+        // stage V(r0:r0+32, i0:i0+TM) and W(...) transposed into LDS; lanes run
+        // along rows so each column segment is one coalesced read
+        for (int e = threadIdx.x; e < kGramRows * TM; e += kBlock) {
+            const int r = e % kGramRows, c = e / kGramRows;
+            const int gr = r0 + r, gc = i0 + c;
+            Vs[r][c] = (gr < r_end && gc < m) ? V[(long)gc * ldv + gr] : 0.0;
+        }
+        for (int e = threadIdx.x; e < kGramRows * TN; e += kBlock) {
+            const int r = e % kGramRows, c = e / kGramRows;
+            const int gr = r0 + r, gc = j0 + c;
+            Ws[r][c] = (gr < r_end && gc < k) ? W[(long)gc * ldw + gr] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int r = 0; r < kGramRows; ++r) {
+            double va[RM], wb[RN];
+#pragma unroll
+            for (int a = 0; a < RM; ++a) va[a] = Vs[r][tx + 16 * a];
+#pragma unroll
+            for (int b = 0; b < RN; ++b) wb[b] = Ws[r][ty + 16 * b];
+#pragma unroll
+            for (int a = 0; a < RM; ++a)
+#pragma unroll
+                for (int b = 0; b < RN; ++b) acc[a][b] += va[a] * wb[b];
+        }
+        __syncthreads();
+    }
+    double *out = partial + (long)bx * m * k;
+#pragma unroll
+    for (int a = 0; a < RM; ++a)
+#pragma unroll
+        for (int b = 0; b < RN; ++b) {
+            const int i = i0 + tx + 16 * a, j = j0 + ty + 16 * b;
+            if (i < m && j < k) out[(long)j * m + i] = acc[a][b];
+        }
+}
+
+// out[e] = sum_b partial[b * stride + e], e < count
+__global__ __launch_bounds__(kBlock) void k_reduce_partials(const double *__restrict__ partial,
+                                                            long stride, int nparts, long count,
+                                                            double *__restrict__ out)
+{
+    const long e = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= count) return;
+    double s = 0.0;
+    for (int b = 0; b < nparts; ++b) s += partial[(long)b * stride + e];
+    out[e] = s;
+}
+
+// ---- linear combination --------------------------------------------------------------
+constexpr int kLcCols = 8;
+
+__global__ __launch_bounds__(kBlock) void k_lincomb(int n, const double *__restrict__ V, long ldv,
+                                                   int m, const double *__restrict__ C,
+                                                   double beta, double *__restrict__ Y, long ldy,
+                                                   int k)
+{
+    extern __shared__ double Cs[];  // m x kLcCols
+    const int row = blockIdx.x * kBlock + threadIdx.x;
+    for (int c0 = 0; c0 < k; c0 += kLcCols) {
+        const int nc = min(kLcCols, k - c0);
+        for (int e = threadIdx.x; e < m * kLcCols; e += kBlock) {
+            const int i = e / kLcCols, cc = e % kLcCols;
+            Cs[e] = (cc < nc) ? C[(long)(c0 + cc) * m + i] : 0.0;
+        }
+        __syncthreads();
+        if (row < n) {
+            double acc[kLcCols];
+#pragma unroll
+            for (int cc = 0; cc < kLcCols; ++cc) acc[cc] = 0.0;
+            for (int i = 0; i < m; ++i) {
+                const double v = V[(long)i * ldv + row];
+#pragma unroll
+                for (int cc = 0; cc < kLcCols; ++cc) acc[cc] += v * Cs[i * kLcCols + cc];
+            }
+            for (int cc = 0; cc < nc; ++cc) {
+                double *y = Y + (long)(c0 + cc) * ldy + row;
+                *y = (beta == 0.0) ? acc[cc] : beta * (*y) + acc[cc];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---- streaming kernels ------------------------------------------------------------------
+// y = a x + b y with the reference special cases (a == 0: x not read;
+// b == 0: y not read)
+__global__ __launch_bounds__(kBlock) void k_axpby(int n, double a, const double *__restrict__ X,
+                                                 long ldx, double b, double *__restrict__ Y,
+                                                 long ldy)
+{
+    const int row = blockIdx.x * kBlock + threadIdx.x;
+    const int c = blockIdx.y;
+    if (row >= n) return;
+    double *y = Y + (long)c * ldy + row;
+    if (a == 0.0) {
+        *y = b * (*y);
+    } else {
+        const double x = X[(long)c * ldx + row];
+        *y = (b == 0.0) ? a * x : a * x + b * (*y);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_swap(int n, double *__restrict__ X, long ldx,
+                                                double *__restrict__ Y, long ldy)
+{
+    const int row = blockIdx.x * kBlock + threadIdx.x;
+    const int c = blockIdx.y;
+    if (row >= n) return;
+    double *x = X + (long)c * ldx + row;
+    double *y = Y + (long)c * ldy + row;
+    const double t = *x;
+    *x = *y;
+    *y = t;
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(kBlock) void k_random(int n, double *__restrict__ X, long ldx,
+                                                  int col0, uint64_t seed, uint64_t draw)
+{
+    const int row = blockIdx.x * kBlock + threadIdx.x;
+    const int c = blockIdx.y;
+    if (row >= n) return;
+    const uint64_t key = mix64(seed ^ mix64((draw << 32) ^ (uint64_t)(col0 + c)));
+    const uint64_t bits = mix64(key + (uint64_t)row);
+    X[(long)c * ldx + row] = (double)(bits >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+
+// per-column dot products, partials[c * nbx + bx]
+__global__ __launch_bounds__(kBlock) void k_coldots(int n, int rows_per_block,
+                                                   const double *__restrict__ A, long lda,
+                                                   const int *__restrict__ amap,
+                                                   const double *__restrict__ B, long ldb,
+                                                   const int *__restrict__ bmap,
+                                                   double *__restrict__ partial)
+{
+    __shared__ double lds[kBlock / kWave];
+    const int bx = blockIdx.x, c = blockIdx.y;
+    const double *a = A + (long)(amap ? amap[c] : c) * lda;
+    const double *b = B + (long)(bmap ? bmap[c] : c) * ldb;
+    const int r0 = bx * rows_per_block, r1 = min(n, r0 + rows_per_block);
+    double s = 0.0;
+    for (int r = r0 + threadIdx.x; r < r1; r += kBlock) s += a[r] * b[r];
+    s = block_sum(s, lds);
+    if (threadIdx.x == 0) partial[(long)c * gridDim.x + bx] = s;
+}
+
+// out[c] = sum of the nbx partials of column c
+__global__ __launch_bounds__(kBlock) void k_reduce_rows(const double *__restrict__ partial,
+                                                       int nbx, double *__restrict__ out)
+{
+    __shared__ double lds[kBlock / kWave];
+    const int c = blockIdx.x;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nbx; b += kBlock) s += partial[(long)c * nbx + b];
+    s = block_sum(s, lds);
+    if (threadIdx.x == 0) out[c] = s;
+}
+
+// residual pieces per column: |AX - lam BX|^2, |X|^2, |BX|^2
+__global__ __launch_bounds__(kBlock) void k_residual(int n, int rows_per_block,
+                                                    const double *__restrict__ AX, long ldax,
+                                                    const double *__restrict__ BX, long ldbx,
+                                                    const double *__restrict__ X, long ldx,
+                                                    const double *__restrict__ lam,
+                                                    double *__restrict__ partial)
+{
+    __shared__ double lds[kBlock / kWave];
+    const int bx = blockIdx.x, c = blockIdx.y, nbx = gridDim.x, k = gridDim.y;
+    const double *ax = AX + (long)c * ldax, *bxp = BX + (long)c * ldbx, *x = X + (long)c * ldx;
+    const double l = lam[c];
+    const int r0 = bx * rows_per_block, r1 = min(n, r0 + rows_per_block);
+    double s_r = 0.0, s_x = 0.0, s_b = 0.0;
+    for (int r = r0 + threadIdx.x; r < r1; r += kBlock) {
+        const double bv = bxp[r];
+        const double rv = ax[r] - l * bv;
+        s_r += rv * rv;
+        const double xv = x[r];
+        s_x += xv * xv;
+        s_b += bv * bv;
+    }
+    s_r = block_sum(s_r, lds);
+    s_x = block_sum(s_x, lds);
+    s_b = block_sum(s_b, lds);
+    if (threadIdx.x == 0) {
+        partial[(long)c * nbx + bx] = s_r;
+        partial[(long)(k + c) * nbx + bx] = s_x;
+        partial[(long)(2 * k + c) * nbx + bx] = s_b;
+    }
+}
+
+// p = r (first) or p = r + beta p, column lists
+__global__ __launch_bounds__(kBlock) void k_bcg_p(int n, double *__restrict__ P, long ldp,
+                                                 const int *__restrict__ pmap,
+                                                 const double *__restrict__ R, long ldr,
+                                                 const int *__restrict__ rmap,
+                                                 const double *__restrict__ beta, int first)
+{
+    const int row = blockIdx.x * kBlock + threadIdx.x;
+    const int c = blockIdx.y;
+    if (row >= n) return;
+    double *p = P + (long)pmap[c] * ldp + row;
+    const double r = R[(long)rmap[c] * ldr + row];
+    *p = first ? r : r + beta[c] * (*p);
+}
+
+// x += alpha p; r -= alpha w; partial r.r, with alpha = rho2[c] / ptw[c]
+__global__ __launch_bounds__(kBlock) void k_bcg_xr(int n, int rows_per_block,
+                                                  double *__restrict__ X, long ldx,
+                                                  const int *__restrict__ xmap,
+                                                  double *__restrict__ R, long ldr,
+                                                  const int *__restrict__ rmap,
+                                                  const double *__restrict__ P, long ldp,
+                                                  const int *__restrict__ pmap,
+                                                  const double *__restrict__ W, long ldw,
+                                                  const int *__restrict__ wmap,
+                                                  const double *__restrict__ rho2,
+                                                  const double *__restrict__ ptw,
+                                                  double *__restrict__ partial)
+{
+    __shared__ double lds[kBlock / kWave];
+    const int bx = blockIdx.x, c = blockIdx.y;
+    const double alpha = rho2[c] / ptw[c];
+    double *x = X + (long)xmap[c] * ldx;
+    double *r = R + (long)rmap[c] * ldr;
+    const double *p = P + (long)pmap[c] * ldp;
+    const double *w = W + (long)wmap[c] * ldw;
+    const int r0 = bx * rows_per_block, r1 = min(n, r0 + rows_per_block);
+    double s = 0.0;
+    for (int i = r0 + threadIdx.x; i < r1; i += kBlock) {
+        x[i] = alpha * p[i] + x[i];
+        const double rn = -alpha * w[i] + r[i];
+        r[i] = rn;
+        s += rn * rn;
+    }
+    s = block_sum(s, lds);
+    if (threadIdx.x == 0) partial[(long)c * gridDim.x + bx] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_axpy_cols(int n, const double *__restrict__ alpha,
+                                                     const double *__restrict__ X, long ldx,
+                                                     const int *__restrict__ xmap,
+                                                     double *__restrict__ Y, long ldy,
+                                                     const int *__restrict__ ymap)
+{
+    const int row = blockIdx.x * kBlock + threadIdx.x;
+    const int c = blockIdx.y;
+    if (row >= n) return;
+    Y[(long)(ymap ? ymap[c] : c) * ldy + row] += alpha[c] * X[(long)(xmap ? xmap[c] : c) * ldx + row];
+}
+
+inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
+
+// rows per block for column reductions: enough blocks to fill the chip
+inline int reduction_rows(int n, int k)
+{
+    const long target_blocks = 2048;
+    long per_col = std::max<long>(1, target_blocks / std::max(1, k));
+    long rows = std::max<long>(kBlock * 4, (n + per_col - 1) / per_col);
+    return (int)((rows + kBlock - 1) / kBlock * kBlock);
+}
+
+}  // namespace
+
+// ---- launchers ---------------------------------------------------------------------------
+
+void spmm(const Mat *A, const double *X, long ldx, double *Y, long ldy, int k, const int *d_xmap,
+          const int *d_ymap)
+{
+    if (k <= 0 || A->nrows <= 0) return;
+    // algorithmic bytes: values + column indices + row pointers once, X read
+    // once, Y written once
+    const double bytes = (double)A->nnz * (8 + 4) + 4.0 * (A->nrows + 1) +
+                         8.0 * (double)k * (A->ncols + A->nrows);
+    ProfScope prof("spmm", bytes);
+    const int nb = ceil_div(A->nslices, kBlock / kWave);
+    hipLaunchKernelGGL(k_sell_spmm<double>, dim3(nb), dim3(kBlock), 0, state().stream, A->nrows,
+                       A->nslices, A->d_slice_ptr, A->d_slice_w, A->d_col, A->d_val, X, ldx, Y,
+                       ldy, k, d_xmap, d_ymap);
+    HIP_CHECK(hipGetLastError());
+}
+
+void gram(int n, const double *V, long ldv, int m, const double *W, long ldw, int k, double *G,
+          int ldg, bool sync_result)
+{
+    if (m <= 0 || k <= 0) return;
+    if (n <= 0) {
+        for (int j = 0; j < k; ++j)
+            for (int i = 0; i < m; ++i) G[(long)j * ldg + i] = 0.0;
+        return;
+    }
+    State &s = state();
+    const bool wide_m = m > 16, wide_k = k > 16;
+    const int TM = wide_m ? 64 : 16, TN = wide_k ? 64 : 16;
+    const int tm = ceil_div(m, TM), tn = ceil_div(k, TN);
+    const int tiles = tm * tn;
+    int nbx = std::max(1, std::min(ceil_div(n, kBlock), std::max(1, 2048 / tiles)));
+    int rows = ceil_div(n, nbx);
+    rows = (rows + kGramRows - 1) / kGramRows * kGramRows;
+    nbx = std::max(1, ceil_div(n, rows));
+    const long mk = (long)m * k;
+    double *partial = partial_buffer((size_t)nbx * mk);
+    double *d_out = small_buffer2((size_t)mk);
+    {
+        ProfScope prof("gram", 8.0 * (double)n * (m + k));
+        dim3 grid(nbx, tm, tn);
+        if (wide_m && wide_k)
+            hipLaunchKernelGGL((k_gram<64, 64>), grid, dim3(kBlock), 0, s.stream, n, rows, V, ldv, m, W, ldw, k, partial);
+        else if (wide_m)
+            hipLaunchKernelGGL((k_gram<64, 16>), grid, dim3(kBlock), 0, s.stream, n, rows, V, ldv, m, W, ldw, k, partial);
+        else if (wide_k)
+            hipLaunchKernelGGL((k_gram<16, 64>), grid, dim3(kBlock), 0, s.stream, n, rows, V, ldv, m, W, ldw, k, partial);
+        else
+            hipLaunchKernelGGL((k_gram<16, 16>), grid, dim3(kBlock), 0, s.stream, n, rows, V, ldv, m, W, ldw, k, partial);
+        HIP_CHECK(hipGetLastError());
+    }
+    {
+        ProfScope prof("gram_reduce");
+        hipLaunchKernelGGL(k_reduce_partials, dim3(ceil_div(mk, kBlock)), dim3(kBlock), 0, s.stream,
+                           partial, mk, nbx, mk, d_out);
+        HIP_CHECK(hipGetLastError());
+    }
+    double *h = result_buffer((size_t)mk);
+    HIP_CHECK(hipMemcpyAsync(h, d_out, mk * sizeof(double), hipMemcpyDeviceToHost, s.stream));
+    if (!sync_result) return;
+    HIP_CHECK(hipStreamSynchronize(s.stream));
+    for (int j = 0; j < k; ++j)
+        for (int i = 0; i < m; ++i) G[(long)j * ldg + i] = h[(long)j * m + i];
+}
+
+void lincomb_dev(int n, const double *V, long ldv, int m, const double *d_C, double beta,
+                 double *Y, long ldy, int k)
+{
+    if (k <= 0 || n <= 0) return;
+    if (m <= 0) {  // empty combination: y = beta y
+        axpby(n, 0.0, Y, ldy, beta, Y, ldy, k);
+        return;
+    }
+    if (m > 1024) fail_msg("linear combination with more than 1024 input columns");
+    ProfScope prof("lincomb", 8.0 * (double)n * (m + k) + (beta == 0.0 ? 0.0 : 8.0 * (double)n * k));
+    const size_t lds = (size_t)m * kLcCols * sizeof(double);
+    hipLaunchKernelGGL(k_lincomb, dim3(ceil_div(n, kBlock)), dim3(kBlock), lds, state().stream, n,
+                       V, ldv, m, d_C, beta, Y, ldy, k);
+    HIP_CHECK(hipGetLastError());
+}
+
+void axpby(int n, double a, const double *X, long ldx, double b, double *Y, long ldy, int k)
+{
+    if (k <= 0 || n <= 0) return;
+    ProfScope prof("axpby");
+    hipLaunchKernelGGL(k_axpby, dim3(ceil_div(n, kBlock), k), dim3(kBlock), 0, state().stream, n,
+                       a, X, ldx, b, Y, ldy);
+    HIP_CHECK(hipGetLastError());
+}
+
+void swap_cols(int n, double *X, long ldx, double *Y, long ldy, int k)
+{
+    if (k <= 0 || n <= 0) return;
+    ProfScope prof("swap");
+    hipLaunchKernelGGL(k_swap, dim3(ceil_div(n, kBlock), k), dim3(kBlock), 0, state().stream, n, X,
+                       ldx, Y, ldy);
+    HIP_CHECK(hipGetLastError());
+}
+
+void random_fill(int n, double *X, long ldx, int col0, int k, uint64_t seed, uint64_t draw)
+{
+    if (k <= 0 || n <= 0) return;
+    ProfScope prof("random");
+    hipLaunchKernelGGL(k_random, dim3(ceil_div(n, kBlock), k), dim3(kBlock), 0, state().stream, n,
+                       X, ldx, col0, seed, draw);
+    HIP_CHECK(hipGetLastError());
+}
+
+void coldots(int n, const double *A, long lda, const int *d_amap, const double *B, long ldb,
+             const int *d_bmap, int k, double *d_out)
+{
+    if (k <= 0) return;
+    const int rows = reduction_rows(n, k);
+    const int nbx = std::max(1, ceil_div(n, rows));
+    double *partial = partial_buffer((size_t)nbx * k);
+    ProfScope prof("coldots");
+    hipLaunchKernelGGL(k_coldots, dim3(nbx, k), dim3(kBlock), 0, state().stream, n, rows, A, lda,
+                       d_amap, B, ldb, d_bmap, partial);
+    hipLaunchKernelGGL(k_reduce_rows, dim3(k), dim3(kBlock), 0, state().stream, partial, nbx, d_out);
+    HIP_CHECK(hipGetLastError());
+}
+
+void residual_sq(int n, const double *AX, long ldax, const double *BX, long ldbx, const double *X,
+                 long ldx, const double *d_lam, int k, double *d_out)
+{
+    if (k <= 0) return;
+    const int rows = reduction_rows(n, k);
+    const int nbx = std::max(1, ceil_div(n, rows));
+    double *partial = partial_buffer((size_t)nbx * 3 * k);
+    ProfScope prof("residual");
+    hipLaunchKernelGGL(k_residual, dim3(nbx, k), dim3(kBlock), 0, state().stream, n, rows, AX, ldax,
+                       BX, ldbx, X, ldx, d_lam, partial);
+    hipLaunchKernelGGL(k_reduce_rows, dim3(3 * k), dim3(kBlock), 0, state().stream, partial, nbx,
+                       d_out);
+    HIP_CHECK(hipGetLastError());
+}
+
+void bcg_p_update(int n, double *P, long ldp, const int *d_pmap, const double *R, long ldr,
+                  const int *d_rmap, const double *d_beta, int first, int k)
+{
+    if (k <= 0) return;
+    ProfScope prof("bcg_p");
+    hipLaunchKernelGGL(k_bcg_p, dim3(ceil_div(n, kBlock), k), dim3(kBlock), 0, state().stream, n, P,
+                       ldp, d_pmap, R, ldr, d_rmap, d_beta, first);
+    HIP_CHECK(hipGetLastError());
+}
+
+void bcg_xr_update(int n, double *X, long ldx, const int *d_xmap, double *R, long ldr,
+                   const int *d_rmap, const double *P, long ldp, const int *d_pmap,
+                   const double *W, long ldw, const int *d_wmap, const double *d_rho2,
+                   const double *d_ptw, int k, double *d_rr_out)
+{
+    if (k <= 0) return;
+    const int rows = reduction_rows(n, k);
+    const int nbx = std::max(1, ceil_div(n, rows));
+    double *partial = partial_buffer((size_t)nbx * k);
+    ProfScope prof("bcg_xr");
+    hipLaunchKernelGGL(k_bcg_xr, dim3(nbx, k), dim3(kBlock), 0, state().stream, n, rows, X, ldx,
+                       d_xmap, R, ldr, d_rmap, P, ldp, d_pmap, W, ldw, d_wmap, d_rho2, d_ptw,
+                       partial);
+    hipLaunchKernelGGL(k_reduce_rows, dim3(k), dim3(kBlock), 0, state().stream, partial, nbx,
+                       d_rr_out);
+    HIP_CHECK(hipGetLastError());
+}
+
+void axpy_cols(int n, const double *d_alpha, const double *X, long ldx, const int *d_xmap,
+               double *Y, long ldy, const int *d_ymap, int k)
+{
+    if (k <= 0) return;
+    ProfScope prof("axpy_cols");
+    hipLaunchKernelGGL(k_axpy_cols, dim3(ceil_div(n, kBlock), k), dim3(kBlock), 0, state().stream,
+                       n, d_alpha, X, ldx, d_xmap, Y, ldy, d_ymap);
+    HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace gcge_hip

@@ -1,0 +1,120 @@
+/*
+ * gcge_hip.h -- the MI355X (gfx950) backend of the GCG eigensolver: the
+ * C ABI of libgcge_hip.so.
+ *
+ * The backend fills the GCGE operations table (gcge_ops.h) with device
+ * implementations, the way the reference CSR backend fills it with host loops
+ * (reference app/csr/gcge_app_csr.c:70-84, GCGE_CSR_SetOps).  A program
+ * written for the reference switches backends by replacing
+ *     GCGE_CSR_SetOps(solver->ops)          with   GCGE_HIP_SetOps(solver->ops)
+ * and creating its matrices with GCGE_HIP_MatCreateCSR instead of the CSR_MAT
+ * constructors (reference app/csr/gcge_app_csr.h:27-46).
+ *
+ * Data layout in device memory:
+ *   matrices   CSR input, stored as sliced ELLPACK with slices of 64 rows
+ *              (one wavefront per slice, entries of a slice column-major),
+ *              int32 column indices, float64 values (and an optional float32
+ *              copy of the values for the mixed-precision inner solves);
+ *   multivecs  one column-major block of n x ncols float64, leading dimension
+ *              n rounded up to 64; column j is a vector view.
+ * All device work runs on one HIP stream per process.  Operations that
+ * return host values synchronise that stream; the others are asynchronous.
+ * Errors: the first HIP error is recorded (GCGE_HIP_LastError) and the
+ * process aborts with a message, since the operations table has no error
+ * channel (every table operation returns void).
+ */
+#ifndef GCGE_HIP_H_
+#define GCGE_HIP_H_
+
+#include <stdint.h>
+
+#include "gcge_ops.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- lifecycle ---------------------------------------------------------- */
+/* Select the device and create the stream and buffers (idempotent).  Returns 0
+ * on success, nonzero when no usable GPU is present.  No counterpart in the
+ * reference (its CSR backend needs no set-up). */
+int GCGE_HIP_Init(int device);
+void GCGE_HIP_Finalize(void);
+/* First recorded error message, or "" */
+const char *GCGE_HIP_LastError(void);
+/* Wait for all queued device work */
+void GCGE_HIP_Synchronize(void);
+/* Seed of the counter-based uniform(-1, 1) generator used by
+ * VecSetRandomValue / MultiVecSetRandomValue (reference
+ * app/csr/gcge_app_csr.c:284-294 draws uniform[0,1) from rand()) */
+void GCGE_HIP_SetSeed(uint64_t seed);
+
+/* ---- matrices (replace CSR_MAT, reference app/csr/gcge_app_csr.h:27-41) --- */
+/* Copy a host CSR matrix (0-based int32 rowptr[nrows+1], colidx[nnz], float64
+ * vals[nnz]) to the device.  Returns an opaque matrix handle for the
+ * operations table, or NULL on error. */
+void *GCGE_HIP_MatCreateCSR(int nrows, int ncols, int nnz, const int *rowptr, const int *colidx,
+                            const double *vals);
+void GCGE_HIP_MatDestroy(void *mat);
+/* fp32 != 0: keep a float32 copy of the values and use it in the inner block
+ * CG (mixed-precision form); fp32 == 0: float64 everywhere (default). */
+void GCGE_HIP_MatSetPrecision(void *mat, int fp32);
+
+/* ---- multivecs (replace the default MultiVecCreateByMat of per-vector
+ *      allocations, reference src/gcge_ops.c:72-80) ----------------------- */
+/* Returns a void** handle usable wherever the table expects a multivec. */
+void **GCGE_HIP_MultiVecCreate(int n, int ncols);
+void GCGE_HIP_MultiVecDestroy(void **mv);
+/* host (column-major, leading dimension ld) <-> columns start..start+ncols-1 */
+void GCGE_HIP_MultiVecSetHost(void **mv, int start, int ncols, const double *host, int ld);
+void GCGE_HIP_MultiVecGetHost(void **mv, int start, int ncols, double *host, int ld);
+
+/* ---- block kernels on multivec column ranges ----------------------------- */
+/* Y(:, ys:ys+k) = A X(:, xs:xs+k)  (reference per-column loop
+ * src/gcge_ops.c:115-129 over app/csr/gcge_app_csr.c:297-323) */
+void GCGE_HIP_SpMM(void *mat, void **X, int xs, void **Y, int ys, int k);
+/* G(i, j) = V(:, vs+i)^T W(:, ws+j), host G with leading dimension ldg
+ * (reference src/gcge_ops.c:226-309, "ns" form) */
+void GCGE_HIP_Gram(void **V, int vs, int m, void **W, int ws, int k, double *G, int ldg);
+/* Y(:, ys+j) = beta Y(:, ys+j) + sum_i V(:, vs+i) C(i, j), host C with leading
+ * dimension ldc (reference src/gcge_ops.c:190-221) */
+void GCGE_HIP_LinearComb(void **V, int vs, int m, const double *C, int ldc, double beta,
+                         void **Y, int ys, int k);
+/* Y(:, ys:ys+k) = a X(:, xs:xs+k) + b Y(:, ys:ys+k) (reference
+ * src/gcge_ops.c:149-163 over app/csr/gcge_app_csr.c:330-365) */
+void GCGE_HIP_Axpby(double a, void **X, int xs, double b, void **Y, int ys, int k);
+/* uniform(-1, 1) in columns start..start+k-1 (one draw) */
+void GCGE_HIP_RandomFill(void **X, int start, int k);
+
+/* ---- the operations table ------------------------------------------------ */
+/* Fill the table with the device operations (replaces GCGE_CSR_SetOps,
+ * reference app/csr/gcge_app_csr.c:70-84).  Besides the mandatory
+ * single-vector operations it provides every multivec operation as a block
+ * kernel, MultiLinearSolver (block CG with the reference GCGE_BCG arithmetic,
+ * src/gcge_cg.c:302-501, parameters from GCGE_HIP_SetLinearSolverParams) and
+ * publishes the GCGE_OPS_EXT block extensions in ops->ops. */
+void GCGE_HIP_SetOps(GCGE_OPS *ops);
+/* Parameters MultiLinearSolver uses when a driver calls it without them
+ * (reference defaults: cg_max_it 20, cg_rate 1e-2, no shift). */
+void GCGE_HIP_SetLinearSolverParams(int max_it, double rate, int if_shift, double shift, void *B);
+/* Descriptor {SetOps, MultiVecSetHost, MultiVecGetHost} for the one-call
+ * driver GCGE_SolveWithBackend (gcge.h). */
+const void *GCGE_HIP_Backend(void);
+
+/* ---- measurement --------------------------------------------------------- */
+/* Per-kernel device time, measured with HIP events on the backend stream
+ * while enabled.  name: "spmm", "gram", "lincomb", "axpby", ...; returns the
+ * number of launches and the summed milliseconds. */
+void GCGE_HIP_ProfileEnable(int on);
+int GCGE_HIP_KernelTime(const char *name, double *ms_total, int *count);
+/* as GCGE_HIP_KernelTime, plus the summed algorithmic bytes of the launches
+ * (SpMM: nnz*12 + 4*(n+1) + 8*k*(ncols+nrows); Gram and linear combination:
+ * 8*n*(m+k), plus 8*n*k when the combination also reads Y) */
+int GCGE_HIP_KernelStats(const char *name, double *ms_total, int *count, double *bytes_total);
+void GCGE_HIP_ProfileReset(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GCGE_HIP_H_ */

@@ -1,0 +1,134 @@
+"""Block kernels of the device backend against the oracle (GPU).
+
+Tolerances: the device sums in a different order than the oracle's serial
+loops, so entries are compared against a forward-error bound built from
+absolute values -- |gpu - cpu| <= tol * (|A| |X|) for SpMM and
+|gpu - cpu| <= tol * (|V|^T |W|) for the Gram product -- with tol far below
+what a wrong index, a dropped term or a transposed operand would produce
+(those give O(1) relative errors).  The random generator is integer
+arithmetic and must match bit for bit.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from gcge_old_amd import DeviceMatrix, DeviceMultiVec, _native, problems
+
+pytestmark = pytest.mark.gpu
+EPS = np.finfo(np.float64).eps
+
+
+def _rng(seed=0):
+    return np.random.default_rng(seed)
+
+
+def _random_csr(n, m, density, seed, long_rows=(), empty_rows=()):
+    import scipy.sparse as sp
+
+    rng = _rng(seed)
+    A = sp.random(n, m, density=density, random_state=rng, format="lil")
+    for r in long_rows:
+        A[r, :] = rng.standard_normal(m)
+    for r in empty_rows:
+        A[r, :] = 0
+    return problems.CSR.from_scipy(A.tocsr())
+
+
+MATS = {
+    "fd45": lambda: problems.fd_laplacian_2d(45),
+    "q1_e10": lambda: problems.q1_fem_3d(10)[0],
+    "q1_mass_e9": lambda: problems.q1_fem_3d(9)[1],
+    "ragged": lambda: _random_csr(1000, 1000, 0.01, 3, long_rows=(5, 700), empty_rows=(0, 64, 999)),
+    "tiny": lambda: problems.CSR(3, 3, np.array([0, 2, 3, 5], np.int32),
+                                 np.array([0, 2, 1, 0, 2], np.int32),
+                                 np.array([2.0, -1.0, 3.0, -1.0, 4.0])),
+}
+
+
+@pytest.mark.parametrize("name", sorted(MATS))
+@pytest.mark.parametrize("k", [1, 2, 3, 5, 8, 13, 17])
+def test_spmm_matches_oracle(name, k):
+    A = MATS[name]()
+    rng = _rng(k)
+    X = rng.standard_normal((A.ncols, k + 3))
+    dA = DeviceMatrix(A)
+    dX = DeviceMultiVec.from_host(X)
+    dY = DeviceMultiVec(A.nrows, k + 2)
+    _native.hip().GCGE_HIP_SpMM(dA.handle, dX.handle, 2, dY.handle, 1, k)
+    got = dY.get(1, k)
+    want = oracle.spmm(oracle.Matrix(A), X[:, 2:2 + k])
+    bound = abs(A.to_scipy()) @ np.abs(X[:, 2:2 + k])
+    assert np.all(np.abs(got - want) <= 64 * EPS * bound + 1e-300)
+
+
+@pytest.mark.parametrize("n,m,k", [(1, 1, 1), (63, 16, 16), (64, 17, 1), (65, 1, 17),
+                                   (1000, 70, 10), (4097, 130, 65), (100000, 12, 3)])
+def test_gram_matches_oracle(n, m, k):
+    rng = _rng(n + m + k)
+    V = rng.standard_normal((n, m + 1))
+    W = rng.standard_normal((n, k + 2))
+    dV, dW = DeviceMultiVec.from_host(V), DeviceMultiVec.from_host(W)
+    G = np.zeros((m + 3, k), order="F")  # leading dimension larger than m
+    _native.hip().GCGE_HIP_Gram(dV.handle, 1, m, dW.handle, 2, k, _native.as_dptr(G), m + 3)
+    want = oracle.gram(V[:, 1:1 + m], W[:, 2:2 + k])
+    bound = np.abs(V[:, 1:1 + m]).T @ np.abs(W[:, 2:2 + k])
+    assert np.all(np.abs(G[:m] - want) <= 1e-13 * bound)
+    assert np.all(G[m:] == 0.0)  # rows beyond m untouched
+
+
+@pytest.mark.parametrize("m,k", [(1, 1), (5, 7), (90, 8), (90, 9), (300, 50)])
+@pytest.mark.parametrize("beta", [0.0, 1.0, -0.5])
+def test_linear_comb_matches_oracle(m, k, beta):
+    n = 3001
+    rng = _rng(m * k)
+    V = rng.standard_normal((n, m + 2))
+    C = rng.standard_normal((m + 4, k))  # leading dimension m + 4
+    Y0 = rng.standard_normal((n, k + 1))
+    if beta == 0.0:
+        Y0[:, 1:] = np.nan  # must not be read when beta == 0
+    dV, dY = DeviceMultiVec.from_host(V), DeviceMultiVec.from_host(Y0)
+    Cf = np.asfortranarray(C)
+    _native.hip().GCGE_HIP_LinearComb(dV.handle, 2, m, _native.as_dptr(Cf), m + 4, beta, dY.handle,
+                                      1, k)
+    got = dY.get(1, k)
+    want = oracle.linear_comb(V[:, 2:2 + m], C[:m], beta, np.nan_to_num(Y0[:, 1:]))
+    bound = np.abs(V[:, 2:2 + m]) @ np.abs(C[:m]) + abs(beta) * np.abs(np.nan_to_num(Y0[:, 1:]))
+    assert np.all(np.isfinite(got))
+    assert np.all(np.abs(got - want) <= 1e-13 * bound)
+    np.testing.assert_array_equal(dY.get(0, 1), Y0[:, :1])  # column outside the range untouched
+
+
+@pytest.mark.parametrize("a,b", [(0.0, 2.0), (1.0, 0.0), (-3.0, 0.0), (0.5, -2.0), (0.0, 0.0)])
+def test_axpby_special_cases(a, b):
+    n, k = 777, 4
+    rng = _rng(7)
+    X = rng.standard_normal((n, k))
+    Y = rng.standard_normal((n, k))
+    Xs, Ys = X.copy(), Y.copy()
+    if a == 0.0:
+        Xs[:] = np.nan  # x is not read when a == 0 (y = b*y, as the reference computes it)
+    elif b == 0.0:
+        Ys[:] = np.nan  # y is not read when b == 0 and a != 0
+    dX, dY = DeviceMultiVec.from_host(Xs), DeviceMultiVec.from_host(Ys)
+    _native.hip().GCGE_HIP_Axpby(a, dX.handle, 0, b, dY.handle, 0, k)
+    got = dY.get()
+    if a == 0.0:
+        want = b * Y
+    elif b == 0.0:
+        want = a * X
+    else:
+        want = a * X + b * Y
+    np.testing.assert_allclose(got, want, rtol=4 * EPS, atol=4 * EPS * np.abs(want).max())
+
+
+def test_random_fill_bitwise():
+    n, k = 5000, 6
+    lib = _native.hip()
+    lib.GCGE_HIP_SetSeed(1234)
+    dX = DeviceMultiVec(n, k + 2)
+    lib.GCGE_HIP_RandomFill(dX.handle, 2, k)  # draw 0, columns 2..7
+    lib.GCGE_HIP_RandomFill(dX.handle, 0, 2)  # draw 1, columns 0..1
+    got = dX.get()
+    np.testing.assert_array_equal(got[:, 2:], oracle.random_fill(1234, 0, 2, k, n))
+    np.testing.assert_array_equal(got[:, :2], oracle.random_fill(1234, 1, 0, 2, n))
+    assert got.min() >= -1.0 and got.max() < 1.0

@@ -1,0 +1,104 @@
+"""End-to-end GCG solves on the device backend (GPU).
+
+Each solve starts from the same initial block on both sides, so the device
+run and the oracle run (the same GCG core on the CPU restatement of the
+reference CSR operations) follow the same iteration up to rounding.  Checks:
+* eigenvalues agree with the oracle to 1e-10 relative and, where a closed
+  form exists, with the exact spectrum;
+* the same number of outer iterations (clusters of equal eigenvalues can
+  shift a convergence decision by one iteration, hence +-2);
+* every returned pair satisfies the residual test it was converged with;
+* the converged subspace agrees with the oracle's (largest principal angle).
+"""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+import gcge_old_amd as gcge
+import oracle
+from gcge_old_amd import problems
+
+pytestmark = pytest.mark.gpu
+
+
+def _inv_sqrt(G):
+    w, Q = np.linalg.eigh((G + G.T) / 2)
+    return (Q / np.sqrt(w)) @ Q.T
+
+
+def _largest_angle(X, Y, B=None):
+    """Largest principal angle between span(X) and span(Y) in the B inner
+    product (B may be only semidefinite, as the shipped mass matrices with
+    Dirichlet rows are; it must be definite on the two spans)."""
+    Bm = (lambda Z: Z) if B is None else (lambda Z: B.to_scipy() @ Z)
+    Xo = X @ _inv_sqrt(X.T @ Bm(X))
+    Yo = Y @ _inv_sqrt(Y.T @ Bm(Y))
+    s = np.linalg.svd(Xo.T @ Bm(Yo), compute_uv=False)
+    return float(np.arccos(np.clip(s.min(), -1.0, 1.0)))
+
+
+def _residuals(A, B, ev, X):
+    As = A.to_scipy()
+    BX = X if B is None else B.to_scipy() @ X
+    R = As @ X - BX * ev
+    return np.linalg.norm(R, axis=0) / (np.abs(ev) * np.linalg.norm(BX, axis=0))
+
+
+def _fixture_matrices(golden_dir, name):
+    d = np.load(f"{golden_dir}/data/{name}.npz")
+    return problems.CSR(int(d["n"]), int(d["n"]), d["rowptr"], d["colidx"], d["vals"])
+
+
+CASES = {
+    # name: (A, B, nev, flags, exact eigenvalues or None)
+    "fd2d_32_nev10": (lambda g: (problems.fd_laplacian_2d(32), None), 10,
+                      ["-gcge_block_size", "10"], lambda: problems.fd_laplacian_2d_eigs(32, 10)),
+    "q1_e12_nev20": (lambda g: problems.q1_fem_3d(12), 20,
+                     ["-gcge_ev_max_it", "60", "-gcge_ev_tol", "1e-10"], lambda: problems.q1_eigs(12, 20)),
+    "A5M5_nev20": (lambda g: (_fixture_matrices(g, "A_5"), _fixture_matrices(g, "M_5")), 20,
+                   ["-gcge_block_size", "10", "-gcge_ev_max_it", "40"], None),
+}
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_device_solve_matches_oracle(case, golden_dir):
+    make, nev, flags, exact = CASES[case]
+    A, B = make(golden_dir)
+    x0 = oracle.random_fill(11, 0, 0, nev, A.nrows)
+    dev = gcge.solve(A, B, nev, flags, x0=x0)
+    cpu = oracle.solve(A, B, nev, flags, x0=x0)
+    assert dev["num_conv"] == nev and cpu["num_conv"] == nev
+    assert abs(dev["num_iter"] - cpu["num_iter"]) <= 2
+    np.testing.assert_allclose(dev["eval"], cpu["eval"], rtol=1e-10)
+    if exact is not None:
+        np.testing.assert_allclose(np.sort(dev["eval"]), exact(), rtol=1e-10)
+    tol = float(flags[flags.index("-gcge_ev_tol") + 1]) if "-gcge_ev_tol" in flags else 1e-8
+    # reference criterion ||r|| / (||x|| max(1, |lambda|)) as reported by the solver
+    assert np.all(dev["res"] < tol)
+    # eigenvector subspace of the whole converged block
+    assert _largest_angle(dev["evec"], cpu["evec"], B) < 1e-6
+
+
+def test_fd2d_128_closed_form_rb_criterion():
+    """2D FD Laplacian 128^2, B = I, nev = 10 at the relative residual
+    ||Ax - lambda Bx|| / (|lambda| ||Bx||) <= 1e-8."""
+    A = problems.fd_laplacian_2d(128)
+    out = gcge.solve(A, None, 10, ["-gcge_conv_type", "RB", "-gcge_ev_tol", "1e-8",
+                                   "-gcge_block_size", "20", "-gcge_ev_max_it", "100"], seed=3)
+    assert out["num_conv"] == 10
+    np.testing.assert_allclose(out["eval"], problems.fd_laplacian_2d_eigs(128, 10), rtol=1e-10)
+    assert np.all(_residuals(A, None, out["eval"], out["evec"]) <= 1.05e-8)
+
+
+def test_q1_generalized_closed_form():
+    """Q1 FEM 20^3 (n = 6859), kappa = 1, nev = 30 against the Kronecker-sum
+    spectrum, B-orthonormal eigenvectors."""
+    K, M = problems.q1_fem_3d(20)
+    out = gcge.solve(K, M, 30, ["-gcge_conv_type", "RB", "-gcge_ev_tol", "1e-9",
+                                "-gcge_ev_max_it", "100"], seed=5)
+    assert out["num_conv"] == 30
+    np.testing.assert_allclose(np.sort(out["eval"]), problems.q1_eigs(20, 30), rtol=1e-10)
+    X = out["evec"]
+    G = X.T @ (M.to_scipy() @ X)
+    assert np.abs(G - np.eye(30)).max() < 1e-10
+    assert np.all(_residuals(K, M, out["eval"], X) <= 1.05e-9)
