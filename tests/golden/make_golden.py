@@ -33,6 +33,7 @@ RUNS = [
     ("A5M5_nev20", ("A_5", "M_5"), 20, ["-gcge_block_size", "10", "-gcge_ev_max_it", "40"], 7),
     ("A5M5_nev40", ("A_5", "M_5"), 40, ["-gcge_ev_max_it", "60"], 7),
     ("A3M3_nev8", ("A_3", "M_3"), 8, ["-gcge_ev_max_it", "60"], 5),
+    ("A5M5_nev9", ("A_5", "M_5"), 9, [], 7),
     ("fd2d_24_nev12", "fd24", 12, ["-gcge_ev_max_it", "60"], 9),
     ("q1_e10_nev15", "q1e10", 15, ["-gcge_ev_max_it", "60", "-gcge_ev_tol", "1e-10"], 3),
 ]
