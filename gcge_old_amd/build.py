@@ -83,15 +83,19 @@ def build_oracle(with_reference: bool = True) -> None:
     _run(["make", "-s", "-C", odir])
     if with_reference and os.path.isdir("/root/reference/src"):
         _run(["make", "-s", "-j8", "-C", odir, "ref"])
+        # the reference core on the device backend (the drop-in check)
+        if os.path.exists(os.path.join(LIB, "libgcge_hip.so")):
+            _run(["make", "-s", "-C", odir, "ref-hip"])
 
 
 def build_all(verbose: bool = True) -> None:
-    with cf.ThreadPoolExecutor(max_workers=3) as ex:
-        futs = [ex.submit(build_core), ex.submit(build_hip), ex.submit(build_oracle)]
+    with cf.ThreadPoolExecutor(max_workers=2) as ex:
+        futs = [ex.submit(build_core), ex.submit(build_hip)]
         for f in futs:
             r = f.result()
             if verbose and r:
                 print("built", r)
+    build_oracle()
 
 
 if __name__ == "__main__":

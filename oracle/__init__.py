@@ -159,17 +159,27 @@ def _native_solve(_native, backend, Ah, Bh, n, nev, flags, x0, history_cap, keep
                 time=float(info[2]), block_size=int(info[3]), history=h)
 
 
+REF_HIP_BIN = os.path.join(HERE, "_ref", "gcge_ref_hip")
+
+
 def ref_available() -> bool:
     return os.path.exists(REF_BIN)
 
 
+def ref_hip_available() -> bool:
+    return os.path.exists(REF_HIP_BIN)
+
+
 def ref_solve(A, B, nev: int, flags: list[str] | None = None, x0: np.ndarray | None = None,
-              print_eval: bool = False, timeout: float = 3600.0):
-    """Run the reference solver (built from its sources) in a subprocess.
-    Returns a dict with eval, res, evec, num_iter, num_conv, time and, when
-    print_eval, the per-check eigenvalue history parsed from its output."""
-    if not ref_available():
-        raise RuntimeError(f"{REF_BIN} is not built (make -C oracle ref)")
+              print_eval: bool = False, timeout: float = 3600.0, on_device: bool = False):
+    """Run the reference solver (built from its sources) in a subprocess, on
+    its CSR backend or (on_device) on the device backend through
+    GCGE_HIP_SetOps.  Returns a dict with eval, res, evec, num_iter,
+    num_conv, time and, when print_eval, the per-check eigenvalue history
+    parsed from its output."""
+    binary = REF_HIP_BIN if on_device else REF_BIN
+    if not os.path.exists(binary):
+        raise RuntimeError(f"{binary} is not built (make -C oracle ref ref-hip)")
     with tempfile.TemporaryDirectory() as d:
         a_path = os.path.join(d, "A.bin")
         A.write_bin(a_path)
@@ -182,7 +192,7 @@ def ref_solve(A, B, nev: int, flags: list[str] | None = None, x0: np.ndarray | N
             x_path = os.path.join(d, "x0.bin")
             np.asfortranarray(x0, dtype=np.float64).T.tofile(x_path)
         out = os.path.join(d, "out.bin")
-        args = [REF_BIN, a_path, b_path, str(nev), x_path, out] + list(flags or [])
+        args = [binary, a_path, b_path, str(nev), x_path, out] + list(flags or [])
         args += ["-gcge_print_para", "0", "-gcge_print_final_part_time", "0",
                  "-gcge_print_eval", "1" if print_eval else "0"]
         p = subprocess.run(args, capture_output=True, text=True, timeout=timeout)
