@@ -47,7 +47,59 @@ def parse():
                    help="GCG iterations timed for the CPU baseline (0: skip)")
     p.add_argument("--no-profile", action="store_true",
                    help="time without per-kernel events (no roofline numbers)")
+    p.add_argument("--kernels", action="store_true",
+                   help="also time SpMM, Gram, linear combination and a plain copy in isolation "
+                        "(printed on stderr; the copy is the calibration for HBM counters)")
     return p.parse_args()
+
+
+def kernel_sweep(K, M):
+    """Isolated kernel timings on the workload's matrices (stderr)."""
+    import ctypes as C
+
+    import gcge_old_amd as gcge
+    from gcge_old_amd import _native
+
+    lib = _native.hip()
+    n = M.nrows
+    dK, dM = gcge.DeviceMatrix(K), gcge.DeviceMatrix(M)
+    X = gcge.DeviceMultiVec(n, 64)
+    Y = gcge.DeviceMultiVec(n, 64)
+    lib.GCGE_HIP_RandomFill(X.handle, 0, 64)
+
+    def timed(name, fn, reps=20):
+        lib.GCGE_HIP_ProfileReset()
+        lib.GCGE_HIP_ProfileEnable(1)
+        for _ in range(reps):
+            fn()
+        lib.GCGE_HIP_ProfileEnable(0)
+        ms, cnt, byt = C.c_double(0.0), C.c_int(0), C.c_double(0.0)
+        lib.GCGE_HIP_KernelStats(name.encode(), C.byref(ms), C.byref(cnt), C.byref(byt))
+        us = ms.value * 1e3 / max(cnt.value, 1)
+        gbs = byt.value / (ms.value * 1e-3) / 1e9 if ms.value > 0 else 0.0
+        return us, gbs
+
+    for mat, label in ((dK, "K"), (dM, "M")):
+        for k in (1, 4, 10, 16, 50):
+            us, gbs = timed("spmm", lambda: lib.GCGE_HIP_SpMM(mat.handle, X.handle, 0, Y.handle, 0, k))
+            print(f"# kernel spmm {label} k={k}: {us:.1f} us/launch, {gbs:.0f} GB/s", file=sys.stderr)
+    G = np.zeros((64, 64), order="F")
+    for m, k in ((70, 10), (60, 60), (10, 10), (1, 1)):
+        us, gbs = timed("gram", lambda: lib.GCGE_HIP_Gram(X.handle, 0, m, Y.handle, 0, k,
+                                                          _native.as_dptr(G), 64))
+        print(f"# kernel gram m={m} k={k}: {us:.1f} us/launch, {gbs:.0f} GB/s", file=sys.stderr)
+    Cm = np.asfortranarray(np.random.default_rng(0).standard_normal((64, 64)))
+    for m, k in ((90, 50), (80, 30), (70, 10)):
+        m = min(m, 64)
+        us, gbs = timed("lincomb", lambda: lib.GCGE_HIP_LinearComb(X.handle, 0, m, _native.as_dptr(Cm),
+                                                                   64, 0.0, Y.handle, 0, k))
+        print(f"# kernel lincomb m={m} k={k}: {us:.1f} us/launch, {gbs:.0f} GB/s", file=sys.stderr)
+    # plain copy of 64 columns (n x 64 doubles read and written): the counter
+    # calibration dispatch
+    us, _ = timed("axpby", lambda: lib.GCGE_HIP_Axpby(1.0, X.handle, 0, 0.0, Y.handle, 0, 64))
+    copy_bytes = 2 * 8.0 * n * 64
+    print(f"# kernel copy {copy_bytes / 2**20:.1f} MiB moved: {us:.1f} us/launch, "
+          f"{copy_bytes / (us * 1e-6) / 1e9:.0f} GB/s", file=sys.stderr)
 
 
 def cpu_baseline(K, M, nev, tol, iters, dev_iters):
@@ -94,6 +146,8 @@ def main():
     flags = ["-gcge_conv_type", "RB", "-gcge_ev_tol", str(args.tol), "-gcge_ev_max_it", "500"]
     print(f"# workload: Q1 {E}^3 n={K.nrows} nnz(K)={K.nnz} nnz(M)={M.nnz} nev={args.nev} "
           f"(assembly {assemble_s:.1f}s)", file=sys.stderr, flush=True)
+    if args.kernels:
+        kernel_sweep(K, M)
 
     for w in range(args.warmup):
         out = gcge.solve(dK, dM, args.nev, flags, seed=100 + w)

@@ -40,7 +40,43 @@ void GCGE_ComputeW(void *A, void *B, void **V, GCGE_DOUBLE *eval, GCGE_OPS *ops,
     void **V_tmp = ws->V_tmp;
     void *rhs, *xc, *wc;
     GCGE_INT mv_s[5], mv_e[5];
+    GCGE_OPS_EXT *ext = GCGE_OPS_GetExt(ops);
 
+    if (para->use_ops_ext && ext && ext->MatDotMultiVecCols && ext->MultiVecAxpbyCols &&
+        w_length > 0) {
+        /* the same per-column operations as the loop below, one call each:
+         * rhs = B x (or x), rhs *= lambda + shift where |lambda| <= 1
+         * (y = 1*y elsewhere), w = x or x / (lambda + shift) */
+        GCGE_INT *cols = (GCGE_INT *)malloc(3 * (size_t)w_length * sizeof(GCGE_INT));
+        GCGE_DOUBLE *coef = (GCGE_DOUBLE *)malloc(4 * (size_t)w_length * sizeof(GCGE_DOUBLE));
+        GCGE_INT *xcols = cols, *rcols = cols + w_length, *wcols = cols + 2 * w_length;
+        GCGE_DOUBLE *ra = coef, *rb = coef + w_length, *wa = coef + 2 * w_length,
+                    *wb = coef + 3 * w_length;
+        for (GCGE_INT idx = 0; idx < w_length; ++idx) {
+            const GCGE_INT xi = unlock[idx];
+            xcols[idx] = xi;
+            rcols[idx] = idx;
+            wcols[idx] = w_start + idx;
+            const int small = fabs(eval[xi]) <= 1.0;
+            ra[idx] = 0.0;
+            rb[idx] = small ? eval[xi] + para->shift : 1.0;
+            wa[idx] = small ? 1.0 : 1.0 / (eval[xi] + para->shift);
+            wb[idx] = 0.0;
+        }
+        if (B == NULL) {
+            /* rhs = x: a = 1, b = 0 (a copy) */
+            GCGE_DOUBLE *ones = (GCGE_DOUBLE *)malloc((size_t)w_length * sizeof(GCGE_DOUBLE));
+            for (GCGE_INT idx = 0; idx < w_length; ++idx) ones[idx] = 1.0;
+            ext->MultiVecAxpbyCols(ones, V, xcols, wb, V_tmp, rcols, w_length, ops);
+            free(ones);
+        } else {
+            ext->MatDotMultiVecCols(B, V, xcols, V_tmp, rcols, w_length, ops);
+        }
+        ext->MultiVecAxpbyCols(ra, V_tmp, rcols, rb, V_tmp, rcols, w_length, ops);
+        ext->MultiVecAxpbyCols(wa, V, xcols, wb, V, wcols, w_length, ops);
+        free(cols);
+        free(coef);
+    } else
     for (GCGE_INT idx = 0; idx < w_length; ++idx) {
         GCGE_INT xi = unlock[idx], wi = w_start + idx;
         ops->GetVecFromMultiVec(V_tmp, idx, &rhs, ops);

@@ -489,11 +489,15 @@ static void op_mv_lincomb(void **x, void **y, GCGE_INT *start, GCGE_INT *end, do
     check_range(X, start[0], std::max(m, 0), "MultiVecLinearComb");
     check_range(Y, start[1], k, "MultiVecLinearComb");
     if (m <= 0) fail_msg("MultiVecLinearComb: empty input range");
-    std::vector<double> packed((size_t)m * k);
-    for (int j = 0; j < k; ++j) memcpy(&packed[(size_t)j * m], a + (size_t)j * lda, (size_t)m * sizeof(double));
+    // coefficients row by row, zero-padded to the kernel's column group
+    const int g = lincomb_group(k);
+    const int ldct = (k + g - 1) / g * g;
+    std::vector<double> packed((size_t)m * ldct, 0.0);
+    for (int j = 0; j < k; ++j)
+        for (int i = 0; i < m; ++i) packed[(size_t)i * ldct + j] = a[(size_t)j * lda + i];
     double *d_C = small_buffer(packed.size());
     upload_async(d_C, packed.data(), packed.size() * sizeof(double));
-    lincomb_dev(Y->n, col_ptr(X, start[0]), X->ld, m, d_C, beta, col_ptr(Y, start[1]), Y->ld, k);
+    lincomb_dev(Y->n, col_ptr(X, start[0]), X->ld, m, d_C, ldct, beta, col_ptr(Y, start[1]), Y->ld, k);
 }
 
 static void op_mv_inner(void **V, void **W, double *a, char *is_sym, GCGE_INT *start,
@@ -759,7 +763,54 @@ static void ext_residual_norms(void *Amat, void *Bmat, void **Xh, GCGE_INT start
 
 static void ext_sync(GCGE_OPS *) { HIP_CHECK(hipStreamSynchronize(state().stream)); }
 
-static GCGE_OPS_EXT g_ext = {GCGE_OPS_EXT_MAGIC, ext_residual_norms, ext_block_cg, ext_sync, nullptr};
+static int *upload_maps(const GCGE_INT *xcols, const GCGE_INT *ycols, int k)
+{
+    std::vector<int> maps((size_t)2 * k);
+    for (int c = 0; c < k; ++c) {
+        maps[c] = xcols[c];
+        maps[k + c] = ycols[c];
+    }
+    int *d = index_buffer(maps.size());
+    upload_async(d, maps.data(), maps.size() * sizeof(int));
+    return d;
+}
+
+static void ext_mat_dot_cols(void *mat, void **x, const GCGE_INT *xcols, void **y,
+                             const GCGE_INT *ycols, GCGE_INT k, GCGE_OPS *)
+{
+    if (k <= 0) return;
+    Mat *A = as_mat(mat);
+    MultiVec *X = as_mv(x), *Y = as_mv(y);
+    for (int c = 0; c < k; ++c) {
+        check_range(X, xcols[c], 1, "MatDotMultiVecCols");
+        check_range(Y, ycols[c], 1, "MatDotMultiVecCols");
+    }
+    if (X->n != A->ncols || Y->n != A->nrows) fail_msg("MatDotMultiVecCols: sizes differ");
+    int *d = upload_maps(xcols, ycols, k);
+    spmm(A, X->data, X->ld, Y->data, Y->ld, k, d, d + k);
+}
+
+static void ext_axpby_cols(const GCGE_DOUBLE *a, void **x, const GCGE_INT *xcols,
+                           const GCGE_DOUBLE *b, void **y, const GCGE_INT *ycols, GCGE_INT k,
+                           GCGE_OPS *)
+{
+    if (k <= 0) return;
+    MultiVec *X = as_mv(x), *Y = as_mv(y);
+    for (int c = 0; c < k; ++c) {
+        check_range(X, xcols[c], 1, "MultiVecAxpbyCols");
+        check_range(Y, ycols[c], 1, "MultiVecAxpbyCols");
+    }
+    int *d = upload_maps(xcols, ycols, k);
+    std::vector<double> ab((size_t)2 * k);
+    memcpy(ab.data(), a, (size_t)k * sizeof(double));
+    memcpy(ab.data() + k, b, (size_t)k * sizeof(double));
+    double *dab = small_buffer(ab.size());
+    upload_async(dab, ab.data(), ab.size() * sizeof(double));
+    axpby_list(Y->n, dab, X->data, X->ld, d, dab + k, Y->data, Y->ld, d + k, k);
+}
+
+static GCGE_OPS_EXT g_ext = {GCGE_OPS_EXT_MAGIC, ext_residual_norms, ext_block_cg, ext_sync,
+                             ext_mat_dot_cols, ext_axpby_cols, nullptr};
 
 // ---- host copies ----------------------------------------------------------------------------------------------
 static void mv_set_host(void **mv, int start, int ncols, const double *host, int ld)

@@ -108,11 +108,16 @@ void spmm(const Mat *A, const double *X, long ldx, double *Y, long ldy, int k,
 // host G(m x k, ldg) = V^T W
 void gram(int n, const double *V, long ldv, int m, const double *W, long ldw, int k, double *G,
           int ldg, bool sync_result = true);
-// Y = beta Y + V C with C on the device (m x k, ldc = m)
-void lincomb_dev(int n, const double *V, long ldv, int m, const double *d_C, double beta,
-                 double *Y, long ldy, int k);
+// Y = beta Y + V C with Ct = C stored row by row on the device (ldct >= k,
+// a multiple of lincomb_group(k), padding columns zero)
+int lincomb_group(int k);
+void lincomb_dev(int n, const double *V, long ldv, int m, const double *d_Ct, int ldct,
+                 double beta, double *Y, long ldy, int k);
 void axpby(int n, double a, const double *X, long ldx, double b, double *Y, long ldy, int k);
 void swap_cols(int n, double *X, long ldx, double *Y, long ldy, int k);
+// Y(:, ymap[c]) = a[c] X(:, xmap[c]) + b[c] Y(:, ymap[c]), scalars and maps on the device
+void axpby_list(int n, const double *d_a, const double *X, long ldx, const int *d_xmap,
+                const double *d_b, double *Y, long ldy, const int *d_ymap, int k);
 void random_fill(int n, double *X, long ldx, int col0, int k, uint64_t seed, uint64_t draw);
 // per-column dots out[c] = A(:, amap[c]) . B(:, bmap[c]) into device memory d_out
 void coldots(int n, const double *A, long lda, const int *d_amap, const double *B, long ldb,

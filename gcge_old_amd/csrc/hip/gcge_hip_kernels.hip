@@ -11,8 +11,10 @@
 //               output tiles, 32-row steps staged through LDS (transposed),
 //               4x4 register blocks per thread, split over row chunks with a
 //               deterministic second-pass reduction.
-//   lincomb     Y = beta Y + V C: one lane per row, coefficient chunks in LDS
-//               (broadcast reads), 8 output columns per pass.
+//   lincomb     Y = beta Y + V C: one lane per row holding 8..64 output
+//               columns in registers; the coefficients of one input column
+//               are a wave-uniform contiguous read, so V is read once per
+//               group of up to 64 output columns.
 //   axpby/swap/random/coldots/residual/bcg_*: streaming kernels.
 #include <hip/hip_runtime.h>
 
@@ -194,37 +196,39 @@ __global__ __launch_bounds__(kBlock) void k_reduce_partials(const double *__rest
 }
 
 // ---- linear combination --------------------------------------------------------------
-constexpr int kLcCols = 8;
 
+// Ct is the coefficient matrix stored row by row (Ct[i*ldct + j] = C(i, j))
+// and padded with zero columns to a multiple of KT, so the KT coefficients a
+// row of V meets are one contiguous, wave-uniform read (scalar loads).  Each
+// lane owns one row and KT accumulators; V is read once per group of KT
+// output columns.
+template <int KT>
 __global__ __launch_bounds__(kBlock) void k_lincomb(int n, const double *__restrict__ V, long ldv,
-                                                   int m, const double *__restrict__ C,
+                                                   int m, const double *__restrict__ Ct, int ldct,
                                                    double beta, double *__restrict__ Y, long ldy,
                                                    int k)
 {
-    extern __shared__ double Cs[];  // m x kLcCols
     const int row = blockIdx.x * kBlock + threadIdx.x;
-    for (int c0 = 0; c0 < k; c0 += kLcCols) {
-        const int nc = min(kLcCols, k - c0);
-        for (int e = threadIdx.x; e < m * kLcCols; e += kBlock) {
-            const int i = e / kLcCols, cc = e % kLcCols;
-            Cs[e] = (cc < nc) ? C[(long)(c0 + cc) * m + i] : 0.0;
+    if (row >= n) return;
+    for (int c0 = 0; c0 < k; c0 += KT) {
+        double acc[KT];
+#pragma unroll
+        for (int cc = 0; cc < KT; ++cc) acc[cc] = 0.0;
+        const double *cp = Ct + c0;
+        for (int i = 0; i < m; ++i) {
+            const double v = V[(long)i * ldv + row];
+            const double *ci = cp + (long)i * ldct;
+#pragma unroll
+            for (int cc = 0; cc < KT; ++cc) acc[cc] += v * ci[cc];
         }
-        __syncthreads();
-        if (row < n) {
-            double acc[kLcCols];
+        const int nc = min(KT, k - c0);
 #pragma unroll
-            for (int cc = 0; cc < kLcCols; ++cc) acc[cc] = 0.0;
-            for (int i = 0; i < m; ++i) {
-                const double v = V[(long)i * ldv + row];
-#pragma unroll
-                for (int cc = 0; cc < kLcCols; ++cc) acc[cc] += v * Cs[i * kLcCols + cc];
-            }
-            for (int cc = 0; cc < nc; ++cc) {
+        for (int cc = 0; cc < KT; ++cc) {
+            if (cc < nc) {
                 double *y = Y + (long)(c0 + cc) * ldy + row;
                 *y = (beta == 0.0) ? acc[cc] : beta * (*y) + acc[cc];
             }
         }
-        __syncthreads();
     }
 }
 
@@ -244,6 +248,27 @@ __global__ __launch_bounds__(kBlock) void k_axpby(int n, double a, const double 
     } else {
         const double x = X[(long)c * ldx + row];
         *y = (b == 0.0) ? a * x : a * x + b * (*y);
+    }
+}
+
+// per-column y = a x + b y over column lists, VecAxpby rules per column
+__global__ __launch_bounds__(kBlock) void k_axpby_list(int n, const double *__restrict__ a,
+                                                      const double *__restrict__ X, long ldx,
+                                                      const int *__restrict__ xmap,
+                                                      const double *__restrict__ b,
+                                                      double *__restrict__ Y, long ldy,
+                                                      const int *__restrict__ ymap)
+{
+    const int row = blockIdx.x * kBlock + threadIdx.x;
+    const int c = blockIdx.y;
+    if (row >= n) return;
+    const double ac = a[c], bc = b[c];
+    double *y = Y + (long)ymap[c] * ldy + row;
+    if (ac == 0.0) {
+        *y = bc * (*y);
+    } else {
+        const double x = X[(long)xmap[c] * ldx + row];
+        *y = (bc == 0.0) ? ac * x : ac * x + bc * (*y);
     }
 }
 
@@ -481,19 +506,25 @@ void gram(int n, const double *V, long ldv, int m, const double *W, long ldw, in
         for (int i = 0; i < m; ++i) G[(long)j * ldg + i] = h[(long)j * m + i];
 }
 
-void lincomb_dev(int n, const double *V, long ldv, int m, const double *d_C, double beta,
-                 double *Y, long ldy, int k)
+int lincomb_group(int k) { return k <= 8 ? 8 : k <= 16 ? 16 : k <= 32 ? 32 : 64; }
+
+void lincomb_dev(int n, const double *V, long ldv, int m, const double *d_Ct, int ldct,
+                 double beta, double *Y, long ldy, int k)
 {
     if (k <= 0 || n <= 0) return;
     if (m <= 0) {  // empty combination: y = beta y
         axpby(n, 0.0, Y, ldy, beta, Y, ldy, k);
         return;
     }
-    if (m > 1024) fail_msg("linear combination with more than 1024 input columns");
     ProfScope prof("lincomb", 8.0 * (double)n * (m + k) + (beta == 0.0 ? 0.0 : 8.0 * (double)n * k));
-    const size_t lds = (size_t)m * kLcCols * sizeof(double);
-    hipLaunchKernelGGL(k_lincomb, dim3(ceil_div(n, kBlock)), dim3(kBlock), lds, state().stream, n,
-                       V, ldv, m, d_C, beta, Y, ldy, k);
+    const dim3 grid(ceil_div(n, kBlock)), block(kBlock);
+    hipStream_t st = state().stream;
+    switch (lincomb_group(k)) {
+    case 8: hipLaunchKernelGGL(k_lincomb<8>, grid, block, 0, st, n, V, ldv, m, d_Ct, ldct, beta, Y, ldy, k); break;
+    case 16: hipLaunchKernelGGL(k_lincomb<16>, grid, block, 0, st, n, V, ldv, m, d_Ct, ldct, beta, Y, ldy, k); break;
+    case 32: hipLaunchKernelGGL(k_lincomb<32>, grid, block, 0, st, n, V, ldv, m, d_Ct, ldct, beta, Y, ldy, k); break;
+    default: hipLaunchKernelGGL(k_lincomb<64>, grid, block, 0, st, n, V, ldv, m, d_Ct, ldct, beta, Y, ldy, k); break;
+    }
     HIP_CHECK(hipGetLastError());
 }
 
@@ -503,6 +534,16 @@ void axpby(int n, double a, const double *X, long ldx, double b, double *Y, long
     ProfScope prof("axpby");
     hipLaunchKernelGGL(k_axpby, dim3(ceil_div(n, kBlock), k), dim3(kBlock), 0, state().stream, n,
                        a, X, ldx, b, Y, ldy);
+    HIP_CHECK(hipGetLastError());
+}
+
+void axpby_list(int n, const double *d_a, const double *X, long ldx, const int *d_xmap,
+                const double *d_b, double *Y, long ldy, const int *d_ymap, int k)
+{
+    if (k <= 0 || n <= 0) return;
+    ProfScope prof("axpby");
+    hipLaunchKernelGGL(k_axpby_list, dim3(ceil_div(n, kBlock), k), dim3(kBlock), 0, state().stream,
+                       n, d_a, X, ldx, d_xmap, d_b, Y, ldy, d_ymap);
     HIP_CHECK(hipGetLastError());
 }
 

@@ -45,6 +45,16 @@ typedef struct GCGE_OPS_EXT_ {
                     void **CG_P, void **CG_W, GCGE_DOUBLE *dtmp, GCGE_INT *itmp);
     /* Wait for all queued device work (timers call it before reading the clock). */
     void (*Synchronize)(GCGE_OPS *ops);
+    /* Y(:, ycols[c]) = M X(:, xcols[c]) for c < k: the column loop of
+     * MatDotVec calls over listed columns, as one call. */
+    void (*MatDotMultiVecCols)(void *mat, void **X, const GCGE_INT *xcols, void **Y,
+                               const GCGE_INT *ycols, GCGE_INT k, GCGE_OPS *ops);
+    /* Y(:, ycols[c]) = a[c] X(:, xcols[c]) + b[c] Y(:, ycols[c]) for c < k,
+     * each column with the VecAxpby rules (a[c] == 0: X not read; b[c] == 0:
+     * Y not read). */
+    void (*MultiVecAxpbyCols)(const GCGE_DOUBLE *a, void **X, const GCGE_INT *xcols,
+                              const GCGE_DOUBLE *b, void **Y, const GCGE_INT *ycols, GCGE_INT k,
+                              GCGE_OPS *ops);
     void *backend; /* backend private state */
 } GCGE_OPS_EXT;
 
