@@ -16,6 +16,11 @@ The JSON line carries:
                  oracle restatement when that binary is absent), one core,
                  timed on a bounded number of GCG iterations of the same
                  workload and scaled to the device run's iteration count
+  mixed          the mixed-precision form on the same workload and seeds
+                 (float32 products and directions in the inner block CG,
+                 float64 everywhere else): seconds per solve, speed-up over
+                 the float64 solve and the largest relative eigenvalue
+                 difference from it
 """
 from __future__ import annotations
 
@@ -45,6 +50,8 @@ def parse():
     p.add_argument("--tol", type=float, default=1e-8)
     p.add_argument("--cpu-iters", type=int, default=3,
                    help="GCG iterations timed for the CPU baseline (0: skip)")
+    p.add_argument("--mixed-steps", type=int, default=-1,
+                   help="solves of the mixed-precision form (default: --steps; 0: skip)")
     p.add_argument("--no-profile", action="store_true",
                    help="time without per-kernel events (no roofline numbers)")
     p.add_argument("--kernels", action="store_true",
@@ -63,9 +70,9 @@ def kernel_sweep(K, M):
     lib = _native.hip()
     n = M.nrows
     dK, dM = gcge.DeviceMatrix(K), gcge.DeviceMatrix(M)
-    X = gcge.DeviceMultiVec(n, 64)
-    Y = gcge.DeviceMultiVec(n, 64)
-    lib.GCGE_HIP_RandomFill(X.handle, 0, 64)
+    X = gcge.DeviceMultiVec(n, 96)
+    Y = gcge.DeviceMultiVec(n, 96)
+    lib.GCGE_HIP_RandomFill(X.handle, 0, 96)
 
     def timed(name, fn, reps=20):
         lib.GCGE_HIP_ProfileReset()
@@ -83,16 +90,15 @@ def kernel_sweep(K, M):
         for k in (1, 4, 10, 16, 50):
             us, gbs = timed("spmm", lambda: lib.GCGE_HIP_SpMM(mat.handle, X.handle, 0, Y.handle, 0, k))
             print(f"# kernel spmm {label} k={k}: {us:.1f} us/launch, {gbs:.0f} GB/s", file=sys.stderr)
-    G = np.zeros((64, 64), order="F")
-    for m, k in ((70, 10), (60, 60), (10, 10), (1, 1)):
+    G = np.zeros((96, 96), order="F")
+    for m, k in ((70, 10), (60, 60), (90, 10), (10, 10), (1, 1)):
         us, gbs = timed("gram", lambda: lib.GCGE_HIP_Gram(X.handle, 0, m, Y.handle, 0, k,
-                                                          _native.as_dptr(G), 64))
+                                                          _native.as_dptr(G), 96))
         print(f"# kernel gram m={m} k={k}: {us:.1f} us/launch, {gbs:.0f} GB/s", file=sys.stderr)
-    Cm = np.asfortranarray(np.random.default_rng(0).standard_normal((64, 64)))
+    Cm = np.asfortranarray(np.random.default_rng(0).standard_normal((96, 96)))
     for m, k in ((90, 50), (80, 30), (70, 10)):
-        m = min(m, 64)
         us, gbs = timed("lincomb", lambda: lib.GCGE_HIP_LinearComb(X.handle, 0, m, _native.as_dptr(Cm),
-                                                                   64, 0.0, Y.handle, 0, k))
+                                                                   96, 0.0, Y.handle, 0, k))
         print(f"# kernel lincomb m={m} k={k}: {us:.1f} us/launch, {gbs:.0f} GB/s", file=sys.stderr)
     # plain copy of 64 columns (n x 64 doubles read and written): the counter
     # calibration dispatch
@@ -100,6 +106,51 @@ def kernel_sweep(K, M):
     copy_bytes = 2 * 8.0 * n * 64
     print(f"# kernel copy {copy_bytes / 2**20:.1f} MiB moved: {us:.1f} us/launch, "
           f"{copy_bytes / (us * 1e-6) / 1e9:.0f} GB/s", file=sys.stderr)
+
+
+KERNELS = ("spmm", "spmm32", "gram", "gram_reduce", "lincomb", "axpby", "axpby_list", "swap",
+           "coldots", "bcg_p", "bcg_xr", "axpy_cols", "residual", "random")
+
+
+def kernel_stats(lib):
+    """Per-kernel HIP-event totals since the last profile reset."""
+    import ctypes as C
+
+    kernels = {}
+    for name in KERNELS:
+        ms, cnt, byt = C.c_double(0.0), C.c_int(0), C.c_double(0.0)
+        lib.GCGE_HIP_KernelStats(name.encode(), C.byref(ms), C.byref(cnt), C.byref(byt))
+        if cnt.value:
+            kernels[name] = {"ms": ms.value, "launches": cnt.value, "bytes": byt.value}
+    return kernels
+
+
+def mixed_leg(gcge, lib, dK, dM, nev, flags, steps, fp64_outs, fp64_value):
+    """The mixed-precision form on the same seeds as the float64 steps."""
+    lib.GCGE_HIP_ProfileReset()
+    lib.GCGE_HIP_ProfileEnable(1)
+    times, iters, errs = [], [], []
+    for s in range(steps):
+        out = gcge.solve(dK, dM, nev, flags, seed=1 + s, precision="mixed")
+        times.append(out["time"])
+        iters.append(out["num_iter"])
+        ref = fp64_outs[s % len(fp64_outs)]["eval"]
+        errs.append(float(np.max(np.abs(np.sort(out["eval"]) - np.sort(ref)) / np.abs(ref))))
+        print(f"# mixed step {s}: {out['time']:.4f}s iters={out['num_iter']} "
+              f"conv={out['num_conv']} max rel eig diff {errs[-1]:.2e}", file=sys.stderr, flush=True)
+    lib.GCGE_HIP_ProfileEnable(0)
+    k = kernel_stats(lib)
+    dK.set_precision(False)
+    dM.set_precision(False)
+    value = float(np.mean(times))
+    out = {"value": round(value, 5), "unit": "s", "speedup_vs_fp64": round(fp64_value / value, 3),
+           "iterations": iters, "max_rel_eig_diff_vs_fp64": max(errs),
+           "dtype": "f32 inner block CG products and directions, f64 elsewhere"}
+    if "spmm32" in k and k["spmm32"]["ms"] > 0:
+        sp = k["spmm32"]
+        out["spmm32_gbs"] = round(sp["bytes"] / (sp["ms"] * 1e-3) / 1e9, 1)
+    out["kernels"] = {n: {"ms": round(v["ms"], 3), "launches": v["launches"]} for n, v in k.items()}
+    return out
 
 
 def cpu_baseline(K, M, nev, tol, iters, dev_iters):
@@ -170,15 +221,7 @@ def main():
     wall = time.perf_counter() - t_start
     lib.GCGE_HIP_ProfileEnable(0)
 
-    import ctypes as C
-
-    kernels = {}
-    for name in ("spmm", "gram", "gram_reduce", "lincomb", "axpby", "swap", "coldots", "bcg_p",
-                 "bcg_xr", "residual", "random"):
-        ms, cnt, byt = C.c_double(0.0), C.c_int(0), C.c_double(0.0)
-        lib.GCGE_HIP_KernelStats(name.encode(), C.byref(ms), C.byref(cnt), C.byref(byt))
-        if cnt.value:
-            kernels[name] = {"ms": ms.value, "launches": cnt.value, "bytes": byt.value}
+    kernels = kernel_stats(lib)
 
     value = float(np.mean(times))
     roofline = None
@@ -195,6 +238,11 @@ def main():
     exact = problems.q1_eigs(E, args.nev)
     last = outs[-1]
     eig_err = float(np.max(np.abs(np.sort(last["eval"]) - exact) / exact))
+
+    mixed = None
+    mixed_steps = args.steps if args.mixed_steps < 0 else args.mixed_steps
+    if mixed_steps > 0:
+        mixed = mixed_leg(gcge, lib, dK, dM, args.nev, flags, mixed_steps, outs, value)
 
     cpu = None
     if args.cpu_iters > 0:
@@ -227,6 +275,7 @@ def main():
         "kernels": {k: {"ms": round(v["ms"], 3), "launches": v["launches"]}
                     for k, v in kernels.items()},
         "cpu_baseline": cpu,
+        "mixed": mixed,
     }
     print(json.dumps(line), flush=True)
 

@@ -51,6 +51,13 @@ class DeviceMatrix:
                                                           _native.as_dptr(v))
         if not self.handle:
             raise RuntimeError("GCGE_HIP_MatCreateCSR failed")
+        self.fp32 = False
+
+    def set_precision(self, fp32: bool) -> None:
+        """fp32=True: keep a float32 copy of the values and run the inner
+        block CG products in float32 (mixed-precision form)."""
+        _native.hip().GCGE_HIP_MatSetPrecision(self.handle, 1 if fp32 else 0)
+        self.fp32 = bool(fp32)
 
     def __del__(self):
         if getattr(self, "handle", None):
@@ -92,15 +99,23 @@ class DeviceMultiVec:
 
 
 def solve(A, B, nev: int, flags: list[str] | None = None, x0: np.ndarray | None = None,
-          seed: int = 1, history_cap: int = 0) -> dict:
+          seed: int = 1, history_cap: int = 0, precision: str = "fp64") -> dict:
     """Smallest nev eigenpairs of A x = lambda B x on the GPU (B may be None).
 
-    A and B are ``problems.CSR`` or ``DeviceMatrix``.  Returns a dict with
-    eval, evec (n x nev), res (last residuals), num_iter, num_conv, time
-    (device solve seconds, inputs resident), block_size and history."""
+    A and B are ``problems.CSR`` or ``DeviceMatrix``.  ``precision`` is
+    "fp64" (float64 throughout) or "mixed" (the products inside the inner
+    block CG in float32; Gram matrices, orthonormalisation, Rayleigh-Ritz and
+    the residual check stay float64).  Returns a dict with eval, evec
+    (n x nev), res (last residuals), num_iter, num_conv, time (device solve
+    seconds, inputs resident), block_size and history."""
+    if precision not in ("fp64", "mixed"):
+        raise ValueError(f"precision must be 'fp64' or 'mixed', not {precision!r}")
     init()
     Ad = A if isinstance(A, DeviceMatrix) else DeviceMatrix(A)
     Bd = None if B is None else (B if isinstance(B, DeviceMatrix) else DeviceMatrix(B))
+    for m in (Ad, Bd):
+        if m is not None:
+            m.set_precision(precision == "mixed")
     lib = _native.hip()
     lib.GCGE_HIP_SetSeed(seed)
     flags = list(flags or [])

@@ -80,6 +80,10 @@ def _declare(name: str, lib: C.CDLL) -> None:
         lib.GCGE_HIP_Axpby.argtypes = [C.c_double, C.c_void_p, C.c_int, C.c_double, C.c_void_p,
                                        C.c_int, C.c_int]
         lib.GCGE_HIP_RandomFill.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        lib.GCGE_HIP_SetLinearSolverParams.argtypes = [C.c_int, C.c_double, C.c_int, C.c_double,
+                                                       C.c_void_p]
+        lib.GCGE_HIP_MultiLinearSolve.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                                  C.c_int, C.c_int]
         lib.GCGE_HIP_Synchronize.argtypes = []
         lib.GCGE_HIP_KernelTime.argtypes = [C.c_char_p, dptr, iptr]
         lib.GCGE_HIP_KernelTime.restype = C.c_int

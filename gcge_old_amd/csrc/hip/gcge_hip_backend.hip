@@ -22,6 +22,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "gcge.h"
@@ -329,30 +330,42 @@ static void check_range(const MultiVec *mv, int start, int k, const char *who)
 
 static inline double *col_ptr(MultiVec *mv, int c) { return mv->data + (size_t)c * mv->ld; }
 
-// Backend scratch multivecs (residual check, block CG without a workspace)
+// Backend scratch blocks (residual check, block CG without a workspace,
+// float32 directions of the mixed-precision block CG); slots 0-3 hold
+// float64 blocks, 4-7 float32 blocks
 struct Scratch {
-    double *data = nullptr;
-    long ld = 0;
-    size_t cap = 0;
+    void *data = nullptr;
+    size_t cap = 0;  // bytes
 };
-static Scratch g_scratch[4];
+constexpr int kScratchSlots = 8;
+static Scratch g_scratch[kScratchSlots];
 
-static double *scratch(int which, int n, int k, long *ld)
+static void *scratch_bytes(int which, size_t need)
 {
     Scratch &s = g_scratch[which];
-    const long l = padded_ld(n);
-    const size_t need = (size_t)l * std::max(k, 1);
     if (need > s.cap || !s.data) {
         if (s.data) {
             HIP_CHECK(hipStreamSynchronize(state().stream));
             HIP_CHECK(hipFree(s.data));
         }
-        HIP_CHECK(hipMalloc((void **)&s.data, need * sizeof(double)));
+        HIP_CHECK(hipMalloc(&s.data, need));
         s.cap = need;
     }
-    s.ld = l;
-    *ld = l;
     return s.data;
+}
+
+static double *scratch(int which, int n, int k, long *ld)
+{
+    const long l = padded_ld(n);
+    *ld = l;
+    return (double *)scratch_bytes(which, (size_t)l * std::max(k, 1) * sizeof(double));
+}
+
+static float *scratch_f32(int which, int n, int k, long *ld)
+{
+    const long l = padded_ld(n);
+    *ld = l;
+    return (float *)scratch_bytes(4 + which, (size_t)l * std::max(k, 1) * sizeof(float));
 }
 
 static double dot_dev(int n, const double *x, const double *y)
@@ -581,6 +594,105 @@ struct CGParams {
 };
 static CGParams g_cg;
 
+// CG iterations of the block CG after the initial residual: P and W hold
+// float64 (the caller's blocks or backend scratch) or float32 (backend
+// scratch; the products A p then use the float32 values).
+//
+// The float32 directions are stored scaled, p = s p_stored, with one scale
+// per column chosen so that |p_stored| <= 1: s_new = |r| + |beta| s_old
+// |p_stored_old|.  The scalars of the iteration are those of the float64
+// form (alpha = rho / (s p_stored . A p_stored)), so both forms follow the
+// same iteration up to rounding -- including when the reference bookkeeping
+// (scalars stored by position in the list of unconverged columns, beta read
+// by column, src/gcge_cg.c:405,467-476) gives a large beta, which float64
+// absorbs and unscaled float32 directions would overflow.
+template <typename PT>
+static void bcg_iterate(Mat *A, Mat *B, int if_shift, double shift, int n, int len, double *Rd,
+                        long ldr, int r0, double *Xd, long ldx, int x0, PT *Pd, long ldp, int p0,
+                        PT *Wd, long ldw, int w0, PT *T, long ldt, int max_it, double rate,
+                        double *rho1, double *rho2, const double *err, double *last_err,
+                        int *unlock, int num_unlock)
+{
+    constexpr bool scaled = std::is_same<PT, float>::value;
+    State &s = state();
+    // device scalars: [beta | rho2 by position | s_old | s_new] uploaded,
+    // [ptw | p.p] and [r.r] computed
+    double *d_scal = small_buffer2((size_t)7 * len + 8);
+    double *d_beta = d_scal, *d_rho = d_scal + len, *d_sold = d_scal + 2 * len,
+           *d_snew = d_scal + 3 * len, *d_ptw = d_scal + 4 * len, *d_rr = d_scal + 6 * len;
+    double *h = result_buffer((size_t)2 * len);
+    int *d_maps = index_buffer((size_t)4 * len);  // p, r, x, w column lists
+    std::vector<int> maps((size_t)4 * len);
+    std::vector<double> coef((size_t)(scaled ? 4 : 2) * len);
+    std::vector<double> s_col(scaled ? len : 0, 1.0), pn_col(scaled ? len : 0, 0.0);
+    int niter = 1;
+    while (num_unlock > 0 && niter < max_it) {
+        const int nu = num_unlock;
+        for (int id = 0; id < nu; ++id) {
+            const int idx = unlock[id];
+            maps[id] = p0 + idx;            // p
+            maps[len + id] = r0 + idx;      // r
+            maps[2 * len + id] = x0 + idx;  // x
+            maps[3 * len + id] = w0 + idx;  // w
+            const double beta = (niter == 1) ? 0.0 : rho2[idx] / rho1[idx];  // by column
+            coef[id] = beta;
+            coef[len + id] = rho2[id];  // alpha numerator, by position
+            if (scaled) {
+                double snew = (niter == 1) ? last_err[idx]
+                                           : last_err[idx] + std::fabs(beta) * s_col[idx] * pn_col[idx];
+                if (!(snew > 0.0) && !std::isnan(snew)) snew = 1.0;
+                coef[2 * len + id] = s_col[idx];
+                coef[3 * len + id] = snew;
+                s_col[idx] = snew;
+            }
+        }
+        upload_async(d_maps, maps.data(), maps.size() * sizeof(int));
+        upload_async(d_beta, coef.data(), coef.size() * sizeof(double));
+        const int *pm = d_maps, *rm = d_maps + len, *xm = d_maps + 2 * len, *wm = d_maps + 3 * len;
+        if constexpr (scaled)
+            bcg_p_update(n, Pd, ldp, pm, Rd, ldr, rm, d_beta, d_sold, d_snew, niter == 1, nu);
+        else
+            bcg_p_update(n, Pd, ldp, pm, Rd, ldr, rm, d_beta, niter == 1, nu);
+        spmm(A, Pd, ldp, Wd, ldw, nu, pm, wm);
+        if (if_shift == 1) {
+            std::vector<double> sh(nu, shift);
+            double *d_sh = small_buffer((size_t)nu);
+            upload_async(d_sh, sh.data(), nu * sizeof(double));
+            if (!B) {
+                axpy_cols(n, d_sh, Pd, ldp, pm, Wd, ldw, wm, nu);
+            } else {
+                spmm(B, Pd, ldp, T, ldt, nu, pm, nullptr);
+                axpy_cols(n, d_sh, T, ldt, nullptr, Wd, ldw, wm, nu);
+            }
+        }
+        if constexpr (scaled) {
+            coldots_pw(n, Pd, ldp, pm, Wd, ldw, wm, nu, d_ptw);
+            bcg_xr_update(n, Xd, ldx, xm, Rd, ldr, rm, Pd, ldp, pm, Wd, ldw, wm, d_rho, d_ptw,
+                          d_snew, nu, d_rr);
+            HIP_CHECK(hipMemcpyAsync(h + len, d_ptw + nu, nu * sizeof(double), hipMemcpyDeviceToHost,
+                                     s.stream));
+        } else {
+            coldots(n, Pd, ldp, pm, Wd, ldw, wm, nu, d_ptw);
+            bcg_xr_update(n, Xd, ldx, xm, Rd, ldr, rm, Pd, ldp, pm, Wd, ldw, wm, d_rho, d_ptw, nu,
+                          d_rr);
+        }
+        HIP_CHECK(hipMemcpyAsync(h, d_rr, nu * sizeof(double), hipMemcpyDeviceToHost, s.stream));
+        HIP_CHECK(hipStreamSynchronize(s.stream));
+        for (int id = 0; id < nu; ++id) {
+            rho1[id] = rho2[id];
+            rho2[id] = h[id];
+            if (scaled) pn_col[unlock[id]] = std::sqrt(h[len + id]);
+        }
+        for (int id = 0; id < nu; ++id) last_err[unlock[id]] = std::sqrt(std::fabs(rho2[id]));
+        num_unlock = 0;
+        for (int id = 0; id < nu; ++id) {
+            const int idx = unlock[id];
+            if (last_err[idx] >= rate * err[idx] && last_err[idx] >= 1e-15) unlock[num_unlock++] = idx;
+        }
+        niter++;
+    }
+}
+
 static void block_cg(void *Amat, int if_shift, double shift, void *Bmat, void **RHS, void **V,
                      int *start, int *end, int max_it, double rate, void **CG_P, void **CG_W,
                      double *dtmp, int *itmp)
@@ -593,16 +705,19 @@ static void block_cg(void *Amat, int if_shift, double shift, void *Bmat, void **
     if (len <= 0) return;
     check_range(R, start[0], len, "BlockCG rhs");
     check_range(X, start[1], len, "BlockCG solution");
+    // float32 directions when the operator (and B, for a shifted solve) has
+    // float32 values
+    const bool f32 = A->fp32 && A->d_val32 && (if_shift != 1 || !B || (B->fp32 && B->d_val32));
     // directions and A p: caller's blocks when given, else backend scratch
-    double *Pd, *Wd;
-    long ldp, ldw;
+    double *Pd = nullptr, *Wd;
+    long ldp = 0, ldw;
     int p0 = start[2], w0 = start[3];
     if (CG_P) {
         MultiVec *P = as_mv(CG_P);
         check_range(P, start[2], len, "BlockCG directions");
         Pd = P->data;
         ldp = P->ld;
-    } else {
+    } else if (!f32) {
         Pd = scratch(0, n, len, &ldp);
         p0 = 0;
     }
@@ -638,7 +753,7 @@ static void block_cg(void *Amat, int if_shift, double shift, void *Bmat, void **
     double *Rd = R->data, *Xd = X->data;
     const long ldr = R->ld, ldx = X->ld;
 
-    // r = A x (+ shift B x) into W, b -= r, rho2 = b.b   (all columns)
+    // r = A x (+ shift B x) into W, b -= r, rho2 = b.b   (all columns, float64)
     spmm(A, Xd + (long)start[1] * ldx, ldx, Wd + (long)w0 * ldw, ldw, len);
     if (if_shift == 1) {
         if (!B) {
@@ -661,54 +776,18 @@ static void block_cg(void *Amat, int if_shift, double shift, void *Bmat, void **
         last_err[i] = err[i];
         if (err[i] >= 1e-15) unlock[num_unlock++] = i;
     }
-    // device scratch: maps (4 x len ints), beta, rho2 at position, ptw, new r.r
-    int *d_maps = index_buffer((size_t)4 * len);
-    double *d_beta = d_scal, *d_rho = d_scal + len, *d_ptw = d_scal + 2 * len, *d_rr = d_scal + 3 * len;
-    std::vector<int> maps((size_t)4 * len);
-    std::vector<double> coef((size_t)2 * len);
-    int niter = 1;
-    while (num_unlock > 0 && niter < max_it) {
-        const int nu = num_unlock;
-        for (int id = 0; id < nu; ++id) {
-            const int idx = unlock[id];
-            maps[id] = p0 + idx;                // p
-            maps[len + id] = start[0] + idx;    // r
-            maps[2 * len + id] = start[1] + idx;  // x
-            maps[3 * len + id] = w0 + idx;      // w
-            coef[id] = (niter == 1) ? 0.0 : rho2[idx] / rho1[idx];  // beta, indexed by column
-            coef[len + id] = rho2[id];                              // alpha numerator, by position
-        }
-        upload_async(d_maps, maps.data(), maps.size() * sizeof(int));
-        upload_async(d_beta, coef.data(), coef.size() * sizeof(double));
-        const int *pm = d_maps, *rm = d_maps + len, *xm = d_maps + 2 * len, *wm = d_maps + 3 * len;
-        bcg_p_update(n, Pd, ldp, pm, Rd, ldr, rm, d_beta, niter == 1, nu);
-        spmm(A, Pd, ldp, Wd, ldw, nu, pm, wm);
-        if (if_shift == 1) {
-            std::vector<double> sh(nu, shift);
-            double *d_sh = small_buffer((size_t)nu);
-            upload_async(d_sh, sh.data(), nu * sizeof(double));
-            if (!B) {
-                axpy_cols(n, d_sh, Pd, ldp, pm, Wd, ldw, wm, nu);
-            } else {
-                spmm(B, Pd, ldp, T, ldt, nu, pm, nullptr);
-                axpy_cols(n, d_sh, T, ldt, nullptr, Wd, ldw, wm, nu);
-            }
-        }
-        coldots(n, Pd, ldp, pm, Wd, ldw, wm, nu, d_ptw);
-        bcg_xr_update(n, Xd, ldx, xm, Rd, ldr, rm, Pd, ldp, pm, Wd, ldw, wm, d_rho, d_ptw, nu, d_rr);
-        HIP_CHECK(hipMemcpyAsync(h, d_rr, nu * sizeof(double), hipMemcpyDeviceToHost, s.stream));
-        HIP_CHECK(hipStreamSynchronize(s.stream));
-        for (int id = 0; id < nu; ++id) {
-            rho1[id] = rho2[id];
-            rho2[id] = h[id];
-        }
-        for (int id = 0; id < nu; ++id) last_err[unlock[id]] = std::sqrt(std::fabs(rho2[id]));
-        num_unlock = 0;
-        for (int id = 0; id < nu; ++id) {
-            const int idx = unlock[id];
-            if (last_err[idx] >= rate * err[idx] && last_err[idx] >= 1e-15) unlock[num_unlock++] = idx;
-        }
-        niter++;
+    if (f32) {
+        long lpf, lwf, ltf = 0;
+        float *Pf = scratch_f32(0, n, len, &lpf);
+        float *Wf = scratch_f32(1, n, len, &lwf);
+        float *Tf = (if_shift == 1 && B) ? scratch_f32(2, n, len, &ltf) : nullptr;
+        bcg_iterate<float>(A, B, if_shift, shift, n, len, Rd, ldr, start[0], Xd, ldx, start[1], Pf,
+                           lpf, 0, Wf, lwf, 0, Tf, ltf, max_it, rate, rho1, rho2, err, last_err,
+                           unlock, num_unlock);
+    } else {
+        bcg_iterate<double>(A, B, if_shift, shift, n, len, Rd, ldr, start[0], Xd, ldx, start[1], Pd,
+                            ldp, p0, Wd, ldw, w0, T, ldt, max_it, rate, rho1, rho2, err, last_err,
+                            unlock, num_unlock);
     }
 }
 
@@ -901,7 +980,7 @@ void GCGE_HIP_Finalize(void)
     State &s = state();
     if (!s.stream) return;
     HIP_CHECK(hipStreamSynchronize(s.stream));
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < kScratchSlots; ++i)
         if (g_scratch[i].data) {
             HIP_CHECK(hipFree(g_scratch[i].data));
             g_scratch[i] = Scratch();
@@ -965,7 +1044,17 @@ void GCGE_HIP_MatDestroy(void *mat)
     free(m);
 }
 
-void GCGE_HIP_MatSetPrecision(void *mat, int fp32) { as_mat(mat)->fp32 = fp32 ? 1 : 0; }
+void GCGE_HIP_MatSetPrecision(void *mat, int fp32)
+{
+    Mat *m = as_mat(mat);
+    if (fp32 && !m->d_val32) {
+        const size_t len = std::max<size_t>(m->sell_len, 1);
+        HIP_CHECK(hipMalloc((void **)&m->d_val32, len * sizeof(float)));
+        to_float(m->d_val, m->d_val32, len);
+        HIP_CHECK(hipStreamSynchronize(state().stream));
+    }
+    m->fp32 = fp32 ? 1 : 0;
+}
 
 void **GCGE_HIP_MultiVecCreate(int n, int ncols) { return (void **)create_mv(n, ncols); }
 void GCGE_HIP_MultiVecDestroy(void **mv)
@@ -1018,6 +1107,14 @@ void GCGE_HIP_SetLinearSolverParams(int max_it, double rate, int if_shift, doubl
     g_cg.if_shift = if_shift;
     g_cg.shift = shift;
     g_cg.B = B;
+}
+
+void GCGE_HIP_MultiLinearSolve(void *mat, void **b, int bs, void **x, int xs, int k)
+{
+    require_init();
+    int start[5] = {bs, xs, 0, 0, 0}, end[5] = {bs + k, xs + k, k, k, 0};
+    op_multi_linear_solver(mat, b, x, start, end, nullptr);
+    HIP_CHECK(hipStreamSynchronize(state().stream));
 }
 
 const void *GCGE_HIP_Backend(void) { return &g_backend; }

@@ -105,6 +105,10 @@ Vec *as_vec(void *handle);
 // Y(:, ymap[c]) = A X(:, xmap[c]) for c < k; maps may be null (identity).
 void spmm(const Mat *A, const double *X, long ldx, double *Y, long ldy, int k,
           const int *d_xmap = nullptr, const int *d_ymap = nullptr);
+// float32 form on the float32 copy of the values (Mat::d_val32)
+void spmm(const Mat *A, const float *X, long ldx, float *Y, long ldy, int k,
+          const int *d_xmap = nullptr, const int *d_ymap = nullptr);
+void to_float(const double *src, float *dst, size_t len);
 // host G(m x k, ldg) = V^T W
 void gram(int n, const double *V, long ldv, int m, const double *W, long ldw, int k, double *G,
           int ldg, bool sync_result = true);
@@ -125,14 +129,29 @@ void coldots(int n, const double *A, long lda, const int *d_amap, const double *
 // residual norms of columns c < k: |AX - lam BX|^2, |X|^2, |BX|^2 -> d_out[3k]
 void residual_sq(int n, const double *AX, long ldax, const double *BX, long ldbx, const double *X,
                  long ldx, const double *d_lam, int k, double *d_out);
-// block CG pieces (column lists on the device)
+// block CG pieces (column lists on the device); the direction block P and
+// its product W are float64 or float32, x and r stay float64.  The float32
+// directions are stored scaled, p = s p_stored with per-column s (d_sold,
+// d_snew, d_scale: one entry per listed column).
 void bcg_p_update(int n, double *P, long ldp, const int *d_pmap, const double *R, long ldr,
                   const int *d_rmap, const double *d_beta, int first, int k);
+void bcg_p_update(int n, float *P, long ldp, const int *d_pmap, const double *R, long ldr,
+                  const int *d_rmap, const double *d_beta, const double *d_sold,
+                  const double *d_snew, int first, int k);
+// d_out[c] = p_c . w_c, d_out[k + c] = p_c . p_c (float32 blocks, float64 sums)
+void coldots_pw(int n, const float *P, long ldp, const int *d_pmap, const float *W, long ldw,
+                const int *d_wmap, int k, double *d_out);
 void bcg_xr_update(int n, double *X, long ldx, const int *d_xmap, double *R, long ldr,
                    const int *d_rmap, const double *P, long ldp, const int *d_pmap,
                    const double *W, long ldw, const int *d_wmap, const double *d_rho2,
                    const double *d_ptw, int k, double *d_rr_out);
+void bcg_xr_update(int n, double *X, long ldx, const int *d_xmap, double *R, long ldr,
+                   const int *d_rmap, const float *P, long ldp, const int *d_pmap, const float *W,
+                   long ldw, const int *d_wmap, const double *d_rho2, const double *d_ptw,
+                   const double *d_scale, int k, double *d_rr_out);
 void axpy_cols(int n, const double *d_alpha, const double *X, long ldx, const int *d_xmap,
                double *Y, long ldy, const int *d_ymap, int k);
+void axpy_cols(int n, const double *d_alpha, const float *X, long ldx, const int *d_xmap,
+               float *Y, long ldy, const int *d_ymap, int k);
 
 }  // namespace gcge_hip

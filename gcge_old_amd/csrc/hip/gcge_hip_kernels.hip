@@ -1,12 +1,14 @@
 // Device kernels of the GCG backend for gfx950 (CDNA4, 64-wide wavefronts).
 //
-//   spmm        sliced-ELLPACK (64-row slices) times a column block; one
+//   spmm        sliced-ELLPACK (64-row slices) times a column block (float64,
+//               or float32 values and vectors for the inner solves); one
 //               wavefront per slice, each lane one row, register blocking
 //               over 4 columns; entries of a slice are stored column-major so
 //               that every entry load is one 512-byte coalesced access, and
 //               workgroups are remapped so each XCD sweeps a contiguous band
 //               of rows (its L2 keeps the band of X that the stencil reuses).
-//               HBM-bound: nnz*(8+4) + n*k*8*2 bytes per launch.
+//               HBM-bound: nnz*(8+4) + n*k*8*2 bytes per launch (float64),
+//               nnz*(4+4) + n*k*4*2 (float32).
 //   gram        V^T W for tall-skinny column-major blocks: 64x64 (or 16-wide)
 //               output tiles, 32-row steps staged through LDS (transposed),
 //               4x4 register blocks per thread, split over row chunks with a
@@ -62,24 +64,26 @@ __device__ __forceinline__ double block_sum(double v, double *lds)
 
 // ---- SpMM ------------------------------------------------------------------------
 // One chunk of CK consecutive output columns for the lane's row; the slice's
-// entries are re-read per chunk (from L1/L2 after the first chunk).
-template <int CK, typename VT>
+// entries are re-read per chunk (from L1/L2 after the first chunk).  VT is the
+// stored value type, XT the vector and accumulation type (double, double) for
+// the float64 form, (float, float) for the float32 inner solves.
+template <int CK, typename VT, typename XT>
 __device__ __forceinline__ void sell_chunk(int row, int n, int w, const int *__restrict__ cp,
                                            const VT *__restrict__ vp,
-                                           const double *__restrict__ X, long ldx,
-                                           double *__restrict__ Y, long ldy, int c0,
+                                           const XT *__restrict__ X, long ldx,
+                                           XT *__restrict__ Y, long ldy, int c0,
                                            const int *__restrict__ xmap,
                                            const int *__restrict__ ymap)
 {
-    const double *xc[CK];
+    const XT *xc[CK];
 #pragma unroll
     for (int cc = 0; cc < CK; ++cc) xc[cc] = X + (long)(xmap ? xmap[c0 + cc] : c0 + cc) * ldx;
-    double acc[CK];
+    XT acc[CK];
 #pragma unroll
-    for (int cc = 0; cc < CK; ++cc) acc[cc] = 0.0;
+    for (int cc = 0; cc < CK; ++cc) acc[cc] = XT(0);
     for (int j = 0; j < w; ++j) {
         const int col = cp[(long)j * kWave];
-        const double v = (double)vp[(long)j * kWave];
+        const XT v = (XT)vp[(long)j * kWave];
 #pragma unroll
         for (int cc = 0; cc < CK; ++cc) acc[cc] += v * xc[cc][col];
     }
@@ -89,14 +93,14 @@ __device__ __forceinline__ void sell_chunk(int row, int n, int w, const int *__r
     }
 }
 
-template <typename VT>
+template <typename VT, typename XT>
 __global__ __launch_bounds__(kBlock) void k_sell_spmm(int n, int nslices,
                                                      const int *__restrict__ sptr,
                                                      const int *__restrict__ sw,
                                                      const int *__restrict__ scol,
                                                      const VT *__restrict__ sval,
-                                                     const double *__restrict__ X, long ldx,
-                                                     double *__restrict__ Y, long ldy, int k,
+                                                     const XT *__restrict__ X, long ldx,
+                                                     XT *__restrict__ Y, long ldy, int k,
                                                      const int *__restrict__ xmap,
                                                      const int *__restrict__ ymap)
 {
@@ -120,6 +124,13 @@ __global__ __launch_bounds__(kBlock) void k_sell_spmm(int n, int nslices,
         c0 += 2;
     }
     if (c0 < k) sell_chunk<1>(row, n, w, cp, vp, X, ldx, Y, ldy, c0, xmap, ymap);
+}
+
+__global__ __launch_bounds__(kBlock) void k_to_float(size_t len, const double *__restrict__ src,
+                                                    float *__restrict__ dst)
+{
+    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < len) dst[i] = (float)src[i];
 }
 
 // ---- Gram -----------------------------------------------------------------------
@@ -304,21 +315,22 @@ __global__ __launch_bounds__(kBlock) void k_random(int n, double *__restrict__ X
     X[(long)c * ldx + row] = (double)(bits >> 11) * (2.0 / 9007199254740992.0) - 1.0;
 }
 
-// per-column dot products, partials[c * nbx + bx]
+// per-column dot products, partials[c * nbx + bx], accumulated in float64
+template <typename TA, typename TB>
 __global__ __launch_bounds__(kBlock) void k_coldots(int n, int rows_per_block,
-                                                   const double *__restrict__ A, long lda,
+                                                   const TA *__restrict__ A, long lda,
                                                    const int *__restrict__ amap,
-                                                   const double *__restrict__ B, long ldb,
+                                                   const TB *__restrict__ B, long ldb,
                                                    const int *__restrict__ bmap,
                                                    double *__restrict__ partial)
 {
     __shared__ double lds[kBlock / kWave];
     const int bx = blockIdx.x, c = blockIdx.y;
-    const double *a = A + (long)(amap ? amap[c] : c) * lda;
-    const double *b = B + (long)(bmap ? bmap[c] : c) * ldb;
+    const TA *a = A + (long)(amap ? amap[c] : c) * lda;
+    const TB *b = B + (long)(bmap ? bmap[c] : c) * ldb;
     const int r0 = bx * rows_per_block, r1 = min(n, r0 + rows_per_block);
     double s = 0.0;
-    for (int r = r0 + threadIdx.x; r < r1; r += kBlock) s += a[r] * b[r];
+    for (int r = r0 + threadIdx.x; r < r1; r += kBlock) s += (double)a[r] * (double)b[r];
     s = block_sum(s, lds);
     if (threadIdx.x == 0) partial[(long)c * gridDim.x + bx] = s;
 }
@@ -367,47 +379,91 @@ __global__ __launch_bounds__(kBlock) void k_residual(int n, int rows_per_block,
     }
 }
 
-// p = r (first) or p = r + beta p, column lists
-__global__ __launch_bounds__(kBlock) void k_bcg_p(int n, double *__restrict__ P, long ldp,
+// p = r (first) or p = r + beta p, column lists.  float64 P: unscaled.
+// float32 P holds p / s with a per-column scale: p_new = (r + beta s_old p) /
+// s_new, s_new chosen by the host so that |p_new| <= 1 (no overflow when
+// beta is large).
+template <typename PT>
+__global__ __launch_bounds__(kBlock) void k_bcg_p(int n, PT *__restrict__ P, long ldp,
                                                  const int *__restrict__ pmap,
                                                  const double *__restrict__ R, long ldr,
                                                  const int *__restrict__ rmap,
-                                                 const double *__restrict__ beta, int first)
+                                                 const double *__restrict__ beta,
+                                                 const double *__restrict__ s_old,
+                                                 const double *__restrict__ s_new, int first)
 {
     const int row = blockIdx.x * kBlock + threadIdx.x;
     const int c = blockIdx.y;
     if (row >= n) return;
-    double *p = P + (long)pmap[c] * ldp + row;
+    PT *p = P + (long)pmap[c] * ldp + row;
     const double r = R[(long)rmap[c] * ldr + row];
-    *p = first ? r : r + beta[c] * (*p);
+    if (!s_new) {
+        *p = (PT)(first ? r : r + beta[c] * (double)(*p));
+    } else {
+        const double v = first ? r : r + (beta[c] * s_old[c]) * (double)(*p);
+        *p = (PT)(v / s_new[c]);
+    }
+}
+
+// p.w and p.p per column (the float32 directions of the block CG),
+// partials[c * nbx + bx] and partials[(k + c) * nbx + bx]
+template <typename PT>
+__global__ __launch_bounds__(kBlock) void k_coldots_pw(int n, int rows_per_block,
+                                                      const PT *__restrict__ P, long ldp,
+                                                      const int *__restrict__ pmap,
+                                                      const PT *__restrict__ W, long ldw,
+                                                      const int *__restrict__ wmap,
+                                                      double *__restrict__ partial)
+{
+    __shared__ double lds[kBlock / kWave];
+    const int bx = blockIdx.x, c = blockIdx.y, nbx = gridDim.x, k = gridDim.y;
+    const PT *p = P + (long)pmap[c] * ldp;
+    const PT *w = W + (long)wmap[c] * ldw;
+    const int r0 = bx * rows_per_block, r1 = min(n, r0 + rows_per_block);
+    double s_pw = 0.0, s_pp = 0.0;
+    for (int r = r0 + threadIdx.x; r < r1; r += kBlock) {
+        const double pv = (double)p[r];
+        s_pw += pv * (double)w[r];
+        s_pp += pv * pv;
+    }
+    s_pw = block_sum(s_pw, lds);
+    s_pp = block_sum(s_pp, lds);
+    if (threadIdx.x == 0) {
+        partial[(long)c * nbx + bx] = s_pw;
+        partial[(long)(k + c) * nbx + bx] = s_pp;
+    }
 }
 
 // x += alpha p; r -= alpha w; partial r.r, with alpha = rho2[c] / ptw[c]
+// (float64 P), or alpha = rho2[c] / (scale[c] ptw[c]) for the scaled float32
+// directions (p = scale p_stored, ptw the stored p . A p_stored); x, r float64
+template <typename PT>
 __global__ __launch_bounds__(kBlock) void k_bcg_xr(int n, int rows_per_block,
                                                   double *__restrict__ X, long ldx,
                                                   const int *__restrict__ xmap,
                                                   double *__restrict__ R, long ldr,
                                                   const int *__restrict__ rmap,
-                                                  const double *__restrict__ P, long ldp,
+                                                  const PT *__restrict__ P, long ldp,
                                                   const int *__restrict__ pmap,
-                                                  const double *__restrict__ W, long ldw,
+                                                  const PT *__restrict__ W, long ldw,
                                                   const int *__restrict__ wmap,
                                                   const double *__restrict__ rho2,
                                                   const double *__restrict__ ptw,
+                                                  const double *__restrict__ scale,
                                                   double *__restrict__ partial)
 {
     __shared__ double lds[kBlock / kWave];
     const int bx = blockIdx.x, c = blockIdx.y;
-    const double alpha = rho2[c] / ptw[c];
+    const double alpha = scale ? rho2[c] / (scale[c] * ptw[c]) : rho2[c] / ptw[c];
     double *x = X + (long)xmap[c] * ldx;
     double *r = R + (long)rmap[c] * ldr;
-    const double *p = P + (long)pmap[c] * ldp;
-    const double *w = W + (long)wmap[c] * ldw;
+    const PT *p = P + (long)pmap[c] * ldp;
+    const PT *w = W + (long)wmap[c] * ldw;
     const int r0 = bx * rows_per_block, r1 = min(n, r0 + rows_per_block);
     double s = 0.0;
     for (int i = r0 + threadIdx.x; i < r1; i += kBlock) {
-        x[i] = alpha * p[i] + x[i];
-        const double rn = -alpha * w[i] + r[i];
+        x[i] = alpha * (double)p[i] + x[i];
+        const double rn = -alpha * (double)w[i] + r[i];
         r[i] = rn;
         s += rn * rn;
     }
@@ -415,16 +471,17 @@ __global__ __launch_bounds__(kBlock) void k_bcg_xr(int n, int rows_per_block,
     if (threadIdx.x == 0) partial[(long)c * gridDim.x + bx] = s;
 }
 
+template <typename T>
 __global__ __launch_bounds__(kBlock) void k_axpy_cols(int n, const double *__restrict__ alpha,
-                                                     const double *__restrict__ X, long ldx,
+                                                     const T *__restrict__ X, long ldx,
                                                      const int *__restrict__ xmap,
-                                                     double *__restrict__ Y, long ldy,
+                                                     T *__restrict__ Y, long ldy,
                                                      const int *__restrict__ ymap)
 {
     const int row = blockIdx.x * kBlock + threadIdx.x;
     const int c = blockIdx.y;
     if (row >= n) return;
-    Y[(long)(ymap ? ymap[c] : c) * ldy + row] += alpha[c] * X[(long)(xmap ? xmap[c] : c) * ldx + row];
+    Y[(long)(ymap ? ymap[c] : c) * ldy + row] += (T)alpha[c] * X[(long)(xmap ? xmap[c] : c) * ldx + row];
 }
 
 inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
@@ -452,9 +509,32 @@ void spmm(const Mat *A, const double *X, long ldx, double *Y, long ldy, int k, c
                          8.0 * (double)k * (A->ncols + A->nrows);
     ProfScope prof("spmm", bytes);
     const int nb = ceil_div(A->nslices, kBlock / kWave);
-    hipLaunchKernelGGL(k_sell_spmm<double>, dim3(nb), dim3(kBlock), 0, state().stream, A->nrows,
-                       A->nslices, A->d_slice_ptr, A->d_slice_w, A->d_col, A->d_val, X, ldx, Y,
-                       ldy, k, d_xmap, d_ymap);
+    hipLaunchKernelGGL((k_sell_spmm<double, double>), dim3(nb), dim3(kBlock), 0, state().stream,
+                       A->nrows, A->nslices, A->d_slice_ptr, A->d_slice_w, A->d_col, A->d_val, X,
+                       ldx, Y, ldy, k, d_xmap, d_ymap);
+    HIP_CHECK(hipGetLastError());
+}
+
+void spmm(const Mat *A, const float *X, long ldx, float *Y, long ldy, int k, const int *d_xmap,
+          const int *d_ymap)
+{
+    if (k <= 0 || A->nrows <= 0) return;
+    if (!A->d_val32) fail_msg("float32 SpMM on a matrix without float32 values");
+    const double bytes = (double)A->nnz * (4 + 4) + 4.0 * (A->nrows + 1) +
+                         4.0 * (double)k * (A->ncols + A->nrows);
+    ProfScope prof("spmm32", bytes);
+    const int nb = ceil_div(A->nslices, kBlock / kWave);
+    hipLaunchKernelGGL((k_sell_spmm<float, float>), dim3(nb), dim3(kBlock), 0, state().stream,
+                       A->nrows, A->nslices, A->d_slice_ptr, A->d_slice_w, A->d_col, A->d_val32,
+                       X, ldx, Y, ldy, k, d_xmap, d_ymap);
+    HIP_CHECK(hipGetLastError());
+}
+
+void to_float(const double *src, float *dst, size_t len)
+{
+    if (!len) return;
+    hipLaunchKernelGGL(k_to_float, dim3(ceil_div((long)len, kBlock)), dim3(kBlock), 0,
+                       state().stream, len, src, dst);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -565,18 +645,25 @@ void random_fill(int n, double *X, long ldx, int col0, int k, uint64_t seed, uin
     HIP_CHECK(hipGetLastError());
 }
 
-void coldots(int n, const double *A, long lda, const int *d_amap, const double *B, long ldb,
-             const int *d_bmap, int k, double *d_out)
+template <typename TA, typename TB>
+static void coldots_t(int n, const TA *A, long lda, const int *d_amap, const TB *B, long ldb,
+                      const int *d_bmap, int k, double *d_out)
 {
     if (k <= 0) return;
     const int rows = reduction_rows(n, k);
     const int nbx = std::max(1, ceil_div(n, rows));
     double *partial = partial_buffer((size_t)nbx * k);
     ProfScope prof("coldots");
-    hipLaunchKernelGGL(k_coldots, dim3(nbx, k), dim3(kBlock), 0, state().stream, n, rows, A, lda,
-                       d_amap, B, ldb, d_bmap, partial);
+    hipLaunchKernelGGL((k_coldots<TA, TB>), dim3(nbx, k), dim3(kBlock), 0, state().stream, n, rows,
+                       A, lda, d_amap, B, ldb, d_bmap, partial);
     hipLaunchKernelGGL(k_reduce_rows, dim3(k), dim3(kBlock), 0, state().stream, partial, nbx, d_out);
     HIP_CHECK(hipGetLastError());
+}
+
+void coldots(int n, const double *A, long lda, const int *d_amap, const double *B, long ldb,
+             const int *d_bmap, int k, double *d_out)
+{
+    coldots_t(n, A, lda, d_amap, B, ldb, d_bmap, k, d_out);
 }
 
 void residual_sq(int n, const double *AX, long ldax, const double *BX, long ldbx, const double *X,
@@ -594,13 +681,64 @@ void residual_sq(int n, const double *AX, long ldax, const double *BX, long ldbx
     HIP_CHECK(hipGetLastError());
 }
 
-void bcg_p_update(int n, double *P, long ldp, const int *d_pmap, const double *R, long ldr,
-                  const int *d_rmap, const double *d_beta, int first, int k)
+template <typename PT>
+static void bcg_p_t(int n, PT *P, long ldp, const int *d_pmap, const double *R, long ldr,
+                    const int *d_rmap, const double *d_beta, const double *d_sold,
+                    const double *d_snew, int first, int k)
 {
     if (k <= 0) return;
     ProfScope prof("bcg_p");
-    hipLaunchKernelGGL(k_bcg_p, dim3(ceil_div(n, kBlock), k), dim3(kBlock), 0, state().stream, n, P,
-                       ldp, d_pmap, R, ldr, d_rmap, d_beta, first);
+    hipLaunchKernelGGL((k_bcg_p<PT>), dim3(ceil_div(n, kBlock), k), dim3(kBlock), 0,
+                       state().stream, n, P, ldp, d_pmap, R, ldr, d_rmap, d_beta, d_sold, d_snew,
+                       first);
+    HIP_CHECK(hipGetLastError());
+}
+
+void bcg_p_update(int n, double *P, long ldp, const int *d_pmap, const double *R, long ldr,
+                  const int *d_rmap, const double *d_beta, int first, int k)
+{
+    bcg_p_t(n, P, ldp, d_pmap, R, ldr, d_rmap, d_beta, (const double *)nullptr,
+            (const double *)nullptr, first, k);
+}
+
+void bcg_p_update(int n, float *P, long ldp, const int *d_pmap, const double *R, long ldr,
+                  const int *d_rmap, const double *d_beta, const double *d_sold,
+                  const double *d_snew, int first, int k)
+{
+    bcg_p_t(n, P, ldp, d_pmap, R, ldr, d_rmap, d_beta, d_sold, d_snew, first, k);
+}
+
+void coldots_pw(int n, const float *P, long ldp, const int *d_pmap, const float *W, long ldw,
+                const int *d_wmap, int k, double *d_out)
+{
+    if (k <= 0) return;
+    const int rows = reduction_rows(n, k);
+    const int nbx = std::max(1, ceil_div(n, rows));
+    double *partial = partial_buffer((size_t)nbx * 2 * k);
+    ProfScope prof("coldots");
+    hipLaunchKernelGGL((k_coldots_pw<float>), dim3(nbx, k), dim3(kBlock), 0, state().stream, n,
+                       rows, P, ldp, d_pmap, W, ldw, d_wmap, partial);
+    hipLaunchKernelGGL(k_reduce_rows, dim3(2 * k), dim3(kBlock), 0, state().stream, partial, nbx,
+                       d_out);
+    HIP_CHECK(hipGetLastError());
+}
+
+template <typename PT>
+static void bcg_xr_t(int n, double *X, long ldx, const int *d_xmap, double *R, long ldr,
+                     const int *d_rmap, const PT *P, long ldp, const int *d_pmap, const PT *W,
+                     long ldw, const int *d_wmap, const double *d_rho2, const double *d_ptw,
+                     const double *d_scale, int k, double *d_rr_out)
+{
+    if (k <= 0) return;
+    const int rows = reduction_rows(n, k);
+    const int nbx = std::max(1, ceil_div(n, rows));
+    double *partial = partial_buffer((size_t)nbx * k);
+    ProfScope prof("bcg_xr");
+    hipLaunchKernelGGL((k_bcg_xr<PT>), dim3(nbx, k), dim3(kBlock), 0, state().stream, n, rows, X,
+                       ldx, d_xmap, R, ldr, d_rmap, P, ldp, d_pmap, W, ldw, d_wmap, d_rho2, d_ptw,
+                       d_scale, partial);
+    hipLaunchKernelGGL(k_reduce_rows, dim3(k), dim3(kBlock), 0, state().stream, partial, nbx,
+                       d_rr_out);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -609,27 +747,40 @@ void bcg_xr_update(int n, double *X, long ldx, const int *d_xmap, double *R, lon
                    const double *W, long ldw, const int *d_wmap, const double *d_rho2,
                    const double *d_ptw, int k, double *d_rr_out)
 {
+    bcg_xr_t(n, X, ldx, d_xmap, R, ldr, d_rmap, P, ldp, d_pmap, W, ldw, d_wmap, d_rho2, d_ptw,
+             (const double *)nullptr, k, d_rr_out);
+}
+
+void bcg_xr_update(int n, double *X, long ldx, const int *d_xmap, double *R, long ldr,
+                   const int *d_rmap, const float *P, long ldp, const int *d_pmap, const float *W,
+                   long ldw, const int *d_wmap, const double *d_rho2, const double *d_ptw,
+                   const double *d_scale, int k, double *d_rr_out)
+{
+    bcg_xr_t(n, X, ldx, d_xmap, R, ldr, d_rmap, P, ldp, d_pmap, W, ldw, d_wmap, d_rho2, d_ptw,
+             d_scale, k, d_rr_out);
+}
+
+template <typename T>
+static void axpy_cols_t(int n, const double *d_alpha, const T *X, long ldx, const int *d_xmap,
+                        T *Y, long ldy, const int *d_ymap, int k)
+{
     if (k <= 0) return;
-    const int rows = reduction_rows(n, k);
-    const int nbx = std::max(1, ceil_div(n, rows));
-    double *partial = partial_buffer((size_t)nbx * k);
-    ProfScope prof("bcg_xr");
-    hipLaunchKernelGGL(k_bcg_xr, dim3(nbx, k), dim3(kBlock), 0, state().stream, n, rows, X, ldx,
-                       d_xmap, R, ldr, d_rmap, P, ldp, d_pmap, W, ldw, d_wmap, d_rho2, d_ptw,
-                       partial);
-    hipLaunchKernelGGL(k_reduce_rows, dim3(k), dim3(kBlock), 0, state().stream, partial, nbx,
-                       d_rr_out);
+    ProfScope prof("axpy_cols");
+    hipLaunchKernelGGL((k_axpy_cols<T>), dim3(ceil_div(n, kBlock), k), dim3(kBlock), 0,
+                       state().stream, n, d_alpha, X, ldx, d_xmap, Y, ldy, d_ymap);
     HIP_CHECK(hipGetLastError());
 }
 
 void axpy_cols(int n, const double *d_alpha, const double *X, long ldx, const int *d_xmap,
                double *Y, long ldy, const int *d_ymap, int k)
 {
-    if (k <= 0) return;
-    ProfScope prof("axpy_cols");
-    hipLaunchKernelGGL(k_axpy_cols, dim3(ceil_div(n, kBlock), k), dim3(kBlock), 0, state().stream,
-                       n, d_alpha, X, ldx, d_xmap, Y, ldy, d_ymap);
-    HIP_CHECK(hipGetLastError());
+    axpy_cols_t(n, d_alpha, X, ldx, d_xmap, Y, ldy, d_ymap, k);
+}
+
+void axpy_cols(int n, const double *d_alpha, const float *X, long ldx, const int *d_xmap,
+               float *Y, long ldy, const int *d_ymap, int k)
+{
+    axpy_cols_t(n, d_alpha, X, ldx, d_xmap, Y, ldy, d_ymap, k);
 }
 
 }  // namespace gcge_hip

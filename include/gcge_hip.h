@@ -97,6 +97,12 @@ void GCGE_HIP_SetOps(GCGE_OPS *ops);
 /* Parameters MultiLinearSolver uses when a driver calls it without them
  * (reference defaults: cg_max_it 20, cg_rate 1e-2, no shift). */
 void GCGE_HIP_SetLinearSolverParams(int max_it, double rate, int if_shift, double shift, void *B);
+/* The table's MultiLinearSolver on columns [bs, bs+k) of b (right-hand
+ * sides, overwritten by the residuals as in src/gcge_cg.c:302-501) and
+ * [xs, xs+k) of x (initial guess in, solution out), with the parameters of
+ * GCGE_HIP_SetLinearSolverParams.  Runs the float32 direction form when the
+ * matrix was switched with GCGE_HIP_MatSetPrecision. */
+void GCGE_HIP_MultiLinearSolve(void *mat, void **b, int bs, void **x, int xs, int k);
 /* Descriptor {SetOps, MultiVecSetHost, MultiVecGetHost} for the one-call
  * driver GCGE_SolveWithBackend (gcge.h). */
 const void *GCGE_HIP_Backend(void);

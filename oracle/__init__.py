@@ -51,6 +51,8 @@ def lib() -> C.CDLL:
                                         C.c_double, _dp, C.c_int]
         L.ORACLE_RandomFill.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.c_int, _dp,
                                         C.c_int]
+        L.ORACLE_BlockCG.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.c_int, _dp,
+                                     C.c_int, _dp, C.c_int, C.c_int, C.c_double]
     return _lib
 
 
@@ -122,6 +124,21 @@ def random_fill(seed: int, draw: int, col0: int, ncols: int, n: int) -> np.ndarr
     return X
 
 
+def block_cg(A, B, rhs: np.ndarray, x0: np.ndarray, max_it: int = 20, rate: float = 1e-2,
+             if_shift: int = 0, shift: float = 0.0):
+    """The reference block CG (src/gcge_cg.c:302-501) restated in C: returns
+    (x, r) -- the solutions and the final residual block that the
+    reference leaves in its right-hand-side block.  A, B: problems.CSR."""
+    Am = Matrix(A)
+    Bm = Matrix(B) if B is not None else None
+    r = np.asfortranarray(rhs, dtype=np.float64).copy(order="F")
+    x = np.asfortranarray(x0, dtype=np.float64).copy(order="F")
+    n, k = r.shape
+    lib().ORACLE_BlockCG(Am.handle, if_shift, shift, None if Bm is None else Bm.handle, k, _d(r),
+                         n, _d(x), n, max_it, rate)
+    return x, r
+
+
 def solve(A, B, nev: int, flags: list[str] | None = None, x0: np.ndarray | None = None,
           seed: int = 1, history_cap: int = 0):
     """Run the solver core on the oracle CPU operations (column loops, serial
@@ -160,6 +177,35 @@ def _native_solve(_native, backend, Ah, Bh, n, nev, flags, x0, history_cap, keep
 
 
 REF_HIP_BIN = os.path.join(HERE, "_ref", "gcge_ref_hip")
+REF_BCG_BIN = os.path.join(HERE, "_ref", "gcge_ref_bcg")
+
+
+def ref_block_cg(A, B, rhs: np.ndarray, x0: np.ndarray, max_it: int = 20, rate: float = 1e-2,
+                 if_shift: int = 0, shift: float = 0.0):
+    """The reference GCGE_BCG built from its sources (oracle/_ref/gcge_ref_bcg)
+    on its CSR operations; returns (x, r) like block_cg."""
+    if not os.path.exists(REF_BCG_BIN):
+        raise RuntimeError(f"{REF_BCG_BIN} is not built (make -C oracle ref)")
+    n, k = rhs.shape
+    with tempfile.TemporaryDirectory() as d:
+        a_path = os.path.join(d, "A.bin")
+        A.write_bin(a_path)
+        b_path = "-"
+        if B is not None:
+            b_path = os.path.join(d, "B.bin")
+            B.write_bin(b_path)
+        r_path, x_path, out = (os.path.join(d, f) for f in ("rhs.bin", "x0.bin", "out.bin"))
+        np.asfortranarray(rhs, dtype=np.float64).T.tofile(r_path)
+        np.asfortranarray(x0, dtype=np.float64).T.tofile(x_path)
+        args = [REF_BCG_BIN, a_path, b_path, str(k), str(if_shift), repr(float(shift)),
+                str(max_it), repr(float(rate)), r_path, x_path, out]
+        p = subprocess.run(args, capture_output=True, text=True, timeout=600)
+        if p.returncode != 0:
+            raise RuntimeError(f"reference block CG failed: {p.stderr[-2000:]}")
+        data = np.fromfile(out, dtype=np.float64)
+    x = data[: n * k].reshape(k, n).T.copy(order="F")
+    r = data[n * k:].reshape(k, n).T.copy(order="F")
+    return x, r
 
 
 def ref_available() -> bool:

@@ -642,3 +642,85 @@ void ORACLE_RandomFill(uint64_t seed, uint64_t draw, int col0, int ncols, int n,
         for (int i = 0; i < n; ++i)
             X[i + (size_t)c * ldx] = ORACLE_RandomValue(seed, draw, (uint64_t)(col0 + c), (uint64_t)i);
 }
+
+/* Block CG of the reference (GCGE_BCG, src/gcge_cg.c:302-501) on arrays:
+ * k independent CG iterations on (A + shift B) x = b (B NULL: identity when
+ * if_shift is 1), started from x, stopped per column when the residual norm
+ * falls below rate times the initial one or below 1e-15, at most max_it - 1
+ * iterations.  rhs is overwritten by the final residuals, as the reference
+ * does when it runs without a separate residual block.  Bookkeeping follows
+ * the reference: the scalars rho1/rho2 are stored by position in the list of
+ * unconverged columns (src/gcge_cg.c:467-476), while beta reads them by
+ * column index (src/gcge_cg.c:405); both agree until a column drops out. */
+void ORACLE_BlockCG(const ORACLE_CSR *A, int if_shift, double shift, const ORACLE_CSR *B, int k,
+                    double *rhs, int ldb, double *x, int ldx, int max_it, double rate)
+{
+    const int n = A->nrows;
+    double *P = (double *)calloc((size_t)n * (k > 0 ? k : 1), sizeof(double));
+    double *W = (double *)calloc((size_t)n * (k > 0 ? k : 1), sizeof(double));
+    double *Bx = (double *)calloc((size_t)n, sizeof(double));
+    double *rho1 = (double *)calloc((size_t)5 * (k > 0 ? k : 1), sizeof(double));
+    double *rho2 = rho1 + k, *err = rho2 + k, *last = err + k, *ptw = last + k;
+    int *unlock = (int *)calloc((size_t)(k > 0 ? k : 1), sizeof(int));
+    int nu = 0;
+    for (int c = 0; c < k; ++c) {
+        double *xc = x + (size_t)c * ldx, *b = rhs + (size_t)c * ldb, *r = W + (size_t)c * n;
+        csr_matvec(A, xc, r);
+        if (if_shift == 1) {
+            if (!B) {
+                axpby_array(shift, xc, 1.0, r, n);
+            } else {
+                csr_matvec(B, xc, Bx);
+                axpby_array(shift, Bx, 1.0, r, n);
+            }
+        }
+        axpby_array(-1.0, r, 1.0, b, n);
+        rho2[c] = dot_array(b, b, n);
+    }
+    for (int c = 0; c < k; ++c) {
+        err[c] = sqrt(rho2[c]);
+        last[c] = err[c];
+        if (err[c] >= 1e-15) unlock[nu++] = c;
+    }
+    for (int it = 1; nu > 0 && it < max_it; ++it) {
+        for (int id = 0; id < nu; ++id) {
+            const int c = unlock[id];
+            double *p = P + (size_t)c * n, *w = W + (size_t)c * n, *r = rhs + (size_t)c * ldb;
+            if (it == 1)
+                axpby_array(1.0, r, 0.0, p, n);
+            else
+                axpby_array(1.0, r, rho2[c] / rho1[c], p, n);
+            csr_matvec(A, p, w);
+            if (if_shift == 1) {
+                if (!B) {
+                    axpby_array(shift, p, 1.0, w, n);
+                } else {
+                    csr_matvec(B, p, Bx);
+                    axpby_array(shift, Bx, 1.0, w, n);
+                }
+            }
+            ptw[id] = dot_array(p, w, n);
+        }
+        for (int id = 0; id < nu; ++id) {
+            const int c = unlock[id];
+            const double alpha = rho2[id] / ptw[id];
+            double *r = rhs + (size_t)c * ldb;
+            axpby_array(alpha, P + (size_t)c * n, 1.0, x + (size_t)c * ldx, n);
+            axpby_array(-alpha, W + (size_t)c * n, 1.0, r, n);
+            rho1[id] = rho2[id];
+            rho2[id] = dot_array(r, r, n);
+        }
+        for (int id = 0; id < nu; ++id) last[unlock[id]] = sqrt(fabs(rho2[id]));
+        const int old = nu;
+        nu = 0;
+        for (int id = 0; id < old; ++id) {
+            const int c = unlock[id];
+            if (last[c] >= rate * err[c] && last[c] >= 1e-15) unlock[nu++] = c;
+        }
+    }
+    free(P);
+    free(W);
+    free(Bx);
+    free(rho1);
+    free(unlock);
+}

@@ -44,6 +44,8 @@ void ORACLE_LinearComb(int n, int m, int k, const double *V, int ldv, const doub
                        double beta, double *Y, int ldy);
 void ORACLE_RandomFill(uint64_t seed, uint64_t draw, int col0, int ncols, int n, double *X,
                        int ldx);
+void ORACLE_BlockCG(const ORACLE_CSR *A, int if_shift, double shift, const ORACLE_CSR *B, int k,
+                    double *rhs, int ldb, double *x, int ldx, int max_it, double rate);
 
 #ifdef __cplusplus
 }

@@ -39,6 +39,34 @@ RUNS = [
 ]
 
 
+# (name, matrices, k, max_it, rate, if_shift, shift, zero column or -1, seed)
+BCG_RUNS = [
+    ("fd12_rate", "fd12", 4, 20, 1e-2, 0, 0.0, -1, 21),
+    ("fd12_zero_col", "fd12", 4, 20, 1e-2, 0, 0.0, 1, 22),
+    ("fd12_fixed_iters", "fd12", 3, 9, 1e-30, 0, 0.0, -1, 23),
+    ("q1e6_shift_B", "q1e6", 4, 20, 1e-2, 1, 3.5, -1, 24),
+    ("q1e6_shift_I", "q1e6A", 3, 20, 1e-3, 1, 2.0, -1, 25),
+]
+
+
+def bcg_case(spec, k, zero_col, seed):
+    """Matrices and inputs of one block CG fixture."""
+    if spec == "fd12":
+        A, B = problems.fd_laplacian_2d(12), None
+    elif spec == "q1e6":
+        A, B = problems.q1_fem_3d(6)
+    else:
+        A, B = problems.q1_fem_3d(6)[0], None
+    rng = np.random.default_rng(seed)
+    n = A.nrows
+    rhs = rng.standard_normal((n, k))
+    x0 = 0.01 * rng.standard_normal((n, k))
+    if zero_col >= 0:
+        rhs[:, zero_col] = 0.0
+        x0[:, zero_col] = 0.0
+    return A, B, np.asfortranarray(rhs), np.asfortranarray(x0)
+
+
 def load(name):
     d = np.load(os.path.join(HERE, "data", name + ".npz"))
     return problems.CSR(int(d["n"]), int(d["n"]), d["rowptr"], d["colidx"], d["vals"])
@@ -77,6 +105,18 @@ def main():
         out[name + "/eval_history"] = r["history"][:, 2:]
         print("ref run", name, r["num_iter"], r["num_conv"])
     np.savez_compressed(os.path.join(HERE, "ref_runs.npz"), **out)
+    out = {}
+    for name, spec, k, max_it, rate, if_shift, shift, zero_col, seed in BCG_RUNS:
+        A, B, rhs, x0 = bcg_case(spec, k, zero_col, seed)
+        x, r = oracle.ref_block_cg(A, B, rhs, x0, max_it, rate, if_shift, shift)
+        out[name + "/rhs"] = rhs
+        out[name + "/x0"] = x0
+        out[name + "/x"] = x
+        out[name + "/r"] = r
+        out[name + "/params"] = np.array([max_it, rate, if_shift, shift])
+        out[name + "/spec"] = np.array(spec)
+        print("ref block CG", name, int(np.isnan(x).any(axis=0).sum()), "NaN columns")
+    np.savez_compressed(os.path.join(HERE, "bcg_runs.npz"), **out)
 
 
 if __name__ == "__main__":

@@ -102,3 +102,31 @@ def test_q1_generalized_closed_form():
     G = X.T @ (M.to_scipy() @ X)
     assert np.abs(G - np.eye(30)).max() < 1e-10
     assert np.all(_residuals(K, M, out["eval"], X) <= 1.05e-9)
+
+
+def test_mixed_precision_q1_closed_form():
+    """Mixed-precision form (float32 inner block CG) on Q1 20^3, kappa = 1:
+    the same eigenvalues as the closed form, the float64 residual test met."""
+    K, M = problems.q1_fem_3d(20)
+    flags = ["-gcge_conv_type", "RB", "-gcge_ev_tol", "1e-9", "-gcge_ev_max_it", "100"]
+    out = gcge.solve(K, M, 30, flags, seed=5, precision="mixed")
+    assert out["num_conv"] == 30
+    np.testing.assert_allclose(np.sort(out["eval"]), problems.q1_eigs(20, 30), rtol=1e-10)
+    assert np.all(_residuals(K, M, out["eval"], out["evec"]) <= 1.05e-9)
+
+
+def test_mixed_precision_lognormal_against_fp64():
+    """Q1 16^3 with lognormal kappa (no closed form): every eigenvalue of the
+    mixed-precision run within 1e-9 relative of the float64 oracle run, and
+    the converged subspaces agree."""
+    E = 16
+    K, M = problems.q1_fem_3d(E, problems.lognormal_kappa(E, seed=3))
+    nev = 20
+    flags = ["-gcge_conv_type", "RB", "-gcge_ev_tol", "1e-8", "-gcge_ev_max_it", "200"]
+    x0 = oracle.random_fill(13, 0, 0, nev, K.nrows)
+    mixed = gcge.solve(K, M, nev, flags, x0=x0, precision="mixed")
+    ref = oracle.solve(K, M, nev, flags, x0=x0)
+    assert mixed["num_conv"] == nev and ref["num_conv"] == nev
+    np.testing.assert_allclose(mixed["eval"], ref["eval"], rtol=1e-9)
+    assert np.all(_residuals(K, M, mixed["eval"], mixed["evec"]) <= 1.05e-8)
+    assert _largest_angle(mixed["evec"], ref["evec"], M) < 1e-5

@@ -213,3 +213,119 @@ def test_live_reference_small_block_spurious_zeros():
     assert r["num_iter"] == o["num_iter"] == 30
     assert r["num_conv"] == o["num_conv"] == 0
     assert np.sum(np.abs(r["eval"]) < 1e-9) == np.sum(np.abs(o["eval"]) < 1e-9) == 4
+
+
+# ---- block CG ------------------------------------------------------------------
+
+BCG_FIXTURES = ["fd12_rate", "fd12_zero_col", "fd12_fixed_iters", "q1e6_shift_B", "q1e6_shift_I"]
+
+
+def bcg_matrices(spec):
+    """The matrices named in bcg_runs.npz."""
+    if spec == "fd12":
+        return problems.fd_laplacian_2d(12), None
+    if spec == "q1e6":
+        return problems.q1_fem_3d(6)
+    if spec == "q1e6A":
+        return problems.q1_fem_3d(6)[0], None
+    raise ValueError(spec)
+
+
+@pytest.mark.parametrize("case", BCG_FIXTURES)
+def test_block_cg_matches_reference_fixture(case, golden_dir):
+    """The restated block CG against the reference GCGE_BCG outputs, bit for
+    bit, NaN columns included (a column that starts converged shifts the
+    position-indexed scalars of the reference bookkeeping)."""
+    g = np.load(os.path.join(golden_dir, "bcg_runs.npz"))
+    max_it, rate, if_shift, shift = g[case + "/params"]
+    A, B = bcg_matrices(str(g[case + "/spec"]))
+    x, r = oracle.block_cg(A, B, g[case + "/rhs"], g[case + "/x0"], int(max_it), float(rate),
+                           int(if_shift), float(shift))
+    np.testing.assert_array_equal(x, g[case + "/x"])
+    np.testing.assert_array_equal(r, g[case + "/r"])
+    if case == "fd12_zero_col":
+        assert np.isnan(x).any()
+
+
+def test_block_cg_exact_on_small_spd():
+    """CG on an n x n SPD system reaches the solution in n steps; the
+    returned residual block is b - (A + shift B) x."""
+    import scipy.sparse as sp
+
+    rng = np.random.default_rng(4)
+    n = 6
+    Q = rng.standard_normal((n, n))
+    S = Q @ Q.T + n * np.eye(n)
+    A = problems.CSR.from_scipy(sp.csr_matrix(S))
+    Bd = np.diag(rng.uniform(1, 2, n))
+    B = problems.CSR.from_scipy(sp.csr_matrix(Bd))
+    b = rng.standard_normal((n, 3))
+    for shift_args, Mop in (((0, 0.0), S), ((1, 0.7), S + 0.7 * Bd)):
+        x, r = oracle.block_cg(A, B, b, np.zeros((n, 3)), max_it=n + 1, rate=1e-30,
+                               if_shift=shift_args[0], shift=shift_args[1])
+        np.testing.assert_allclose(x, np.linalg.solve(Mop, b), rtol=1e-10, atol=1e-12)
+        np.testing.assert_allclose(r, b - Mop @ x, atol=1e-10)
+    # identity shift when B is absent
+    x, _ = oracle.block_cg(A, None, b, np.zeros((n, 3)), max_it=n + 1, rate=1e-30, if_shift=1,
+                           shift=1.5)
+    np.testing.assert_allclose(x, np.linalg.solve(S + 1.5 * np.eye(n), b), rtol=1e-10)
+
+
+def test_block_cg_stopping_rule():
+    """A column stops once ||r|| < rate ||r0||; the first step from x0 = 0
+    is the steepest-descent step.  (With several columns, those left after
+    the first column stops read another column's scalars -- the reference
+    bookkeeping -- and may stagnate; the fixtures above pin that case.)"""
+    A = problems.fd_laplacian_2d(16)
+    rng = np.random.default_rng(5)
+    b = rng.standard_normal((A.nrows, 4))
+    for c in range(4):
+        bc = b[:, c:c + 1]
+        x, r = oracle.block_cg(A, None, bc, np.zeros_like(bc), max_it=200, rate=1e-3)
+        ratio = np.linalg.norm(r) / np.linalg.norm(bc)
+        assert ratio < 1e-3
+        np.testing.assert_allclose(r, bc - A.to_scipy() @ x, atol=1e-10)
+        # the step before the last did not meet the rate
+        _, r_prev = oracle.block_cg(A, None, bc, np.zeros_like(bc), max_it=_steps(A, bc), rate=1e-3)
+        assert np.linalg.norm(r_prev) / np.linalg.norm(bc) >= 1e-3
+    _, r1 = oracle.block_cg(A, None, b, np.zeros_like(b), max_it=2, rate=1e-3)
+    Am = A.to_scipy()
+    alpha = (b * b).sum(0) / (b * (Am @ b)).sum(0)
+    np.testing.assert_allclose(r1, b - alpha * (Am @ b), rtol=1e-12, atol=1e-12)
+
+
+def _steps(A, b):
+    """max_it that allows one step fewer than the single-column CG needs to
+    reach rate 1e-3 (plain textbook CG count)."""
+    Am = A.to_scipy()
+    x = np.zeros_like(b)
+    r = b.copy()
+    p = r.copy()
+    rr = float((r * r).sum())
+    r0 = np.sqrt(rr)
+    steps = 0
+    while np.sqrt(rr) >= 1e-3 * r0:
+        w = Am @ p
+        a = rr / float((p * w).sum())
+        x += a * p
+        r -= a * w
+        rr_new = float((r * r).sum())
+        p = r + rr_new / rr * p
+        rr = rr_new
+        steps += 1
+    return steps  # max_it = steps allows steps - 1 iterations
+
+
+@pytest.mark.skipif(not os.path.exists(oracle.REF_BCG_BIN), reason="reference build absent")
+def test_live_reference_block_cg():
+    """Restatement against the reference GCGE_BCG on a larger shifted
+    generalized system with mixed convergence speeds."""
+    K, M = problems.q1_fem_3d(9)
+    rng = np.random.default_rng(6)
+    b = rng.standard_normal((K.nrows, 6)) * np.logspace(0, -3, 6)
+    x0 = 1e-3 * rng.standard_normal((K.nrows, 6))
+    for args in ((20, 1e-2, 0, 0.0), (30, 1e-4, 1, 12.0)):
+        x, r = oracle.block_cg(K, M, b, x0, *args)
+        xr, rr = oracle.ref_block_cg(K, M, b, x0, *args)
+        np.testing.assert_array_equal(x, xr)
+        np.testing.assert_array_equal(r, rr)
