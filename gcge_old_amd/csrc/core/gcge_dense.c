@@ -153,6 +153,13 @@ void GCGE_Default_ArrayScale(GCGE_DOUBLE alpha, GCGE_DOUBLE *a, GCGE_INT length)
 }
 
 /* ---- symmetric eigensolver -------------------------------------------- */
+/* Householder tridiagonalisation, explicit accumulation of the orthogonal
+ * factor and implicit-shift QL on the tridiagonal, all on column-major
+ * storage so that every inner loop runs over one contiguous column (the
+ * matrix-vector product, the rank-2 update, the reflector applications and
+ * the plane rotations).  Backward stable; eigenvalues agree with LAPACK
+ * dsyev to rounding, eigenvectors up to sign and the choice of basis inside
+ * multiple eigenvalues. */
 
 static double hypot_safe(double a, double b)
 {
@@ -166,78 +173,84 @@ static double hypot_safe(double a, double b)
     return bb * sqrt(1.0 + r * r);
 }
 
-/* Householder tridiagonalisation of the symmetric n x n matrix z (row-major
- * z[i*n+j], full storage).  On return d holds the diagonal, e[1..n-1] the
- * sub-diagonal (e[0] = 0) and z the accumulated orthogonal transformation,
- * whose column j is basis vector j. */
-static void tridiagonalize(int n, double *z, double *d, double *e)
+/* A (n x n, column-major, full symmetric) -> tridiagonal (d, e) with
+ * A = Q T Q^T; e[i] couples i and i+1.  The Householder vectors are left in
+ * the columns of A below the subdiagonal (unit leading entry implied),
+ * their factors in tau. */
+static void sym_tridiag(int n, double *A, double *d, double *e, double *tau, double *p)
 {
-#define Z(i, j) z[(size_t)(i) * n + (j)]
-    for (int i = n - 1; i > 0; --i) {
-        int l = i - 1;
-        double h = 0.0;
-        if (l > 0) {
-            double scale = 0.0;
-            for (int k = 0; k <= l; ++k) scale += fabs(Z(i, k));
-            if (scale == 0.0) {
-                e[i] = Z(i, l);
-            } else {
-                for (int k = 0; k <= l; ++k) {
-                    Z(i, k) /= scale;
-                    h += Z(i, k) * Z(i, k);
-                }
-                double f = Z(i, l);
-                double g = (f >= 0.0) ? -sqrt(h) : sqrt(h);
-                e[i] = scale * g;
-                h -= f * g;
-                Z(i, l) = f - g;
-                f = 0.0;
-                for (int j = 0; j <= l; ++j) {
-                    Z(j, i) = Z(i, j) / h;
-                    double gg = 0.0;
-                    for (int k = 0; k <= j; ++k) gg += Z(j, k) * Z(i, k);
-                    for (int k = j + 1; k <= l; ++k) gg += Z(k, j) * Z(i, k);
-                    e[j] = gg / h;
-                    f += e[j] * Z(i, j);
-                }
-                double hh = f / (h + h);
-                for (int j = 0; j <= l; ++j) {
-                    double fj = Z(i, j);
-                    double gj = e[j] - hh * fj;
-                    e[j] = gj;
-                    for (int k = 0; k <= j; ++k) Z(j, k) -= (fj * e[k] + gj * Z(i, k));
-                }
-            }
-        } else {
-            e[i] = Z(i, l);
+#define AC(i, j) A[(size_t)(j) * n + (i)]
+    for (int j = 0; j < n - 2; ++j) {
+        const int m = n - j - 1; /* length of the column below the diagonal */
+        double *x = &AC(j + 1, j);
+        double alpha = x[0], xnorm = 0.0;
+        for (int i = 1; i < m; ++i) xnorm = hypot_safe(xnorm, x[i]);
+        d[j] = AC(j, j);
+        if (xnorm == 0.0) {
+            tau[j] = 0.0;
+            e[j] = alpha;
+            continue;
         }
-        d[i] = h;
-    }
-    d[0] = 0.0;
-    e[0] = 0.0;
-    for (int i = 0; i < n; ++i) {
-        int l = i - 1;
-        if (d[i] != 0.0) {
-            for (int j = 0; j <= l; ++j) {
-                double g = 0.0;
-                for (int k = 0; k <= l; ++k) g += Z(i, k) * Z(k, j);
-                for (int k = 0; k <= l; ++k) Z(k, j) -= g * Z(k, i);
-            }
+        double beta = -copysign(hypot_safe(alpha, xnorm), alpha);
+        tau[j] = (beta - alpha) / beta;
+        const double scal = 1.0 / (alpha - beta);
+        for (int i = 1; i < m; ++i) x[i] *= scal;
+        x[0] = 1.0;
+        e[j] = beta;
+        /* p = tau A22 v, A22 = A(j+1:, j+1:) */
+        for (int i = 0; i < m; ++i) p[i] = 0.0;
+        for (int c = 0; c < m; ++c) {
+            const double vc = tau[j] * x[c];
+            const double *col = &AC(j + 1, j + 1 + c);
+            for (int i = 0; i < m; ++i) p[i] += col[i] * vc;
         }
-        d[i] = Z(i, i);
-        Z(i, i) = 1.0;
-        for (int j = 0; j <= l; ++j) Z(j, i) = Z(i, j) = 0.0;
+        /* w = p - (tau/2)(p.v) v, stored in p */
+        double pv = 0.0;
+        for (int i = 0; i < m; ++i) pv += p[i] * x[i];
+        const double h = 0.5 * tau[j] * pv;
+        for (int i = 0; i < m; ++i) p[i] -= h * x[i];
+        /* A22 -= v w^T + w v^T */
+        for (int c = 0; c < m; ++c) {
+            double *col = &AC(j + 1, j + 1 + c);
+            const double wc = p[c], vc = x[c];
+            for (int i = 0; i < m; ++i) col[i] -= x[i] * wc + p[i] * vc;
+        }
     }
-#undef Z
+    if (n >= 2) {
+        d[n - 2] = AC(n - 2, n - 2);
+        e[n - 2] = AC(n - 1, n - 2);
+        tau[n - 2] = 0.0;
+    }
+    d[n - 1] = AC(n - 1, n - 1);
+    e[n - 1] = 0.0;
+#undef AC
 }
 
-/* Implicit-shift QL on the tridiagonal (d, e), rotating the columns of z.
- * Returns 0 on success, 1 when an eigenvalue needs more than 60 sweeps. */
-static int tridiagonal_ql(int n, double *d, double *e, double *z)
+/* Q = H_0 H_1 ... H_{n-3} into Z (column-major, ldz), reflectors applied
+ * from the last to the first so each touches only the trailing block. */
+static void form_q(int n, const double *A, const double *tau, double *Z, int ldz)
 {
-#define Z(i, j) z[(size_t)(i) * n + (j)]
-    for (int i = 1; i < n; ++i) e[i - 1] = e[i];
-    if (n > 0) e[n - 1] = 0.0;
+    for (int c = 0; c < n; ++c)
+        for (int i = 0; i < n; ++i) Z[(size_t)c * ldz + i] = (i == c) ? 1.0 : 0.0;
+    for (int j = n - 3; j >= 0; --j) {
+        if (tau[j] == 0.0) continue;
+        const int m = n - j - 1;
+        const double *v = A + (size_t)j * n + j + 1;
+        for (int c = j + 1; c < n; ++c) {
+            double *z = Z + (size_t)c * ldz + j + 1;
+            double s = z[0];
+            for (int i = 1; i < m; ++i) s += v[i] * z[i];
+            s *= tau[j];
+            z[0] -= s;
+            for (int i = 1; i < m; ++i) z[i] -= s * v[i];
+        }
+    }
+}
+
+/* Implicit-shift QL on the tridiagonal (d, e), rotating columns of Z.
+ * Returns 0 on success, 1 when an eigenvalue needs more than 60 sweeps. */
+static int tridiag_ql(int n, double *d, double *e, double *Z, int ldz)
+{
     for (int l = 0; l < n; ++l) {
         int iter = 0, m;
         do {
@@ -269,10 +282,11 @@ static int tridiagonal_ql(int n, double *d, double *e, double *z)
                     p = s * r;
                     d[i + 1] = g + p;
                     g = c * r - b;
+                    double *zi = Z + (size_t)i * ldz, *zj = Z + (size_t)(i + 1) * ldz;
                     for (int k = 0; k < n; ++k) {
-                        double zf = Z(k, i + 1);
-                        Z(k, i + 1) = s * Z(k, i) + c * zf;
-                        Z(k, i) = c * Z(k, i) - s * zf;
+                        const double zf = zj[k], ze = zi[k];
+                        zj[k] = s * ze + c * zf;
+                        zi[k] = c * ze - s * zf;
                     }
                 }
                 if (r == 0.0 && i >= l) continue;
@@ -283,49 +297,47 @@ static int tridiagonal_ql(int n, double *d, double *e, double *z)
         } while (m != l);
     }
     return 0;
-#undef Z
 }
 
 /* All eigenpairs of the symmetric n x n matrix whose upper triangle is in
  * a (column-major, lda).  Eigenvalues ascending in eval; eigenvector j in
- * evec[j*lde .. j*lde+n-1].  work needs n*n + 2n doubles. */
+ * evec[j*lde .. j*lde+n-1] (lde >= n).  work needs n*n + 4n doubles. */
 void GCGE_Dense_SymEig(GCGE_DOUBLE *a, GCGE_INT lda, GCGE_INT n, GCGE_DOUBLE *eval,
                        GCGE_DOUBLE *evec, GCGE_INT lde, GCGE_DOUBLE *work)
 {
     if (n <= 0) return;
-    double *z = work, *d = work + (size_t)n * n, *e = d + n;
-    /* full symmetric copy from the upper triangle: z(i,j) = a(min,max) */
+    double *A = work, *d = work + (size_t)n * n, *e = d + n, *tau = e + n, *p = tau + n;
+    /* full symmetric copy from the upper triangle */
     for (int j = 0; j < n; ++j)
         for (int i = 0; i <= j; ++i) {
-            double v = a[i + (size_t)j * lda];
-            z[(size_t)i * n + j] = v;
-            z[(size_t)j * n + i] = v;
+            const double v = a[i + (size_t)j * lda];
+            A[(size_t)j * n + i] = v;
+            A[(size_t)i * n + j] = v;
         }
-    tridiagonalize(n, z, d, e);
-    tridiagonal_ql(n, d, e, z);
-    /* selection sort of the eigenvalues, permuting vector columns */
+    sym_tridiag(n, A, d, e, tau, p);
+    form_q(n, A, tau, evec, lde);
+    tridiag_ql(n, d, e, evec, lde);
+    /* selection sort of the eigenvalues, swapping vector columns */
     for (int i = 0; i < n - 1; ++i) {
         int k = i;
-        double p = d[i];
+        double pv = d[i];
         for (int j = i + 1; j < n; ++j)
-            if (d[j] < p) {
+            if (d[j] < pv) {
                 k = j;
-                p = d[j];
+                pv = d[j];
             }
         if (k != i) {
             d[k] = d[i];
-            d[i] = p;
+            d[i] = pv;
+            double *zi = evec + (size_t)i * lde, *zk = evec + (size_t)k * lde;
             for (int r = 0; r < n; ++r) {
-                double t = z[(size_t)r * n + i];
-                z[(size_t)r * n + i] = z[(size_t)r * n + k];
-                z[(size_t)r * n + k] = t;
+                const double t = zi[r];
+                zi[r] = zk[r];
+                zk[r] = t;
             }
         }
     }
-    for (int j = 0; j < n; ++j) {
-        eval[j] = d[j];
-        for (int r = 0; r < n; ++r) evec[r + (size_t)j * lde] = z[(size_t)r * n + j];
-    }
+    for (int j = 0; j < n; ++j) eval[j] = d[j];
 }
 
 void GCGE_Default_DenseMatEigenSolver(GCGE_DOUBLE *a, GCGE_INT lda, GCGE_INT nrows,
@@ -339,7 +351,7 @@ void GCGE_Default_DenseMatEigenSolver(GCGE_DOUBLE *a, GCGE_INT lda, GCGE_INT nro
     if (n <= 0) return;
     double *vals = (double *)malloc((size_t)n * sizeof(double));
     double *vecs = (double *)malloc((size_t)n * n * sizeof(double));
-    double *work = (double *)malloc(((size_t)n * n + 2 * (size_t)n) * sizeof(double));
+    double *work = (double *)malloc(((size_t)n * n + 4 * (size_t)n) * sizeof(double));
     GCGE_Dense_SymEig(a, lda, n, vals, vecs, n, work);
     if (il == 1 && iu == n) {
         /* whole spectrum: vectors replace a (leading dimension lda), then n*n

@@ -12,6 +12,13 @@
  * :507-805: the locked rows of the W columns are zeroed, eigenpairs
  * unlock[0]+1 .. min(max_dim_x, ldm) of the trailing block are computed,
  * and num_soft_locked_in_last_iter becomes unlock[0].
+ *
+ * With full_rr the converged prefix stays in both steps: its rows of the
+ * new columns are computed, the whole projected matrix is solved and every
+ * Ritz vector is recomputed.  Zeroing those rows treats the converged
+ * vectors as exact; they are only converged to the tolerance, which caps the
+ * attainable residual of a vector at relative eigenvalue distance g from
+ * them near tol / g and can stall the iteration when g < 1.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -61,7 +68,7 @@ void GCGE_ComputeSubspaceMatrix(void *A, void **V, GCGE_OPS *ops, GCGE_PARA *par
         for (GCGE_INT i = 0; i < dim_x; ++i) H[(size_t)i * ldm + i] = ws->eval[i];
     }
     para->opt_rr_eig_partly = 1;
-    const GCGE_INT start_V = ws->num_soft_locked_in_last_iter;
+    const GCGE_INT start_V = para->full_rr ? 0 : ws->num_soft_locked_in_last_iter;
     GCGE_ComputeSubspaceMatrixVTAW(V, A, dim_xp, start_V, ldm, H + (size_t)dim_xp * ldm, ops,
                                    ws->evec);
     if (para->if_shift == 1)
@@ -120,7 +127,7 @@ void GCGE_ComputeSubspaceEigenpairs(GCGE_DOUBLE *subspace_matrix, GCGE_DOUBLE *e
     GCGE_DOUBLE *dwork = temp + (size_t)ldm * ldm;
     const GCGE_INT max_dim_x = ws->max_dim_x;
     GCGE_INT nsl = ws->unlock[0];
-    if (para->opt_rr_eig_partly == 0) nsl = 0;
+    if (para->opt_rr_eig_partly == 0 || para->full_rr) nsl = 0;
     GCGE_INT il = nsl + 1;
     ws->num_soft_locked_in_last_iter = nsl;
     GCGE_INT iu = max_dim_x < ldm ? max_dim_x : ldm;

@@ -194,16 +194,86 @@ __global__ __launch_bounds__(kBlock) void k_gram(int n, int rows_per_block,
         }
 }
 
-// out[e] = sum_b partial[b * stride + e], e < count
+// Small Gram (m <= MR, k <= KR, MR, KR <= 8): every thread walks rows with
+// stride 256 and keeps the whole m x k block in registers; the block's sums
+// go through wave shuffles and LDS to partial[bx * m * k + j * m + i].
+template <int MR, int KR>
+__global__ __launch_bounds__(kBlock) void k_gram_small(int n, int rows_per_block,
+                                                      const double *__restrict__ V, long ldv,
+                                                      int m, const double *__restrict__ W,
+                                                      long ldw, int k,
+                                                      double *__restrict__ partial)
+{
+    __shared__ double lds[kBlock / kWave][MR * KR];
+    const int bx = blockIdx.x;
+    const int r0 = bx * rows_per_block, r1 = min(n, r0 + rows_per_block);
+    double acc[MR][KR];
+#pragma unroll
+    for (int a = 0; a < MR; ++a)
+#pragma unroll
+        for (int b = 0; b < KR; ++b) acc[a][b] = 0.0;
+    for (int r = r0 + threadIdx.x; r < r1; r += kBlock) {
+        double v[MR], w[KR];
+#pragma unroll
+        for (int a = 0; a < MR; ++a) v[a] = a < m ? V[(long)a * ldv + r] : 0.0;
+#pragma unroll
+        for (int b = 0; b < KR; ++b) w[b] = b < k ? W[(long)b * ldw + r] : 0.0;
+#pragma unroll
+        for (int a = 0; a < MR; ++a)
+#pragma unroll
+            for (int b = 0; b < KR; ++b) acc[a][b] += v[a] * w[b];
+    }
+    const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+#pragma unroll
+    for (int a = 0; a < MR; ++a)
+#pragma unroll
+        for (int b = 0; b < KR; ++b) {
+            const double t = wave_sum(acc[a][b]);
+            if (lane == 0) lds[wid][a * KR + b] = t;
+        }
+    __syncthreads();
+    const int e = threadIdx.x;
+    if (e < MR * KR) {
+        const int a = e / KR, b = e % KR;
+        if (a < m && b < k) {
+            double t = 0.0;
+            for (int w = 0; w < kBlock / kWave; ++w) t += lds[w][e];
+            partial[(long)bx * m * k + (long)b * m + a] = t;
+        }
+    }
+}
+
+// out[e] = sum_b partial[b * stride + e], e < count.  A block owns OUTS
+// consecutive outputs; its 256 / OUTS threads per output split the partials
+// (coalesced over e), each with 4 independent running sums, and combine them
+// through LDS in a fixed order, so the result does not depend on timing.
+template <int OUTS>
 __global__ __launch_bounds__(kBlock) void k_reduce_partials(const double *__restrict__ partial,
                                                             long stride, int nparts, long count,
                                                             double *__restrict__ out)
 {
-    const long e = (long)blockIdx.x * kBlock + threadIdx.x;
-    if (e >= count) return;
-    double s = 0.0;
-    for (int b = 0; b < nparts; ++b) s += partial[(long)b * stride + e];
-    out[e] = s;
+    constexpr int T = kBlock / OUTS;  // threads per output
+    __shared__ double lds[kBlock];
+    const int o = threadIdx.x % OUTS, t = threadIdx.x / OUTS;
+    const long e = (long)blockIdx.x * OUTS + o;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    if (e < count) {
+        int b = t;
+        for (; b + 3 * T < nparts; b += 4 * T) {
+            s0 += partial[(long)b * stride + e];
+            s1 += partial[(long)(b + T) * stride + e];
+            s2 += partial[(long)(b + 2 * T) * stride + e];
+            s3 += partial[(long)(b + 3 * T) * stride + e];
+        }
+        for (; b < nparts; b += T) s0 += partial[(long)b * stride + e];
+    }
+    lds[threadIdx.x] = (s0 + s1) + (s2 + s3);
+    __syncthreads();
+    for (int h = T / 2; h > 0; h >>= 1) {
+        if (t < h) lds[threadIdx.x] += lds[threadIdx.x + h * OUTS];
+        __syncthreads();
+    }
+    if (t == 0 && e < count) out[e] = lds[o];
 }
 
 // ---- linear combination --------------------------------------------------------------
@@ -486,6 +556,22 @@ __global__ __launch_bounds__(kBlock) void k_axpy_cols(int n, const double *__res
 
 inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
 
+// out[e] = sum over nparts partial rows (row stride `stride`) of entry e
+void reduce_partials(const double *partial, long stride, int nparts, long count, double *out)
+{
+    hipStream_t st = state().stream;
+    if (count <= 1)
+        hipLaunchKernelGGL(k_reduce_partials<1>, dim3(1), dim3(kBlock), 0, st, partial, stride, nparts, count, out);
+    else if (count <= 4)
+        hipLaunchKernelGGL(k_reduce_partials<4>, dim3(1), dim3(kBlock), 0, st, partial, stride, nparts, count, out);
+    else if (count <= 16)
+        hipLaunchKernelGGL(k_reduce_partials<16>, dim3(1), dim3(kBlock), 0, st, partial, stride, nparts, count, out);
+    else
+        hipLaunchKernelGGL(k_reduce_partials<64>, dim3(ceil_div(count, 64)), dim3(kBlock), 0, st, partial, stride,
+                           nparts, count, out);
+    HIP_CHECK(hipGetLastError());
+}
+
 // rows per block for column reductions: enough blocks to fill the chip
 inline int reduction_rows(int n, int k)
 {
@@ -548,15 +634,50 @@ void gram(int n, const double *V, long ldv, int m, const double *W, long ldw, in
         return;
     }
     State &s = state();
+    const long mk = (long)m * k;
+    if (m <= 8 && k <= 8) {
+        // register-blocked path for the small blocks of the orthonormalisation
+        int nbx = std::max(1, std::min(ceil_div(n, 4 * kBlock), 512));
+        const int rows = ceil_div(n, nbx);
+        nbx = std::max(1, ceil_div(n, rows));
+        double *partial = partial_buffer((size_t)nbx * mk);
+        double *d_out = small_buffer2((size_t)mk);
+        {
+            ProfScope prof("gram", 8.0 * (double)n * (m + k));
+            const int MR = m <= 1 ? 1 : m <= 2 ? 2 : m <= 4 ? 4 : 8;
+            const int KR = k <= 1 ? 1 : k <= 2 ? 2 : k <= 4 ? 4 : 8;
+#define GCGE_GRAM_SMALL(A, B)                                                                        \
+    if (MR == A && KR == B)                                                                          \
+        hipLaunchKernelGGL((k_gram_small<A, B>), dim3(nbx), dim3(kBlock), 0, s.stream, n, rows, V, ldv, \
+                           m, W, ldw, k, partial);
+            GCGE_GRAM_SMALL(1, 1) GCGE_GRAM_SMALL(1, 2) GCGE_GRAM_SMALL(1, 4) GCGE_GRAM_SMALL(1, 8)
+            GCGE_GRAM_SMALL(2, 1) GCGE_GRAM_SMALL(2, 2) GCGE_GRAM_SMALL(2, 4) GCGE_GRAM_SMALL(2, 8)
+            GCGE_GRAM_SMALL(4, 1) GCGE_GRAM_SMALL(4, 2) GCGE_GRAM_SMALL(4, 4) GCGE_GRAM_SMALL(4, 8)
+            GCGE_GRAM_SMALL(8, 1) GCGE_GRAM_SMALL(8, 2) GCGE_GRAM_SMALL(8, 4) GCGE_GRAM_SMALL(8, 8)
+#undef GCGE_GRAM_SMALL
+            HIP_CHECK(hipGetLastError());
+        }
+        {
+            ProfScope prof("gram_reduce");
+            reduce_partials(partial, mk, nbx, mk, d_out);
+        }
+        double *h = result_buffer((size_t)mk);
+        HIP_CHECK(hipMemcpyAsync(h, d_out, mk * sizeof(double), hipMemcpyDeviceToHost, s.stream));
+        if (!sync_result) return;
+        HIP_CHECK(hipStreamSynchronize(s.stream));
+        for (int j = 0; j < k; ++j)
+            for (int i = 0; i < m; ++i) G[(long)j * ldg + i] = h[(long)j * m + i];
+        return;
+    }
     const bool wide_m = m > 16, wide_k = k > 16;
     const int TM = wide_m ? 64 : 16, TN = wide_k ? 64 : 16;
     const int tm = ceil_div(m, TM), tn = ceil_div(k, TN);
     const int tiles = tm * tn;
-    int nbx = std::max(1, std::min(ceil_div(n, kBlock), std::max(1, 2048 / tiles)));
+    // about two workgroups per CU over all tiles, at least 256 rows each
+    int nbx = std::max(1, std::min(ceil_div(n, kBlock), std::max(1, 512 / tiles)));
     int rows = ceil_div(n, nbx);
     rows = (rows + kGramRows - 1) / kGramRows * kGramRows;
     nbx = std::max(1, ceil_div(n, rows));
-    const long mk = (long)m * k;
     double *partial = partial_buffer((size_t)nbx * mk);
     double *d_out = small_buffer2((size_t)mk);
     {
@@ -574,9 +695,7 @@ void gram(int n, const double *V, long ldv, int m, const double *W, long ldw, in
     }
     {
         ProfScope prof("gram_reduce");
-        hipLaunchKernelGGL(k_reduce_partials, dim3(ceil_div(mk, kBlock)), dim3(kBlock), 0, s.stream,
-                           partial, mk, nbx, mk, d_out);
-        HIP_CHECK(hipGetLastError());
+        reduce_partials(partial, mk, nbx, mk, d_out);
     }
     double *h = result_buffer((size_t)mk);
     HIP_CHECK(hipMemcpyAsync(h, d_out, mk * sizeof(double), hipMemcpyDeviceToHost, s.stream));

@@ -162,6 +162,11 @@ typedef struct GCGE_PARA_ {
      * extensions switch (1: use GCGE_OPS_EXT when the backend has it) */
     GCGE_DOUBLE solve_time;
     GCGE_INT use_ops_ext;
+    /* added: 1 keeps the converged prefix in the Rayleigh-Ritz step with its
+     * computed coupling to the new directions (they still get no new W
+     * directions); 0 (default) zeroes that coupling and solves only the
+     * trailing block, as src/gcge_rayleighritz.c:507-805 does */
+    GCGE_INT full_rr;
     /* added: optional per-check record; row r (r < history_cap) holds
      * num_iter, num_conv_continuous, then eval[0..nev-1] after the r-th
      * convergence check; history_len counts the rows written */
@@ -310,6 +315,9 @@ void GCGE_SortEigenpairs(GCGE_DOUBLE *eval, GCGE_DOUBLE *evec, GCGE_INT nev, GCG
                          GCGE_DOUBLE *work);
 
 /* ---- dense host routines used as the table defaults ---- */
+/* all eigenpairs of the symmetric matrix in the upper triangle of a
+ * (column-major, lda); ascending eval, vector j in evec[j*lde ..], lde >= n;
+ * work holds n*n + 4n doubles */
 void GCGE_Dense_SymEig(GCGE_DOUBLE *a, GCGE_INT lda, GCGE_INT n, GCGE_DOUBLE *eval,
                        GCGE_DOUBLE *evec, GCGE_INT lde, GCGE_DOUBLE *work);
 

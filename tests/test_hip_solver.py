@@ -130,3 +130,19 @@ def test_mixed_precision_lognormal_against_fp64():
     np.testing.assert_allclose(mixed["eval"], ref["eval"], rtol=1e-9)
     assert np.all(_residuals(K, M, mixed["eval"], mixed["evec"]) <= 1.05e-8)
     assert _largest_angle(mixed["evec"], ref["evec"], M) < 1e-5
+
+
+def test_full_rr_on_device():
+    """-gcge_full_rr 1 through the device backend: the Q1 14^3, nev = 30
+    start on which soft locking stalls converges to the closed form, with
+    the float64 and the mixed-precision inner solves."""
+    E, nev = 14, 30
+    K, M = problems.q1_fem_3d(E)
+    x0 = oracle.random_fill(2, 0, 0, nev, K.nrows)
+    flags = ["-gcge_conv_type", "RB", "-gcge_ev_tol", "1e-8", "-gcge_ev_max_it", "80",
+             "-gcge_full_rr", "1"]
+    for precision in ("fp64", "mixed"):
+        out = gcge.solve(K, M, nev, flags, x0=x0, precision=precision)
+        assert out["num_conv"] == nev and out["num_iter"] < 60, (precision, out["num_iter"])
+        np.testing.assert_allclose(np.sort(out["eval"]), problems.q1_eigs(E, nev), rtol=1e-10)
+        assert np.all(_residuals(K, M, out["eval"], out["evec"]) <= 1.05e-8)

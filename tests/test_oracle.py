@@ -329,3 +329,37 @@ def test_live_reference_block_cg():
         xr, rr = oracle.ref_block_cg(K, M, b, x0, *args)
         np.testing.assert_array_equal(x, xr)
         np.testing.assert_array_equal(r, rr)
+
+
+# ---- soft-locking stall and the full Rayleigh-Ritz option -------------------------
+
+def test_full_rr_converges_where_soft_locking_stalls():
+    """Q1 14^3, nev = 30, RB 1e-8 from the seed-2 block: zeroing the coupling
+    of the converged prefix in the Rayleigh-Ritz step (the reference's soft
+    locking) caps later vectors of a partly converged cluster above the
+    tolerance; with -gcge_full_rr 1 every pair converges, to the closed-form
+    spectrum."""
+    E, nev = 14, 30
+    K, M = problems.q1_fem_3d(E)
+    x0 = oracle.random_fill(2, 0, 0, nev, K.nrows)
+    flags = ["-gcge_conv_type", "RB", "-gcge_ev_tol", "1e-8", "-gcge_ev_max_it", "80"]
+    full = oracle.solve(K, M, nev, flags + ["-gcge_full_rr", "1"], x0=x0)
+    assert full["num_conv"] == nev and full["num_iter"] < 60
+    np.testing.assert_allclose(np.sort(full["eval"]), problems.q1_eigs(E, nev), rtol=1e-10)
+    X, lam = full["evec"], full["eval"]
+    R = K.to_scipy() @ X - (M.to_scipy() @ X) * lam
+    rb = np.linalg.norm(R, axis=0) / (np.abs(lam) * np.linalg.norm(M.to_scipy() @ X, axis=0))
+    assert np.all(rb <= 1.05e-8)
+    G = X.T @ (M.to_scipy() @ X)
+    assert np.abs(G - np.eye(nev)).max() < 1e-10
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="reference build absent")
+def test_live_reference_soft_locking_stall():
+    """The reference itself stalls on the same start at a strict tolerance
+    of its own criterion: 150 iterations, fewer than nev pairs."""
+    E, nev = 14, 30
+    K, M = problems.q1_fem_3d(E)
+    x0 = oracle.random_fill(2, 0, 0, nev, K.nrows)
+    r = oracle.ref_solve(K, M, nev, ["-gcge_ev_tol", "3e-12", "-gcge_ev_max_it", "150"], x0=x0)
+    assert r["num_iter"] == 150 and r["num_conv"] < nev
