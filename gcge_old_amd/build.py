@@ -72,7 +72,8 @@ def build_hip() -> str:
     with cf.ThreadPoolExecutor(max_workers=8) as ex:
         list(ex.map(_compile, jobs))
     out = os.path.join(LIB, "libgcge_hip.so")
-    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", out] + objs)
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", out] + objs +
+         ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"])
     return out
 
 

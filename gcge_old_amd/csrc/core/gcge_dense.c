@@ -247,9 +247,40 @@ static void form_q(int n, const double *A, const double *tau, double *Z, int ldz
     }
 }
 
-/* Implicit-shift QL on the tridiagonal (d, e), rotating columns of Z.
- * Returns 0 on success, 1 when an eigenvalue needs more than 60 sweeps. */
-static int tridiag_ql(int n, double *d, double *e, double *Z, int ldz)
+/* Apply the plane rotations i = hi-1 .. lo of one QL sweep to columns
+ * lo .. hi of Z, as the sweep would one at a time:
+ *     (z_i, z_{i+1}) <- (c_i z_i - s_i z_{i+1}, s_i z_i + c_i z_{i+1}).
+ * Rows are taken in chunks whose running column stays in registers, so every
+ * element is loaded and stored once per sweep. */
+static void apply_sweep(double *Z, int ldz, int n, int lo, int hi, const double *cs,
+                        const double *sn)
+{
+    enum { CH = 16 };
+    for (int k0 = 0; k0 < n; k0 += CH) {
+        const int kn = (n - k0 < CH) ? n - k0 : CH;
+        double t[CH];
+        const double *zh = Z + (size_t)hi * ldz + k0;
+        for (int k = 0; k < kn; ++k) t[k] = zh[k];
+        for (int i = hi - 1; i >= lo; --i) {
+            const double c = cs[i], s = sn[i];
+            const double *zi = Z + (size_t)i * ldz + k0;
+            double *zn = Z + (size_t)(i + 1) * ldz + k0;
+            for (int k = 0; k < kn; ++k) {
+                const double ze = zi[k];
+                zn[k] = s * ze + c * t[k];
+                t[k] = c * ze - s * t[k];
+            }
+        }
+        double *zl = Z + (size_t)lo * ldz + k0;
+        for (int k = 0; k < kn; ++k) zl[k] = t[k];
+    }
+}
+
+/* Implicit-shift QL on the tridiagonal (d, e), rotating columns of Z; the
+ * rotations of a sweep are recorded in cs/sn (n entries each) and applied
+ * together.  Returns 0 on success, 1 when an eigenvalue needs more than 60
+ * sweeps. */
+static int tridiag_ql(int n, double *d, double *e, double *Z, int ldz, double *cs, double *sn)
 {
     for (int l = 0; l < n; ++l) {
         int iter = 0, m;
@@ -282,13 +313,11 @@ static int tridiag_ql(int n, double *d, double *e, double *Z, int ldz)
                     p = s * r;
                     d[i + 1] = g + p;
                     g = c * r - b;
-                    double *zi = Z + (size_t)i * ldz, *zj = Z + (size_t)(i + 1) * ldz;
-                    for (int k = 0; k < n; ++k) {
-                        const double zf = zj[k], ze = zi[k];
-                        zj[k] = s * ze + c * zf;
-                        zi[k] = c * ze - s * zf;
-                    }
+                    cs[i] = c;
+                    sn[i] = s;
                 }
+                /* rotations m-1 .. i+1 were taken (all of them when i < l) */
+                if (i + 1 < m) apply_sweep(Z, ldz, n, i + 1, m, cs, sn);
                 if (r == 0.0 && i >= l) continue;
                 d[l] -= p;
                 e[l] = g;
@@ -301,12 +330,13 @@ static int tridiag_ql(int n, double *d, double *e, double *Z, int ldz)
 
 /* All eigenpairs of the symmetric n x n matrix whose upper triangle is in
  * a (column-major, lda).  Eigenvalues ascending in eval; eigenvector j in
- * evec[j*lde .. j*lde+n-1] (lde >= n).  work needs n*n + 4n doubles. */
+ * evec[j*lde .. j*lde+n-1] (lde >= n).  work needs n*n + 5n doubles. */
 void GCGE_Dense_SymEig(GCGE_DOUBLE *a, GCGE_INT lda, GCGE_INT n, GCGE_DOUBLE *eval,
                        GCGE_DOUBLE *evec, GCGE_INT lde, GCGE_DOUBLE *work)
 {
     if (n <= 0) return;
-    double *A = work, *d = work + (size_t)n * n, *e = d + n, *tau = e + n, *p = tau + n;
+    double *A = work, *d = work + (size_t)n * n, *e = d + n, *tau = e + n, *p = tau + n,
+           *sn = p + n;
     /* full symmetric copy from the upper triangle */
     for (int j = 0; j < n; ++j)
         for (int i = 0; i <= j; ++i) {
@@ -316,7 +346,8 @@ void GCGE_Dense_SymEig(GCGE_DOUBLE *a, GCGE_INT lda, GCGE_INT n, GCGE_DOUBLE *ev
         }
     sym_tridiag(n, A, d, e, tau, p);
     form_q(n, A, tau, evec, lde);
-    tridiag_ql(n, d, e, evec, lde);
+    /* tau and p are free again: they hold the rotations of a sweep */
+    tridiag_ql(n, d, e, evec, lde, tau, sn);
     /* selection sort of the eigenvalues, swapping vector columns */
     for (int i = 0; i < n - 1; ++i) {
         int k = i;
@@ -351,7 +382,7 @@ void GCGE_Default_DenseMatEigenSolver(GCGE_DOUBLE *a, GCGE_INT lda, GCGE_INT nro
     if (n <= 0) return;
     double *vals = (double *)malloc((size_t)n * sizeof(double));
     double *vecs = (double *)malloc((size_t)n * n * sizeof(double));
-    double *work = (double *)malloc(((size_t)n * n + 4 * (size_t)n) * sizeof(double));
+    double *work = (double *)malloc(((size_t)n * n + 5 * (size_t)n) * sizeof(double));
     GCGE_Dense_SymEig(a, lda, n, vals, vecs, n, work);
     if (il == 1 && iu == n) {
         /* whole spectrum: vectors replace a (leading dimension lda), then n*n

@@ -141,9 +141,14 @@ void GCGE_ComputeSubspaceEigenpairs(GCGE_DOUBLE *subspace_matrix, GCGE_DOUBLE *e
     memcpy(temp, subspace_matrix, (size_t)ldm * ldm * sizeof(GCGE_DOUBLE));
 
     GCGE_INT nrows = ldm - nsl;
-    ops->DenseMatEigenSolver(temp + (size_t)nsl * ldm + nsl, ldm, nrows, eval + nsl,
-                             subspace_evec + (size_t)nsl * ldm + nsl, ldm, il - nsl, iu - nsl, itmp,
-                             dwork);
+    GCGE_OPS_EXT *ext = GCGE_OPS_GetExt(ops);
+    if (!(para->use_ops_ext && ext && ext->DenseMatEigenSolver &&
+          ext->DenseMatEigenSolver(temp + (size_t)nsl * ldm + nsl, ldm, nrows, eval + nsl,
+                                   subspace_evec + (size_t)nsl * ldm + nsl, ldm, il - nsl,
+                                   iu - nsl) == 0))
+        ops->DenseMatEigenSolver(temp + (size_t)nsl * ldm + nsl, ldm, nrows, eval + nsl,
+                                 subspace_evec + (size_t)nsl * ldm + nsl, ldm, il - nsl, iu - nsl,
+                                 itmp, dwork);
     (void)m;
 
     if (strcmp(para->eval_type, "sm") == 0) {

@@ -25,6 +25,9 @@
 #include <type_traits>
 #include <vector>
 
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
 #include "gcge.h"
 #include "gcge_hip.h"
 #include "gcge_hip_internal.h"
@@ -888,8 +891,77 @@ static void ext_axpby_cols(const GCGE_DOUBLE *a, void **x, const GCGE_INT *xcols
     axpby_list(Y->n, dab, X->data, X->ld, d, dab + k, Y->data, Y->ld, d + k, k);
 }
 
+// ---- dense symmetric eigensolve of the Rayleigh-Ritz matrix on the device ---------------
+// rocSOLVER syevd (tridiagonal reduction + divide and conquer) from n = 128 on;
+// below that the host solver is as fast and the call is declined.
+constexpr int kDeviceEigMinN = 128;
+
+struct DenseEig {
+    rocblas_handle handle = nullptr;
+    double *dA = nullptr, *dD = nullptr, *dE = nullptr;
+    int *dinfo = nullptr;
+    int cap = 0;
+};
+static DenseEig g_eig;
+
+static int ext_dense_eig(GCGE_DOUBLE *a, GCGE_INT lda, GCGE_INT n, GCGE_DOUBLE *eval,
+                         GCGE_DOUBLE *evec, GCGE_INT lde, GCGE_INT il, GCGE_INT iu)
+{
+    if (n < kDeviceEigMinN) return 1;
+    State &s = state();
+    DenseEig &e = g_eig;
+    if (!e.handle) {
+        if (rocblas_create_handle(&e.handle) != rocblas_status_success) fail_msg("rocblas_create_handle failed");
+        if (rocblas_set_stream(e.handle, s.stream) != rocblas_status_success) fail_msg("rocblas_set_stream failed");
+    }
+    if (n > e.cap) {
+        HIP_CHECK(hipStreamSynchronize(s.stream));
+        if (e.dA) {
+            HIP_CHECK(hipFree(e.dA));
+            HIP_CHECK(hipFree(e.dD));
+            HIP_CHECK(hipFree(e.dE));
+            HIP_CHECK(hipFree(e.dinfo));
+        }
+        HIP_CHECK(hipMalloc((void **)&e.dA, (size_t)n * n * sizeof(double)));
+        HIP_CHECK(hipMalloc((void **)&e.dD, (size_t)n * sizeof(double)));
+        HIP_CHECK(hipMalloc((void **)&e.dE, (size_t)n * sizeof(double)));
+        HIP_CHECK(hipMalloc((void **)&e.dinfo, sizeof(int)));
+        e.cap = n;
+    }
+    // upper triangle of a (leading dimension lda) -> dA (leading dimension n)
+    HIP_CHECK(hipMemcpy2DAsync(e.dA, (size_t)n * sizeof(double), a, (size_t)lda * sizeof(double),
+                               (size_t)n * sizeof(double), n, hipMemcpyHostToDevice, s.stream));
+    if (rocsolver_dsyevd(e.handle, rocblas_evect_original, rocblas_fill_upper, n, e.dA, n, e.dD,
+                         e.dE, e.dinfo) != rocblas_status_success)
+        fail_msg("rocsolver_dsyevd failed");
+    std::vector<double> vals(n), vecs((size_t)n * n);
+    int info = 0;
+    HIP_CHECK(hipMemcpyAsync(vals.data(), e.dD, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, s.stream));
+    HIP_CHECK(hipMemcpyAsync(vecs.data(), e.dA, (size_t)n * n * sizeof(double), hipMemcpyDeviceToHost, s.stream));
+    HIP_CHECK(hipMemcpyAsync(&info, e.dinfo, sizeof(int), hipMemcpyDeviceToHost, s.stream));
+    HIP_CHECK(hipStreamSynchronize(s.stream));
+    if (info != 0) return 1;  // not converged: let the host solver try
+    // the table operation's output contract
+    if (il == 1 && iu == n) {
+        // whole spectrum: vectors replace a (leading dimension lda), then n*n
+        // contiguous doubles are copied from a to evec
+        for (int j = 0; j < n; ++j) {
+            eval[j] = vals[j];
+            memcpy(a + (size_t)j * lda, vecs.data() + (size_t)j * n, (size_t)n * sizeof(double));
+        }
+        memcpy(evec, a, (size_t)n * n * sizeof(double));
+    } else {
+        for (int j = il - 1; j < iu; ++j) {
+            eval[j - (il - 1)] = vals[j];
+            memcpy(evec + (size_t)(j - (il - 1)) * lde, vecs.data() + (size_t)j * n,
+                   (size_t)n * sizeof(double));
+        }
+    }
+    return 0;
+}
+
 static GCGE_OPS_EXT g_ext = {GCGE_OPS_EXT_MAGIC, ext_residual_norms, ext_block_cg, ext_sync,
-                             ext_mat_dot_cols, ext_axpby_cols, nullptr};
+                             ext_mat_dot_cols, ext_axpby_cols, ext_dense_eig, nullptr};
 
 // ---- host copies ----------------------------------------------------------------------------------------------
 static void mv_set_host(void **mv, int start, int ncols, const double *host, int ld)
@@ -985,6 +1057,16 @@ void GCGE_HIP_Finalize(void)
             HIP_CHECK(hipFree(g_scratch[i].data));
             g_scratch[i] = Scratch();
         }
+    if (g_eig.handle) {
+        rocblas_destroy_handle(g_eig.handle);
+        if (g_eig.dA) {
+            HIP_CHECK(hipFree(g_eig.dA));
+            HIP_CHECK(hipFree(g_eig.dD));
+            HIP_CHECK(hipFree(g_eig.dE));
+            HIP_CHECK(hipFree(g_eig.dinfo));
+        }
+        g_eig = DenseEig();
+    }
     if (s.d_partial) HIP_CHECK(hipFree(s.d_partial));
     if (s.d_small) HIP_CHECK(hipFree(s.d_small));
     if (s.d_small2) HIP_CHECK(hipFree(s.d_small2));

@@ -2,8 +2,10 @@
 //
 //   spmm        sliced-ELLPACK (64-row slices) times a column block (float64,
 //               or float32 values and vectors for the inner solves); one
-//               wavefront per slice, each lane one row, register blocking
-//               over 4 columns; entries of a slice are stored column-major so
+//               wavefront per slice, each lane one row, the columns of a
+//               16-column chunk in registers, four entries' index and value
+//               loads issued ahead of their gathers; entries of a slice are
+//               stored column-major so
 //               that every entry load is one 512-byte coalesced access, and
 //               workgroups are remapped so each XCD sweeps a contiguous band
 //               of rows (its L2 keeps the band of X that the stencil reuses).
@@ -66,7 +68,10 @@ __device__ __forceinline__ double block_sum(double v, double *lds)
 // One chunk of CK consecutive output columns for the lane's row; the slice's
 // entries are re-read per chunk (from L1/L2 after the first chunk).  VT is the
 // stored value type, XT the vector and accumulation type (double, double) for
-// the float64 form, (float, float) for the float32 inner solves.
+// the float64 form, (float, float) for the float32 inner solves.  Entries are
+// taken four at a time: their column indices and values are loaded first, so
+// the gathers of X for four entries are in flight together; each accumulator
+// still adds the entries in order.
 template <int CK, typename VT, typename XT>
 __device__ __forceinline__ void sell_chunk(int row, int n, int w, const int *__restrict__ cp,
                                            const VT *__restrict__ vp,
@@ -81,7 +86,25 @@ __device__ __forceinline__ void sell_chunk(int row, int n, int w, const int *__r
     XT acc[CK];
 #pragma unroll
     for (int cc = 0; cc < CK; ++cc) acc[cc] = XT(0);
-    for (int j = 0; j < w; ++j) {
+    int j = 0;
+    for (; j + 4 <= w; j += 4) {
+        int col[4];
+        XT v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            col[u] = cp[(long)(j + u) * kWave];
+            v[u] = (XT)vp[(long)(j + u) * kWave];
+        }
+#pragma unroll
+        for (int cc = 0; cc < CK; ++cc) {
+            XT xv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) xv[u] = xc[cc][col[u]];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc[cc] += v[u] * xv[u];
+        }
+    }
+    for (; j < w; ++j) {
         const int col = cp[(long)j * kWave];
         const XT v = (XT)vp[(long)j * kWave];
 #pragma unroll
@@ -93,7 +116,25 @@ __device__ __forceinline__ void sell_chunk(int row, int n, int w, const int *__r
     }
 }
 
-template <typename VT, typename XT>
+// remainder chunk of R0 <= r < MAXCK columns, r chosen at run time
+template <int R, int MAXCK, typename VT, typename XT>
+__device__ __forceinline__ void sell_remainder(int r, int row, int n, int w, const int *cp,
+                                               const VT *vp, const XT *X, long ldx, XT *Y,
+                                               long ldy, int c0, const int *xmap,
+                                               const int *ymap)
+{
+    if constexpr (R < MAXCK) {
+        if (r == R)
+            sell_chunk<R>(row, n, w, cp, vp, X, ldx, Y, ldy, c0, xmap, ymap);
+        else
+            sell_remainder<R + 1, MAXCK>(r, row, n, w, cp, vp, X, ldx, Y, ldy, c0, xmap, ymap);
+    }
+}
+
+// all k columns: chunks of MAXCK, then one chunk of the remainder (MAXCK is
+// picked per launch so that small k does not pay the registers of wide
+// chunks)
+template <typename VT, typename XT, int MAXCK>
 __global__ __launch_bounds__(kBlock) void k_sell_spmm(int n, int nslices,
                                                      const int *__restrict__ sptr,
                                                      const int *__restrict__ sw,
@@ -114,16 +155,27 @@ __global__ __launch_bounds__(kBlock) void k_sell_spmm(int n, int nslices,
     const int *cp = scol + off + lane;
     const VT *vp = sval + off + lane;
     int c0 = 0;
-    for (; c0 + 8 <= k; c0 += 8) sell_chunk<8>(row, n, w, cp, vp, X, ldx, Y, ldy, c0, xmap, ymap);
-    if (c0 + 4 <= k) {
-        sell_chunk<4>(row, n, w, cp, vp, X, ldx, Y, ldy, c0, xmap, ymap);
-        c0 += 4;
-    }
-    if (c0 + 2 <= k) {
-        sell_chunk<2>(row, n, w, cp, vp, X, ldx, Y, ldy, c0, xmap, ymap);
-        c0 += 2;
-    }
-    if (c0 < k) sell_chunk<1>(row, n, w, cp, vp, X, ldx, Y, ldy, c0, xmap, ymap);
+    for (; c0 + MAXCK <= k; c0 += MAXCK)
+        sell_chunk<MAXCK>(row, n, w, cp, vp, X, ldx, Y, ldy, c0, xmap, ymap);
+    if (c0 < k) sell_remainder<1, MAXCK>(k - c0, row, n, w, cp, vp, X, ldx, Y, ldy, c0, xmap, ymap);
+}
+
+template <typename VT, typename XT>
+void launch_sell_spmm(const Mat *A, const VT *vals, const XT *X, long ldx, XT *Y, long ldy, int k,
+                      const int *d_xmap, const int *d_ymap)
+{
+    const int nb = (A->nslices + kBlock / kWave - 1) / (kBlock / kWave);
+    hipStream_t st = state().stream;
+    if (k <= 4)
+        hipLaunchKernelGGL((k_sell_spmm<VT, XT, 4>), dim3(nb), dim3(kBlock), 0, st, A->nrows, A->nslices,
+                           A->d_slice_ptr, A->d_slice_w, A->d_col, vals, X, ldx, Y, ldy, k, d_xmap, d_ymap);
+    else if (k <= 8)
+        hipLaunchKernelGGL((k_sell_spmm<VT, XT, 8>), dim3(nb), dim3(kBlock), 0, st, A->nrows, A->nslices,
+                           A->d_slice_ptr, A->d_slice_w, A->d_col, vals, X, ldx, Y, ldy, k, d_xmap, d_ymap);
+    else
+        hipLaunchKernelGGL((k_sell_spmm<VT, XT, 16>), dim3(nb), dim3(kBlock), 0, st, A->nrows, A->nslices,
+                           A->d_slice_ptr, A->d_slice_w, A->d_col, vals, X, ldx, Y, ldy, k, d_xmap, d_ymap);
+    HIP_CHECK(hipGetLastError());
 }
 
 __global__ __launch_bounds__(kBlock) void k_to_float(size_t len, const double *__restrict__ src,
@@ -594,11 +646,7 @@ void spmm(const Mat *A, const double *X, long ldx, double *Y, long ldy, int k, c
     const double bytes = (double)A->nnz * (8 + 4) + 4.0 * (A->nrows + 1) +
                          8.0 * (double)k * (A->ncols + A->nrows);
     ProfScope prof("spmm", bytes);
-    const int nb = ceil_div(A->nslices, kBlock / kWave);
-    hipLaunchKernelGGL((k_sell_spmm<double, double>), dim3(nb), dim3(kBlock), 0, state().stream,
-                       A->nrows, A->nslices, A->d_slice_ptr, A->d_slice_w, A->d_col, A->d_val, X,
-                       ldx, Y, ldy, k, d_xmap, d_ymap);
-    HIP_CHECK(hipGetLastError());
+    launch_sell_spmm(A, A->d_val, X, ldx, Y, ldy, k, d_xmap, d_ymap);
 }
 
 void spmm(const Mat *A, const float *X, long ldx, float *Y, long ldy, int k, const int *d_xmap,
@@ -609,11 +657,7 @@ void spmm(const Mat *A, const float *X, long ldx, float *Y, long ldy, int k, con
     const double bytes = (double)A->nnz * (4 + 4) + 4.0 * (A->nrows + 1) +
                          4.0 * (double)k * (A->ncols + A->nrows);
     ProfScope prof("spmm32", bytes);
-    const int nb = ceil_div(A->nslices, kBlock / kWave);
-    hipLaunchKernelGGL((k_sell_spmm<float, float>), dim3(nb), dim3(kBlock), 0, state().stream,
-                       A->nrows, A->nslices, A->d_slice_ptr, A->d_slice_w, A->d_col, A->d_val32,
-                       X, ldx, Y, ldy, k, d_xmap, d_ymap);
-    HIP_CHECK(hipGetLastError());
+    launch_sell_spmm(A, A->d_val32, X, ldx, Y, ldy, k, d_xmap, d_ymap);
 }
 
 void to_float(const double *src, float *dst, size_t len)

@@ -55,6 +55,11 @@ typedef struct GCGE_OPS_EXT_ {
     void (*MultiVecAxpbyCols)(const GCGE_DOUBLE *a, void **X, const GCGE_INT *xcols,
                               const GCGE_DOUBLE *b, void **Y, const GCGE_INT *ycols, GCGE_INT k,
                               GCGE_OPS *ops);
+    /* The table's DenseMatEigenSolver contract (reference src/gcge_ops.c:401-432)
+     * on the backend's own solver; returns 0 when it solved the problem,
+     * nonzero to decline (the core then calls ops->DenseMatEigenSolver). */
+    int (*DenseMatEigenSolver)(GCGE_DOUBLE *a, GCGE_INT lda, GCGE_INT nrows, GCGE_DOUBLE *eval,
+                               GCGE_DOUBLE *evec, GCGE_INT lde, GCGE_INT il, GCGE_INT iu);
     void *backend; /* backend private state */
 } GCGE_OPS_EXT;
 
@@ -317,7 +322,7 @@ void GCGE_SortEigenpairs(GCGE_DOUBLE *eval, GCGE_DOUBLE *evec, GCGE_INT nev, GCG
 /* ---- dense host routines used as the table defaults ---- */
 /* all eigenpairs of the symmetric matrix in the upper triangle of a
  * (column-major, lda); ascending eval, vector j in evec[j*lde ..], lde >= n;
- * work holds n*n + 4n doubles */
+ * work holds n*n + 5n doubles */
 void GCGE_Dense_SymEig(GCGE_DOUBLE *a, GCGE_INT lda, GCGE_INT n, GCGE_DOUBLE *eval,
                        GCGE_DOUBLE *evec, GCGE_INT lde, GCGE_DOUBLE *work);
 

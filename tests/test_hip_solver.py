@@ -146,3 +146,16 @@ def test_full_rr_on_device():
         assert out["num_conv"] == nev and out["num_iter"] < 60, (precision, out["num_iter"])
         np.testing.assert_allclose(np.sort(out["eval"]), problems.q1_eigs(E, nev), rtol=1e-10)
         assert np.all(_residuals(K, M, out["eval"], out["evec"]) <= 1.05e-8)
+
+
+def test_large_rayleigh_ritz_on_device_eigensolver():
+    """nev = 100 makes the projected matrix larger than 128, so the
+    Rayleigh-Ritz eigensolve runs on rocSOLVER through the backend's dense
+    eigensolver extension; the spectrum must still be the closed form."""
+    E, nev = 12, 100
+    K, M = problems.q1_fem_3d(E)
+    out = gcge.solve(K, M, nev, ["-gcge_conv_type", "RB", "-gcge_ev_tol", "1e-8",
+                                 "-gcge_ev_max_it", "200", "-gcge_full_rr", "1"], seed=7)
+    assert out["num_conv"] == nev
+    np.testing.assert_allclose(np.sort(out["eval"]), problems.q1_eigs(E, nev), rtol=1e-10)
+    assert np.all(_residuals(K, M, out["eval"], out["evec"]) <= 1.05e-8)
