@@ -10,7 +10,11 @@ device inside each solve from the seed.  The solver runs with
 -gcge_full_rr 1: the converged prefix stays in the Rayleigh-Ritz step with
 its computed coupling.  The reference's soft locking zeroes that coupling,
 which at this tolerance stalls some starts short of nev pairs (the reference
-build does the same); --reference-locking restores it.
+build does the same); --reference-locking restores it.  The W block is
+B-orthonormalised with -gcge_w_orth_type svqb (the reference's projection
+against [X, P], then two SVQB passes instead of the recursive single-column
+leaves: 5 host round trips instead of about 40, same span);
+--reference-orth restores "multi".
 
 The JSON line carries:
   value          mean seconds per solve over the timed steps (lower is better)
@@ -49,8 +53,13 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--mesh", type=int, default=64, help="elements per axis")
-    p.add_argument("--nev", type=int, default=50)
+    p.add_argument("--workload", choices=("q1-64", "q1-128-lognormal"), default="q1-64",
+                   help="q1-64: Q1 64^3, kappa = 1, nev 50 (default, float64 value); "
+                        "q1-128-lognormal: Q1 128^3, lognormal kappa, nev 100, block 200 "
+                        "requested (the mixed-precision comparison on one seed)")
+    p.add_argument("--mesh", type=int, default=None, help="elements per axis (overrides the workload)")
+    p.add_argument("--nev", type=int, default=None)
+    p.add_argument("--kappa-seed", type=int, default=1)
     p.add_argument("--tol", type=float, default=1e-8)
     p.add_argument("--cpu-iters", type=int, default=3,
                    help="GCG iterations timed for the CPU baseline (0: skip)")
@@ -58,12 +67,56 @@ def parse():
                    help="solves of the mixed-precision form (default: --steps; 0: skip)")
     p.add_argument("--reference-locking", action="store_true",
                    help="soft locking as in the reference (no -gcge_full_rr)")
+    p.add_argument("--reference-orth", action="store_true",
+                   help="recursive \"multi\" W orthonormalisation as in the reference "
+                        "(default here: -gcge_w_orth_type svqb)")
     p.add_argument("--no-profile", action="store_true",
                    help="time without per-kernel events (no roofline numbers)")
+    p.add_argument("--pmc-probe", action="store_true",
+                   help="only run 5 SpMM launches (K, k=10) and 5 plain copies of n x 64 doubles, "
+                        "for rocprofv3 --pmc counter passes; prints one line")
     p.add_argument("--kernels", action="store_true",
                    help="also time SpMM, Gram, linear combination and a plain copy in isolation "
                         "(printed on stderr; the copy is the calibration for HBM counters)")
     return p.parse_args()
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "spmm_q1_64_k10_hbm_traffic.md")
+
+
+def read_traffic():
+    """HBM bytes per SpMM launch (K, k = 10) from the committed counter
+    summary: the line 'traffic_bytes_per_launch: <value>'."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def pmc_probe(K):
+    """Isolated launches for the counter passes: SpMM on K with 10 columns
+    (the block-CG shape of the workload) and a plain copy of known size."""
+    import gcge_old_amd as gcge
+    from gcge_old_amd import _native
+
+    lib = _native.hip()
+    n = K.nrows
+    dK = gcge.DeviceMatrix(K)
+    X = gcge.DeviceMultiVec(n, 64)
+    Y = gcge.DeviceMultiVec(n, 64)
+    lib.GCGE_HIP_RandomFill(X.handle, 0, 64)
+    for _ in range(5):
+        lib.GCGE_HIP_SpMM(dK.handle, X.handle, 0, Y.handle, 0, 10)
+    for _ in range(5):
+        lib.GCGE_HIP_Axpby(1.0, X.handle, 0, 0.0, Y.handle, 0, 64)
+    lib.GCGE_HIP_Synchronize()
+    spmm_bytes = K.nnz * 12 + 4 * (n + 1) + 8 * 10 * 2 * n
+    print(json.dumps({"pmc_probe": True, "spmm_algorithmic_bytes": spmm_bytes,
+                      "copy_bytes_read": 8 * n * 64, "copy_bytes_written": 8 * n * 64}), flush=True)
 
 
 def kernel_sweep(K, M):
@@ -195,16 +248,27 @@ def main():
 
     gcge.init(0)
     lib = _native.hip()
-    E = args.mesh
+    lognormal = args.workload == "q1-128-lognormal"
+    E = args.mesh or (128 if lognormal else 64)
+    if args.nev is None:
+        args.nev = 100 if lognormal else 50
     t0 = time.time()
-    K, M = problems.q1_fem_3d(E)
+    kappa = problems.lognormal_kappa(E, args.kappa_seed) if lognormal else None
+    K, M = problems.q1_fem_3d(E, kappa)
     assemble_s = time.time() - t0
     dK, dM = gcge.DeviceMatrix(K), gcge.DeviceMatrix(M)
     flags = ["-gcge_conv_type", "RB", "-gcge_ev_tol", str(args.tol), "-gcge_ev_max_it", "500"]
+    if lognormal:
+        flags += ["-gcge_block_size", str(2 * args.nev)]
     if not args.reference_locking:
         flags += ["-gcge_full_rr", "1"]
-    print(f"# workload: Q1 {E}^3 n={K.nrows} nnz(K)={K.nnz} nnz(M)={M.nnz} nev={args.nev} "
-          f"(assembly {assemble_s:.1f}s)", file=sys.stderr, flush=True)
+    if not args.reference_orth:
+        flags += ["-gcge_w_orth_type", "svqb"]
+    print(f"# workload: Q1 {E}^3 {'lognormal kappa ' if lognormal else ''}n={K.nrows} nnz(K)={K.nnz} "
+          f"nnz(M)={M.nnz} nev={args.nev} (assembly {assemble_s:.1f}s)", file=sys.stderr, flush=True)
+    if args.pmc_probe:
+        pmc_probe(K)
+        return
     if args.kernels:
         kernel_sweep(K, M)
 
@@ -237,15 +301,22 @@ def main():
         sp = kernels["spmm"]
         achieved = sp["bytes"] / (sp["ms"] * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": read_traffic(),
+                    "traffic_note": "HBM bytes per SpMM launch on K with 10 columns (the block-CG "
+                                    "shape), rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a copy "
+                                    "calibration; algorithmic bytes of that launch: "
+                                    f"{K.nnz * 12 + 4 * (K.nrows + 1) + 160 * K.nrows}",
                     "kernel": "spmm", "launches": sp["launches"],
                     "avg_launch_us": round(sp["ms"] * 1e3 / sp["launches"], 2),
                     "algorithmic_bytes_per_launch": round(sp["bytes"] / sp["launches"])}
 
-    # accuracy of the last solve against the closed form
-    exact = problems.q1_eigs(E, args.nev)
+    # accuracy of the last solve against the closed form (kappa = 1 only)
     last = outs[-1]
-    eig_err = float(np.max(np.abs(np.sort(last["eval"]) - exact) / exact))
+    eig_err = None
+    if not lognormal:
+        exact = problems.q1_eigs(E, args.nev)
+        eig_err = float(np.max(np.abs(np.sort(last["eval"]) - exact) / exact))
 
     mixed = None
     mixed_steps = args.steps if args.mixed_steps < 0 else args.mixed_steps
@@ -272,9 +343,12 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"Q1 FEM {E}^3 unit cube, kappa=1, K x = lambda M x, nev={args.nev}, "
-                               f"FP64, ||r||/(|lambda| ||Mx||) <= {args.tol:g}",
+        "config": {"workload": f"Q1 FEM {E}^3 unit cube, "
+                               f"{'lognormal kappa (seed %d)' % args.kappa_seed if lognormal else 'kappa=1'}, "
+                               f"K x = lambda M x, nev={args.nev}, FP64, "
+                               f"||r||/(|lambda| ||Mx||) <= {args.tol:g}",
                    "rayleigh_ritz": "reference soft locking" if args.reference_locking else "full (-gcge_full_rr 1)",
+                   "w_orth": "multi (reference)" if args.reference_orth else "svqb (-gcge_w_orth_type svqb)",
                    "n": K.nrows, "nnz_K": K.nnz, "nnz_M": M.nnz, "nev": args.nev,
                    "block_size": last["block_size"], "parallelism": "1 GPU"},
         "iterations": iters,

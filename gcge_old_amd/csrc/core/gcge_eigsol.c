@@ -67,7 +67,7 @@ static void orth_w(void **V, GCGE_INT start, GCGE_INT *end, void *Orth_mat, GCGE
                    GCGE_PARA *para, GCGE_WORKSPACE *ws, int in_loop)
 {
     const char *t = para->w_orth_type;
-    if (strcmp(t, "multi") == 0)
+    if (strcmp(t, "multi") == 0 || strcmp(t, "svqb") == 0)
         GCGE_StableMultiOrthonormalization(V, start, end, Orth_mat, ops, para, ws);
     else if (strcmp(t, "cbgs") == 0 || strcmp(t, "scbgs") == 0 || (in_loop && strcmp(t, "bgs") == 0))
         unsupported("w_orth_type", t);

@@ -191,7 +191,97 @@ void GCGE_StableMultiOrthonormalizationPartly(void **V, GCGE_INT start, GCGE_INT
             if (iter >= para->orth_para->max_reorth_time) break;
         }
     }
-    GCGE_MultiOrthonormalization(V, start, end, B, ops, para, ws);
+    if (strcmp(para->w_orth_type, "svqb") == 0)
+        GCGE_SVQBOrthonormalization(V, start, end, B, ops, para, ws);
+    else
+        GCGE_MultiOrthonormalization(V, start, end, B, ops, para, ws);
+}
+
+/* B-orthonormalise columns start .. *end-1 among themselves by two SVQB
+ * passes: G = W^T B W, D = diag(G)^(-1/2) (columns with sqrt(G_jj) <=
+ * orth_zero_tol are zero columns, as in the recursive leaves, and dropped),
+ * D G D = U diag(l) U^T, W <- W D U diag(l)^(-1/2) over the directions with
+ * l > 1e-14 max(l), largest first.  One Gram and one linear combination per
+ * pass instead of one Gram per bisection level and leaf; the span of the
+ * surviving columns is the same as with the recursive algorithm. */
+void GCGE_SVQBOrthonormalization(void **V, GCGE_INT start, GCGE_INT *end, void *B, GCGE_OPS *ops,
+                                 GCGE_PARA *para, GCGE_WORKSPACE *ws)
+{
+    const GCGE_DOUBLE zero_tol = para->orth_para->orth_zero_tol;
+    GCGE_INT k = *end - start;
+    if (k <= 0) return;
+    GCGE_DOUBLE *G = (GCGE_DOUBLE *)malloc((size_t)k * k * sizeof(GCGE_DOUBLE));
+    GCGE_DOUBLE *S = (GCGE_DOUBLE *)malloc((size_t)k * k * sizeof(GCGE_DOUBLE));
+    GCGE_DOUBLE *U = (GCGE_DOUBLE *)malloc((size_t)k * k * sizeof(GCGE_DOUBLE));
+    GCGE_DOUBLE *C = (GCGE_DOUBLE *)malloc((size_t)k * k * sizeof(GCGE_DOUBLE));
+    GCGE_DOUBLE *lam = (GCGE_DOUBLE *)malloc((size_t)k * sizeof(GCGE_DOUBLE));
+    GCGE_DOUBLE *dsc = (GCGE_DOUBLE *)malloc((size_t)k * sizeof(GCGE_DOUBLE));
+    GCGE_DOUBLE *work = (GCGE_DOUBLE *)malloc(((size_t)k * k + 5 * (size_t)k) * sizeof(GCGE_DOUBLE));
+    GCGE_INT *keep = (GCGE_INT *)malloc((size_t)k * sizeof(GCGE_INT));
+    GCGE_INT mv_s[2], mv_e[2];
+    for (int pass = 0; pass < 2 && k > 0; ++pass) {
+        /* V_tmp = B W; G = W^T V_tmp */
+        mv_s[0] = start; mv_e[0] = *end;
+        mv_s[1] = 0; mv_e[1] = k;
+        if (B == NULL)
+            ops->MultiVecAxpby(1.0, V, 0.0, ws->V_tmp, mv_s, mv_e, ops);
+        else
+            ops->MatDotMultiVec(B, V, ws->V_tmp, mv_s, mv_e, ops);
+        ops->MultiVecInnerProd(V, ws->V_tmp, G, "ns", mv_s, mv_e, k, 0, ops);
+        GCGE_INT m = 0;
+        for (GCGE_INT j = 0; j < k; ++j) {
+            const GCGE_DOUBLE gjj = G[(size_t)j * k + j];
+            const GCGE_DOUBLE nrm = gjj > 0.0 ? sqrt(gjj) : 0.0;
+            if (nrm > zero_tol) {
+                keep[m++] = j;
+                dsc[j] = 1.0 / nrm;
+            } else {
+                dsc[j] = 0.0;
+            }
+        }
+        GCGE_INT r = 0;
+        if (m > 0) {
+            /* scaled, symmetrised Gram of the kept columns (upper triangle) */
+            for (GCGE_INT b = 0; b < m; ++b)
+                for (GCGE_INT a = 0; a <= b; ++a) {
+                    const GCGE_INT ja = keep[a], jb = keep[b];
+                    S[(size_t)b * m + a] = 0.5 * (G[(size_t)jb * k + ja] + G[(size_t)ja * k + jb]) *
+                                           dsc[ja] * dsc[jb];
+                }
+            GCGE_Dense_SymEig(S, m, m, lam, U, m, work);
+            const GCGE_DOUBLE lmax = lam[m - 1];
+            /* coefficients over all k columns, directions largest first */
+            for (GCGE_INT q = m - 1; q >= 0; --q) {
+                if (!(lam[q] > 1e-14 * lmax)) break;
+                GCGE_DOUBLE *c = C + (size_t)r * k;
+                for (GCGE_INT j = 0; j < k; ++j) c[j] = 0.0;
+                const GCGE_DOUBLE f = 1.0 / sqrt(lam[q]);
+                for (GCGE_INT a = 0; a < m; ++a) c[keep[a]] = U[(size_t)q * m + a] * dsc[keep[a]] * f;
+                ++r;
+            }
+        }
+        if (r > 0) {
+            /* V_tmp(:, 0:r) = W C; W(:, 0:r) = V_tmp(:, 0:r) */
+            mv_s[0] = start; mv_e[0] = *end;
+            mv_s[1] = 0; mv_e[1] = r;
+            ops->MultiVecLinearComb(V, ws->V_tmp, mv_s, mv_e, C, k, 0, 1.0, 0.0, ops);
+            mv_s[0] = 0; mv_e[0] = r;
+            mv_s[1] = start; mv_e[1] = start + r;
+            ops->MultiVecAxpby(1.0, ws->V_tmp, 0.0, V, mv_s, mv_e, ops);
+        }
+        if (r < k && para->orth_para->print_orth_zero == 1)
+            GCGE_Printf("In GCGE_SVQBOrthonormalization, %d of %d columns dropped\n", k - r, k);
+        *end = start + r;
+        k = r;
+    }
+    free(G);
+    free(S);
+    free(U);
+    free(C);
+    free(lam);
+    free(dsc);
+    free(work);
+    free(keep);
 }
 
 void GCGE_SubOrthonormalization(void **V, GCGE_INT *start, GCGE_INT *end, void *B,

@@ -73,6 +73,18 @@ int ensure_init()
         return 1;
     }
     if (s.device < 0) s.device = 0;
+    // how host waits for the device: the solver waits for small results
+    // thousands of times per solve, so the wait mode is a latency knob
+    // (GCGE_HIP_SYNC = spin | yield | blocking | auto; default auto)
+    const char *mode = getenv("GCGE_HIP_SYNC");
+    if (mode) {
+        unsigned flags = hipDeviceScheduleAuto;
+        if (!strcmp(mode, "spin")) flags = hipDeviceScheduleSpin;
+        else if (!strcmp(mode, "yield")) flags = hipDeviceScheduleYield;
+        else if (!strcmp(mode, "blocking")) flags = hipDeviceScheduleBlockingSync;
+        if (hipSetDeviceFlags(flags) != hipSuccess)
+            fprintf(stderr, "gcge_hip: GCGE_HIP_SYNC=%s not applied (device already active)\n", mode);
+    }
     HIP_CHECK(hipSetDevice(s.device));
     HIP_CHECK(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
     for (int i = 0; i < State::kRing; ++i) HIP_CHECK(hipEventCreateWithFlags(&s.ring_event[i], hipEventDisableTiming));

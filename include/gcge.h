@@ -292,6 +292,8 @@ void GCGE_StableMultiOrthonormalizationPartly(void **V, GCGE_INT start, GCGE_INT
                                               GCGE_WORKSPACE *workspace);
 void GCGE_SubOrthonormalization(void **V, GCGE_INT *start, GCGE_INT *end, void *B,
                                 void **V_tmp, GCGE_DOUBLE *subspace_dtmp, GCGE_OPS *ops);
+void GCGE_SVQBOrthonormalization(void **V, GCGE_INT start, GCGE_INT *end, void *B, GCGE_OPS *ops,
+                                 GCGE_PARA *para, GCGE_WORKSPACE *ws);
 void GCGE_MultiOrthonormalization(void **V, GCGE_INT start, GCGE_INT *end, void *B,
                                   GCGE_OPS *ops, GCGE_PARA *para, GCGE_WORKSPACE *workspace);
 void GCGE_SubOrthonormalizationSelfBGS(void **V, GCGE_INT start, GCGE_INT *end, void *B,

@@ -159,3 +159,20 @@ def test_large_rayleigh_ritz_on_device_eigensolver():
     assert out["num_conv"] == nev
     np.testing.assert_allclose(np.sort(out["eval"]), problems.q1_eigs(E, nev), rtol=1e-10)
     assert np.all(_residuals(K, M, out["eval"], out["evec"]) <= 1.05e-8)
+
+
+def test_svqb_full_rr_device_matches_oracle():
+    """The benchmark's solver options (-gcge_full_rr 1, -gcge_w_orth_type svqb)
+    on the device against the same core on the oracle operations."""
+    E, nev = 16, 40
+    K, M = problems.q1_fem_3d(E)
+    x0 = oracle.random_fill(1, 0, 0, nev, K.nrows)
+    flags = ["-gcge_conv_type", "RB", "-gcge_ev_tol", "1e-8", "-gcge_ev_max_it", "150",
+             "-gcge_full_rr", "1", "-gcge_w_orth_type", "svqb"]
+    dev = gcge.solve(K, M, nev, flags, x0=x0)
+    cpu = oracle.solve(K, M, nev, flags, x0=x0)
+    assert dev["num_conv"] == nev and cpu["num_conv"] == nev
+    assert abs(dev["num_iter"] - cpu["num_iter"]) <= 3
+    np.testing.assert_allclose(dev["eval"], cpu["eval"], rtol=1e-10)
+    np.testing.assert_allclose(np.sort(dev["eval"]), problems.q1_eigs(E, nev), rtol=1e-10)
+    assert np.all(_residuals(K, M, dev["eval"], dev["evec"]) <= 1.05e-8)

@@ -363,3 +363,22 @@ def test_live_reference_soft_locking_stall():
     x0 = oracle.random_fill(2, 0, 0, nev, K.nrows)
     r = oracle.ref_solve(K, M, nev, ["-gcge_ev_tol", "3e-12", "-gcge_ev_max_it", "150"], x0=x0)
     assert r["num_iter"] == 150 and r["num_conv"] < nev
+
+
+def test_svqb_w_orth_same_spectrum_and_iteration_count():
+    """-gcge_w_orth_type svqb (two SVQB passes inside W after the same
+    projection against [X, P]) spans the same space as the recursive "multi"
+    orthonormalisation, so the iteration converges to the same pairs in about
+    as many steps; the eigenvectors come out B-orthonormal."""
+    E, nev = 14, 30
+    K, M = problems.q1_fem_3d(E)
+    x0 = oracle.random_fill(3, 0, 0, nev, K.nrows)
+    base = ["-gcge_conv_type", "RB", "-gcge_ev_tol", "1e-8", "-gcge_ev_max_it", "100",
+            "-gcge_full_rr", "1"]
+    multi = oracle.solve(K, M, nev, base, x0=x0)
+    svqb = oracle.solve(K, M, nev, base + ["-gcge_w_orth_type", "svqb"], x0=x0)
+    assert multi["num_conv"] == nev and svqb["num_conv"] == nev
+    assert abs(multi["num_iter"] - svqb["num_iter"]) <= 3
+    np.testing.assert_allclose(np.sort(svqb["eval"]), problems.q1_eigs(E, nev), rtol=1e-10)
+    X = svqb["evec"]
+    assert np.abs(X.T @ (M.to_scipy() @ X) - np.eye(nev)).max() < 1e-10
