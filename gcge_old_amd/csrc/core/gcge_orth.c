@@ -13,6 +13,9 @@
  *   GCGE_SubOrthonormalization     :2022-2071   V2 -= V1 (V1^T B V2)
  *   GCGE_MultiOrthonormalization   :2087-2178   recursive bisection
  *   GCGE_SubOrthonormalizationSelfBGS :2180-2263 (leaves)
+ *   GCGE_SVQBOrthonormalization    no reference counterpart: w_orth_type
+ *       "svqb" replaces the recursive self-orthonormalisation of W by two
+ *       SVQB passes (one Gram block and one linear combination each)
  *   GCGE_OrthonormalizationInSubspace :584-648
  *   GCGE_MultiOrthonormalizationInSubspace / StableMulti...InSubspace
  *       :2278-2395 (default for the P coefficients)

@@ -172,19 +172,24 @@ static hipEvent_t pooled_event()
     return e;
 }
 
-ProfScope::ProfScope(const char *name, double bytes) : slot(-1)
+ProfScope::ProfScope(const char *name, double bytes, bool pkt) : slot(-1), packet(pkt)
 {
     State &s = state();
     if (!s.profiling) return;
+    // profile mode 2: only the SpMM launches (the roofline kernel)
+    if (s.profile_mode == 2 && strncmp(name, "spmm", 4) != 0) return;
     ProfRecord r{name, pooled_event(), pooled_event(), bytes};
-    HIP_CHECK(hipEventRecord(r.start, s.stream));
+    if (!packet) HIP_CHECK(hipEventRecord(r.start, s.stream));
     slot = (int)g_prof.size();
     g_prof.push_back(r);
 }
 
+hipEvent_t ProfScope::start_event() const { return slot < 0 ? nullptr : g_prof[slot].start; }
+hipEvent_t ProfScope::stop_event() const { return slot < 0 ? nullptr : g_prof[slot].stop; }
+
 ProfScope::~ProfScope()
 {
-    if (slot < 0) return;
+    if (slot < 0 || packet) return;
     HIP_CHECK(hipEventRecord(g_prof[slot].stop, state().stream));
 }
 
@@ -1213,7 +1218,11 @@ void GCGE_HIP_MultiLinearSolve(void *mat, void **b, int bs, void **x, int xs, in
 
 const void *GCGE_HIP_Backend(void) { return &g_backend; }
 
-void GCGE_HIP_ProfileEnable(int on) { state().profiling = on != 0; }
+void GCGE_HIP_ProfileEnable(int on)
+{
+    state().profiling = on != 0;
+    state().profile_mode = on;
+}
 
 void GCGE_HIP_ProfileReset(void)
 {

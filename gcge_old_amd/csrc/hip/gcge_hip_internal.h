@@ -41,6 +41,7 @@ struct State {
     uint64_t seed = 0x5eed;
     uint64_t draw = 0;
     bool profiling = false;
+    int profile_mode = 0;  // 1: every launch, 2: SpMM launches only
 };
 State &state();
 int ensure_init();
@@ -56,10 +57,17 @@ void upload_async(void *dst, const void *src, size_t bytes);
 
 // ---- profiling -----------------------------------------------------------------
 // Brackets one launch with HIP events on the backend stream while profiling
-// is enabled; `bytes` is the launch's algorithmic HBM traffic.
+// is enabled; `bytes` is the launch's algorithmic HBM traffic.  With
+// packet = true the caller passes start_event()/stop_event() to
+// hipExtLaunchKernelGGL, which records them in the dispatch packet itself
+// (no separate event packets around the kernel, so the interval is the
+// kernel's own execution time).
 struct ProfScope {
     int slot;
-    explicit ProfScope(const char *name, double bytes = 0.0);
+    bool packet;
+    explicit ProfScope(const char *name, double bytes = 0.0, bool packet = false);
+    hipEvent_t start_event() const;
+    hipEvent_t stop_event() const;
     ~ProfScope();
 };
 

@@ -20,6 +20,7 @@
 //               are a wave-uniform contiguous read, so V is read once per
 //               group of up to 64 output columns.
 //   axpby/swap/random/coldots/residual/bcg_*: streaming kernels.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -162,19 +163,23 @@ __global__ __launch_bounds__(kBlock) void k_sell_spmm(int n, int nslices,
 
 template <typename VT, typename XT>
 void launch_sell_spmm(const Mat *A, const VT *vals, const XT *X, long ldx, XT *Y, long ldy, int k,
-                      const int *d_xmap, const int *d_ymap)
+                      const int *d_xmap, const int *d_ymap, hipEvent_t ev0, hipEvent_t ev1)
 {
     const int nb = (A->nslices + kBlock / kWave - 1) / (kBlock / kWave);
     hipStream_t st = state().stream;
+    // hipExtLaunchKernelGGL records the (optional) events in the dispatch packet
     if (k <= 4)
-        hipLaunchKernelGGL((k_sell_spmm<VT, XT, 4>), dim3(nb), dim3(kBlock), 0, st, A->nrows, A->nslices,
-                           A->d_slice_ptr, A->d_slice_w, A->d_col, vals, X, ldx, Y, ldy, k, d_xmap, d_ymap);
+        hipExtLaunchKernelGGL((k_sell_spmm<VT, XT, 4>), dim3(nb), dim3(kBlock), 0, st, ev0, ev1, 0, A->nrows,
+                              A->nslices, A->d_slice_ptr, A->d_slice_w, A->d_col, vals, X, ldx, Y, ldy, k,
+                              d_xmap, d_ymap);
     else if (k <= 8)
-        hipLaunchKernelGGL((k_sell_spmm<VT, XT, 8>), dim3(nb), dim3(kBlock), 0, st, A->nrows, A->nslices,
-                           A->d_slice_ptr, A->d_slice_w, A->d_col, vals, X, ldx, Y, ldy, k, d_xmap, d_ymap);
+        hipExtLaunchKernelGGL((k_sell_spmm<VT, XT, 8>), dim3(nb), dim3(kBlock), 0, st, ev0, ev1, 0, A->nrows,
+                              A->nslices, A->d_slice_ptr, A->d_slice_w, A->d_col, vals, X, ldx, Y, ldy, k,
+                              d_xmap, d_ymap);
     else
-        hipLaunchKernelGGL((k_sell_spmm<VT, XT, 16>), dim3(nb), dim3(kBlock), 0, st, A->nrows, A->nslices,
-                           A->d_slice_ptr, A->d_slice_w, A->d_col, vals, X, ldx, Y, ldy, k, d_xmap, d_ymap);
+        hipExtLaunchKernelGGL((k_sell_spmm<VT, XT, 16>), dim3(nb), dim3(kBlock), 0, st, ev0, ev1, 0, A->nrows,
+                              A->nslices, A->d_slice_ptr, A->d_slice_w, A->d_col, vals, X, ldx, Y, ldy, k,
+                              d_xmap, d_ymap);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -293,6 +298,54 @@ __global__ __launch_bounds__(kBlock) void k_gram_small(int n, int rows_per_block
             partial[(long)bx * m * k + (long)b * m + a] = t;
         }
     }
+}
+
+// Gram for m up to a few hundred and k <= 16 (the W orthonormalisation and
+// Rayleigh-Ritz blocks): wave w of a workgroup owns V columns
+// [(blockIdx.y * 4 + w) * MR, +MR) and every W column (KR >= k); the four
+// waves walk the same rows, so each W row is fetched from memory once per
+// workgroup, and V is read once.  Lanes take consecutive rows.  Each wave
+// reduces its MR x KR block over lanes and writes
+// partial[bx * m * k + j * m + i].
+template <int MR, int KR>
+__global__ __launch_bounds__(kBlock) void k_gram_cols(int n, int rows_per_block,
+                                                     const double *__restrict__ V, long ldv,
+                                                     int m, const double *__restrict__ W,
+                                                     long ldw, int k,
+                                                     double *__restrict__ partial)
+{
+    const int bx = blockIdx.x;
+    const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+    const int i0 = (blockIdx.y * (kBlock / kWave) + wid) * MR;
+    if (i0 >= m) return;
+    const int r0 = bx * rows_per_block, r1 = min(n, r0 + rows_per_block);
+    double acc[MR][KR];
+#pragma unroll
+    for (int a = 0; a < MR; ++a)
+#pragma unroll
+        for (int b = 0; b < KR; ++b) acc[a][b] = 0.0;
+    const double *vcol[MR];
+#pragma unroll
+    for (int a = 0; a < MR; ++a) vcol[a] = V + (long)min(i0 + a, m - 1) * ldv;
+    for (int r = r0 + lane; r < r1; r += kWave) {
+        double v[MR], w[KR];
+#pragma unroll
+        for (int a = 0; a < MR; ++a) v[a] = (i0 + a < m) ? vcol[a][r] : 0.0;
+#pragma unroll
+        for (int b = 0; b < KR; ++b) w[b] = b < k ? W[(long)b * ldw + r] : 0.0;
+#pragma unroll
+        for (int a = 0; a < MR; ++a)
+#pragma unroll
+            for (int b = 0; b < KR; ++b) acc[a][b] += v[a] * w[b];
+    }
+    double *out = partial + (long)bx * m * k;
+#pragma unroll
+    for (int a = 0; a < MR; ++a)
+#pragma unroll
+        for (int b = 0; b < KR; ++b) {
+            const double t = wave_sum(acc[a][b]);
+            if (lane == 0 && i0 + a < m && b < k) out[(long)b * m + i0 + a] = t;
+        }
 }
 
 // out[e] = sum_b partial[b * stride + e], e < count.  A block owns OUTS
@@ -645,8 +698,8 @@ void spmm(const Mat *A, const double *X, long ldx, double *Y, long ldy, int k, c
     // once, Y written once
     const double bytes = (double)A->nnz * (8 + 4) + 4.0 * (A->nrows + 1) +
                          8.0 * (double)k * (A->ncols + A->nrows);
-    ProfScope prof("spmm", bytes);
-    launch_sell_spmm(A, A->d_val, X, ldx, Y, ldy, k, d_xmap, d_ymap);
+    ProfScope prof("spmm", bytes, true);
+    launch_sell_spmm(A, A->d_val, X, ldx, Y, ldy, k, d_xmap, d_ymap, prof.start_event(), prof.stop_event());
 }
 
 void spmm(const Mat *A, const float *X, long ldx, float *Y, long ldy, int k, const int *d_xmap,
@@ -656,8 +709,8 @@ void spmm(const Mat *A, const float *X, long ldx, float *Y, long ldy, int k, con
     if (!A->d_val32) fail_msg("float32 SpMM on a matrix without float32 values");
     const double bytes = (double)A->nnz * (4 + 4) + 4.0 * (A->nrows + 1) +
                          4.0 * (double)k * (A->ncols + A->nrows);
-    ProfScope prof("spmm32", bytes);
-    launch_sell_spmm(A, A->d_val32, X, ldx, Y, ldy, k, d_xmap, d_ymap);
+    ProfScope prof("spmm32", bytes, true);
+    launch_sell_spmm(A, A->d_val32, X, ldx, Y, ldy, k, d_xmap, d_ymap, prof.start_event(), prof.stop_event());
 }
 
 void to_float(const double *src, float *dst, size_t len)
@@ -699,6 +752,38 @@ void gram(int n, const double *V, long ldv, int m, const double *W, long ldw, in
             GCGE_GRAM_SMALL(4, 1) GCGE_GRAM_SMALL(4, 2) GCGE_GRAM_SMALL(4, 4) GCGE_GRAM_SMALL(4, 8)
             GCGE_GRAM_SMALL(8, 1) GCGE_GRAM_SMALL(8, 2) GCGE_GRAM_SMALL(8, 4) GCGE_GRAM_SMALL(8, 8)
 #undef GCGE_GRAM_SMALL
+            HIP_CHECK(hipGetLastError());
+        }
+        {
+            ProfScope prof("gram_reduce");
+            reduce_partials(partial, mk, nbx, mk, d_out);
+        }
+        double *h = result_buffer((size_t)mk);
+        HIP_CHECK(hipMemcpyAsync(h, d_out, mk * sizeof(double), hipMemcpyDeviceToHost, s.stream));
+        if (!sync_result) return;
+        HIP_CHECK(hipStreamSynchronize(s.stream));
+        for (int j = 0; j < k; ++j)
+            for (int i = 0; i < m; ++i) G[(long)j * ldg + i] = h[(long)j * m + i];
+        return;
+    }
+    if (k <= 16) {
+        // column-block register path: 8 x 8 or 4 x 16 accumulators per lane
+        const int MR = k <= 8 ? 8 : 4;
+        const int colblocks = ceil_div(m, MR);
+        const int gy = ceil_div(colblocks, kBlock / kWave);
+        int nbx = std::max(1, std::min(ceil_div(n, 4 * kWave), std::max(1, 1024 / gy)));
+        const int rows = ceil_div(n, nbx);
+        nbx = std::max(1, ceil_div(n, rows));
+        double *partial = partial_buffer((size_t)nbx * mk);
+        double *d_out = small_buffer2((size_t)mk);
+        {
+            ProfScope prof("gram", 8.0 * (double)n * (m + k));
+            if (k <= 8)
+                hipLaunchKernelGGL((k_gram_cols<8, 8>), dim3(nbx, gy), dim3(kBlock), 0, s.stream, n, rows, V,
+                                   ldv, m, W, ldw, k, partial);
+            else
+                hipLaunchKernelGGL((k_gram_cols<4, 16>), dim3(nbx, gy), dim3(kBlock), 0, s.stream, n, rows, V,
+                                   ldv, m, W, ldw, k, partial);
             HIP_CHECK(hipGetLastError());
         }
         {

@@ -109,8 +109,10 @@ const void *GCGE_HIP_Backend(void);
 
 /* ---- measurement --------------------------------------------------------- */
 /* Per-kernel device time, measured with HIP events on the backend stream
- * while enabled.  name: "spmm", "gram", "lincomb", "axpby", ...; returns the
- * number of launches and the summed milliseconds. */
+ * while enabled (on = 1: every launch; on = 2: the SpMM launches only, whose
+ * events are recorded by the dispatch itself).  name: "spmm", "gram",
+ * "lincomb", "axpby", ...; returns the number of launches and the summed
+ * milliseconds. */
 void GCGE_HIP_ProfileEnable(int on);
 int GCGE_HIP_KernelTime(const char *name, double *ms_total, int *count);
 /* as GCGE_HIP_KernelTime, plus the summed algorithmic bytes of the launches

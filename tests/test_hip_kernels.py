@@ -64,7 +64,8 @@ def test_spmm_matches_oracle(name, k):
 @pytest.mark.parametrize("n,m,k", [(1, 1, 1), (63, 16, 16), (64, 17, 1), (65, 1, 17),
                                    (1000, 70, 10), (4097, 130, 65), (100000, 12, 3),
                                    (100000, 1, 1), (250047, 2, 3), (5000, 8, 8), (777, 5, 5),
-                                   (3, 4, 2), (1025, 1, 7), (300000, 8, 1)])
+                                   (3, 4, 2), (1025, 1, 7), (300000, 8, 1),
+                                   (250047, 80, 10), (3000, 200, 16), (777, 33, 9)])
 def test_gram_matches_oracle(n, m, k):
     rng = _rng(n + m + k)
     V = rng.standard_normal((n, m + 1))
