@@ -87,6 +87,15 @@ void GCGE_BCG(void *Matrix, GCGE_INT if_shift, GCGE_DOUBLE shift, void *B, void 
               GCGE_OPS *ops, void *V_tmp1, void **V_tmp2, void **V_tmp3,
               GCGE_DOUBLE *subspace_dtmp, GCGE_INT *subspace_itmp)
 {
+    GCGE_BlockCG(Matrix, if_shift, shift, B, RHS, V, start, end, max_it, rate, ops, V_tmp1, V_tmp2,
+                 V_tmp3, subspace_dtmp, subspace_itmp, 0);
+}
+
+void GCGE_BlockCG(void *Matrix, GCGE_INT if_shift, GCGE_DOUBLE shift, void *B, void **RHS,
+                  void **V, GCGE_INT *start, GCGE_INT *end, GCGE_INT max_it, GCGE_DOUBLE rate,
+                  GCGE_OPS *ops, void *V_tmp1, void **V_tmp2, void **V_tmp3,
+                  GCGE_DOUBLE *subspace_dtmp, GCGE_INT *subspace_itmp, GCGE_INT by_column)
+{
     void *x, *r, *p, *w, *b, *Bx = NULL;
     void **CG_R = RHS, **CG_P = (void **)V_tmp1, **CG_W = V_tmp2, **CG_X = V;
     const GCGE_INT n = end[1] - start[1];
@@ -135,7 +144,9 @@ void GCGE_BCG(void *Matrix, GCGE_INT if_shift, GCGE_DOUBLE shift, void *B, void 
         }
         for (GCGE_INT id = 0; id < num_unlock; ++id) {
             GCGE_INT idx = unlock[id];
-            GCGE_DOUBLE alpha = rho2[id] / ptw[id];
+            /* the reference takes alpha's numerator by position */
+            const GCGE_INT q = by_column ? idx : id;
+            GCGE_DOUBLE alpha = rho2[q] / ptw[id];
             ops->GetVecFromMultiVec(CG_P, start[2] + idx, &p, ops);
             ops->GetVecFromMultiVec(CG_X, start[1] + idx, &x, ops);
             ops->VecAxpby(alpha, p, 1.0, x, ops);
@@ -144,12 +155,13 @@ void GCGE_BCG(void *Matrix, GCGE_INT if_shift, GCGE_DOUBLE shift, void *B, void 
             ops->GetVecFromMultiVec(CG_R, start[0] + idx, &r, ops);
             ops->GetVecFromMultiVec(CG_W, start[3] + idx, &w, ops);
             ops->VecAxpby(-alpha, w, 1.0, r, ops);
-            rho1[id] = rho2[id];
-            ops->VecLocalInnerProd(r, r, rho2 + id, ops);
+            rho1[q] = rho2[q];
+            ops->VecLocalInnerProd(r, r, rho2 + q, ops);
             ops->RestoreVecForMultiVec(CG_R, start[0] + idx, &r, ops);
             ops->RestoreVecForMultiVec(CG_W, start[3] + idx, &w, ops);
         }
-        for (GCGE_INT id = 0; id < num_unlock; ++id) last_error[unlock[id]] = sqrt(fabs(rho2[id]));
+        for (GCGE_INT id = 0; id < num_unlock; ++id)
+            last_error[unlock[id]] = sqrt(fabs(rho2[by_column ? unlock[id] : id]));
         old_num_unlock = num_unlock;
         num_unlock = 0;
         for (GCGE_INT id = 0; id < old_num_unlock; ++id) {

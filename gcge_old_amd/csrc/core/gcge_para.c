@@ -115,6 +115,7 @@ void GCGE_PARA_Create(GCGE_PARA **para)
     p->solve_time = 0.0;
     p->use_ops_ext = 1;
     p->full_rr = 0;
+    p->bcg_by_column = 0;
     p->history = NULL;
     p->history_cap = 0;
     p->history_len = 0;
@@ -190,6 +191,7 @@ static const flag_row flag_table[] = {
     /* added by this backend */
     PF("-gcge_use_ops_ext", F_INT, use_ops_ext),
     PF("-gcge_full_rr", F_INT, full_rr),
+    PF("-gcge_bcg_by_column", F_INT, bcg_by_column),
 };
 
 static void print_usage(const char *prog)
@@ -208,6 +210,7 @@ static void print_usage(const char *prog)
     GCGE_Printf("  -gcge_cg_rate <i>           inner CG reduction (parsed as integer; default 1e-2)\n");
     GCGE_Printf("  -gcge_use_ops_ext <i>       use backend block extensions (default 1)\n");
     GCGE_Printf("  -gcge_full_rr <i>           keep converged vectors in the Rayleigh-Ritz step (default 0)\n");
+    GCGE_Printf("  -gcge_bcg_by_column <i>     block CG scalars kept per column (default 0)\n");
     GCGE_Printf("  (all other reference -gcge_* flags are accepted with the same meaning)\n\n");
 }
 
@@ -376,5 +379,6 @@ void GCGE_PrintParaInfo(GCGE_PARA *para)
     GCGE_Printf("  use_shift                  : %8d\n", para->use_shift);
     GCGE_Printf("  opt_rr_eig_partly          : %8d\n", para->opt_rr_eig_partly);
     GCGE_Printf("  use_ops_ext                : %8d\n", para->use_ops_ext);
-    GCGE_Printf("  full_rr                    : %8d\n\n", para->full_rr);
+    GCGE_Printf("  full_rr                    : %8d\n", para->full_rr);
+    GCGE_Printf("  bcg_by_column              : %8d\n\n", para->bcg_by_column);
 }

@@ -128,11 +128,11 @@ void GCGE_ComputeW(void *A, void *B, void **V, GCGE_DOUBLE *eval, GCGE_OPS *ops,
         if (para->use_ops_ext && ext && ext->BlockCG) {
             ext->BlockCG(A, para->if_shift, para->shift, B, V_tmp, V, mv_s, mv_e, para->cg_max_it,
                          para->cg_rate, ops, ws->CG_p, ws->evec, ws->subspace_dtmp,
-                         ws->subspace_itmp);
+                         ws->subspace_itmp, para->bcg_by_column);
         } else {
-            GCGE_BCG(A, para->if_shift, para->shift, B, V_tmp, V, mv_s, mv_e, para->cg_max_it,
-                     para->cg_rate, ops, ws->CG_p, ws->evec, NULL, ws->subspace_dtmp,
-                     ws->subspace_itmp);
+            GCGE_BlockCG(A, para->if_shift, para->shift, B, V_tmp, V, mv_s, mv_e, para->cg_max_it,
+                         para->cg_rate, ops, ws->CG_p, ws->evec, NULL, ws->subspace_dtmp,
+                         ws->subspace_itmp, para->bcg_by_column);
         }
     } else {
         for (GCGE_INT idx = 0; idx < w_length; ++idx) {

@@ -631,7 +631,7 @@ static void bcg_iterate(Mat *A, Mat *B, int if_shift, double shift, int n, int l
                         long ldr, int r0, double *Xd, long ldx, int x0, PT *Pd, long ldp, int p0,
                         PT *Wd, long ldw, int w0, PT *T, long ldt, int max_it, double rate,
                         double *rho1, double *rho2, const double *err, double *last_err,
-                        int *unlock, int num_unlock)
+                        int *unlock, int num_unlock, int by_column)
 {
     constexpr bool scaled = std::is_same<PT, float>::value;
     State &s = state();
@@ -656,7 +656,7 @@ static void bcg_iterate(Mat *A, Mat *B, int if_shift, double shift, int n, int l
             maps[3 * len + id] = w0 + idx;  // w
             const double beta = (niter == 1) ? 0.0 : rho2[idx] / rho1[idx];  // by column
             coef[id] = beta;
-            coef[len + id] = rho2[id];  // alpha numerator, by position
+            coef[len + id] = rho2[by_column ? idx : id];  // alpha numerator (reference: by position)
             if (scaled) {
                 double snew = (niter == 1) ? last_err[idx]
                                            : last_err[idx] + std::fabs(beta) * s_col[idx] * pn_col[idx];
@@ -673,8 +673,11 @@ static void bcg_iterate(Mat *A, Mat *B, int if_shift, double shift, int n, int l
             bcg_p_update(n, Pd, ldp, pm, Rd, ldr, rm, d_beta, d_sold, d_snew, niter == 1, nu);
         else
             bcg_p_update(n, Pd, ldp, pm, Rd, ldr, rm, d_beta, niter == 1, nu);
-        spmm(A, Pd, ldp, Wd, ldw, nu, pm, wm);
-        if (if_shift == 1) {
+        if (if_shift != 1) {
+            // product and its p.w (and p.p) in one pass
+            spmm_dot(A, Pd, ldp, Wd, ldw, nu, pm, wm, d_ptw);
+        } else {
+            spmm(A, Pd, ldp, Wd, ldw, nu, pm, wm);
             std::vector<double> sh(nu, shift);
             double *d_sh = small_buffer((size_t)nu);
             upload_async(d_sh, sh.data(), nu * sizeof(double));
@@ -686,24 +689,26 @@ static void bcg_iterate(Mat *A, Mat *B, int if_shift, double shift, int n, int l
             }
         }
         if constexpr (scaled) {
-            coldots_pw(n, Pd, ldp, pm, Wd, ldw, wm, nu, d_ptw);
+            if (if_shift == 1) coldots_pw(n, Pd, ldp, pm, Wd, ldw, wm, nu, d_ptw);
             bcg_xr_update(n, Xd, ldx, xm, Rd, ldr, rm, Pd, ldp, pm, Wd, ldw, wm, d_rho, d_ptw,
                           d_snew, nu, d_rr);
             HIP_CHECK(hipMemcpyAsync(h + len, d_ptw + nu, nu * sizeof(double), hipMemcpyDeviceToHost,
                                      s.stream));
         } else {
-            coldots(n, Pd, ldp, pm, Wd, ldw, wm, nu, d_ptw);
+            if (if_shift == 1) coldots(n, Pd, ldp, pm, Wd, ldw, wm, nu, d_ptw);
             bcg_xr_update(n, Xd, ldx, xm, Rd, ldr, rm, Pd, ldp, pm, Wd, ldw, wm, d_rho, d_ptw, nu,
                           d_rr);
         }
         HIP_CHECK(hipMemcpyAsync(h, d_rr, nu * sizeof(double), hipMemcpyDeviceToHost, s.stream));
         HIP_CHECK(hipStreamSynchronize(s.stream));
         for (int id = 0; id < nu; ++id) {
-            rho1[id] = rho2[id];
-            rho2[id] = h[id];
+            const int q = by_column ? unlock[id] : id;
+            rho1[q] = rho2[q];
+            rho2[q] = h[id];
             if (scaled) pn_col[unlock[id]] = std::sqrt(h[len + id]);
         }
-        for (int id = 0; id < nu; ++id) last_err[unlock[id]] = std::sqrt(std::fabs(rho2[id]));
+        for (int id = 0; id < nu; ++id)
+            last_err[unlock[id]] = std::sqrt(std::fabs(rho2[by_column ? unlock[id] : id]));
         num_unlock = 0;
         for (int id = 0; id < nu; ++id) {
             const int idx = unlock[id];
@@ -715,7 +720,7 @@ static void bcg_iterate(Mat *A, Mat *B, int if_shift, double shift, int n, int l
 
 static void block_cg(void *Amat, int if_shift, double shift, void *Bmat, void **RHS, void **V,
                      int *start, int *end, int max_it, double rate, void **CG_P, void **CG_W,
-                     double *dtmp, int *itmp)
+                     double *dtmp, int *itmp, int by_column = 0)
 {
     Mat *A = as_mat(Amat);
     Mat *B = Bmat ? as_mat(Bmat) : nullptr;
@@ -803,20 +808,21 @@ static void block_cg(void *Amat, int if_shift, double shift, void *Bmat, void **
         float *Tf = (if_shift == 1 && B) ? scratch_f32(2, n, len, &ltf) : nullptr;
         bcg_iterate<float>(A, B, if_shift, shift, n, len, Rd, ldr, start[0], Xd, ldx, start[1], Pf,
                            lpf, 0, Wf, lwf, 0, Tf, ltf, max_it, rate, rho1, rho2, err, last_err,
-                           unlock, num_unlock);
+                           unlock, num_unlock, by_column);
     } else {
         bcg_iterate<double>(A, B, if_shift, shift, n, len, Rd, ldr, start[0], Xd, ldx, start[1], Pd,
                             ldp, p0, Wd, ldw, w0, T, ldt, max_it, rate, rho1, rho2, err, last_err,
-                            unlock, num_unlock);
+                            unlock, num_unlock, by_column);
     }
 }
 
 static void ext_block_cg(void *A, GCGE_INT if_shift, GCGE_DOUBLE shift, void *B, void **RHS,
                          void **V, GCGE_INT *start, GCGE_INT *end, GCGE_INT max_it,
                          GCGE_DOUBLE rate, GCGE_OPS *, void **CG_P, void **CG_W,
-                         GCGE_DOUBLE *dtmp, GCGE_INT *itmp)
+                         GCGE_DOUBLE *dtmp, GCGE_INT *itmp, GCGE_INT by_column)
 {
-    block_cg(A, if_shift, shift, B, RHS, V, start, end, max_it, rate, CG_P, CG_W, dtmp, itmp);
+    block_cg(A, if_shift, shift, B, RHS, V, start, end, max_it, rate, CG_P, CG_W, dtmp, itmp,
+             by_column);
 }
 
 static void op_multi_linear_solver(void *A, void **b, void **x, int *start, int *end, GCGE_OPS *)

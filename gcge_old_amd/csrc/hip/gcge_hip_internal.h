@@ -117,6 +117,13 @@ void spmm(const Mat *A, const double *X, long ldx, double *Y, long ldy, int k,
 void spmm(const Mat *A, const float *X, long ldx, float *Y, long ldy, int k,
           const int *d_xmap = nullptr, const int *d_ymap = nullptr);
 void to_float(const double *src, float *dst, size_t len);
+// block-CG product: W(:, wmap[c]) = A P(:, pmap[c]) and, from the same pass,
+// d_out[c] = P(:, pmap[c]) . W(:, wmap[c]); the float32 form also gives
+// d_out[k + c] = P(:, pmap[c]) . P(:, pmap[c]) (sums in float64)
+void spmm_dot(const Mat *A, const double *P, long ldp, double *W, long ldw, int k,
+              const int *d_pmap, const int *d_wmap, double *d_out);
+void spmm_dot(const Mat *A, const float *P, long ldp, float *W, long ldw, int k, const int *d_pmap,
+              const int *d_wmap, double *d_out);
 // host G(m x k, ldg) = V^T W
 void gram(int n, const double *V, long ldv, int m, const double *W, long ldw, int k, double *G,
           int ldg, bool sync_result = true);

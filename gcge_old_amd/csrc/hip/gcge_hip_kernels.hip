@@ -73,13 +73,18 @@ __device__ __forceinline__ double block_sum(double v, double *lds)
 // taken four at a time: their column indices and values are loaded first, so
 // the gathers of X for four entries are in flight together; each accumulator
 // still adds the entries in order.
-template <int CK, typename VT, typename XT>
+// DOT > 0 (block CG, X = P): also p.w (DOT 1) and p.p (DOT 2) of the lane's
+// row, summed over the wave into dpart[c * nslices + slice] and
+// dpart[(k + c) * nslices + slice].
+template <int CK, int DOT = 0, typename VT, typename XT>
 __device__ __forceinline__ void sell_chunk(int row, int n, int w, const int *__restrict__ cp,
                                            const VT *__restrict__ vp,
                                            const XT *__restrict__ X, long ldx,
                                            XT *__restrict__ Y, long ldy, int c0,
                                            const int *__restrict__ xmap,
-                                           const int *__restrict__ ymap)
+                                           const int *__restrict__ ymap,
+                                           double *__restrict__ dpart = nullptr, int slice = 0,
+                                           int nslices = 0, int k = 0)
 {
     const XT *xc[CK];
 #pragma unroll
@@ -115,27 +120,45 @@ __device__ __forceinline__ void sell_chunk(int row, int n, int w, const int *__r
 #pragma unroll
         for (int cc = 0; cc < CK; ++cc) Y[(long)(ymap ? ymap[c0 + cc] : c0 + cc) * ldy + row] = acc[cc];
     }
-}
-
-// remainder chunk of R0 <= r < MAXCK columns, r chosen at run time
-template <int R, int MAXCK, typename VT, typename XT>
-__device__ __forceinline__ void sell_remainder(int r, int row, int n, int w, const int *cp,
-                                               const VT *vp, const XT *X, long ldx, XT *Y,
-                                               long ldy, int c0, const int *xmap,
-                                               const int *ymap)
-{
-    if constexpr (R < MAXCK) {
-        if (r == R)
-            sell_chunk<R>(row, n, w, cp, vp, X, ldx, Y, ldy, c0, xmap, ymap);
-        else
-            sell_remainder<R + 1, MAXCK>(r, row, n, w, cp, vp, X, ldx, Y, ldy, c0, xmap, ymap);
+    if constexpr (DOT > 0) {
+#pragma unroll
+        for (int cc = 0; cc < CK; ++cc) {
+            const double pv = row < n ? (double)xc[cc][row] : 0.0;
+            const double pw = wave_sum(pv * (double)acc[cc]);
+            double pp = 0.0;
+            if constexpr (DOT > 1) pp = wave_sum(pv * pv);
+            if (threadIdx.x % kWave == 0) {
+                dpart[(long)(c0 + cc) * nslices + slice] = pw;
+                if constexpr (DOT > 1) dpart[(long)(k + c0 + cc) * nslices + slice] = pp;
+            }
+        }
     }
 }
 
-// all k columns: chunks of MAXCK, then one chunk of the remainder (MAXCK is
-// picked per launch so that small k does not pay the registers of wide
-// chunks)
-template <typename VT, typename XT, int MAXCK>
+// remainder chunk of R0 <= r < MAXCK columns, r chosen at run time
+template <int R, int MAXCK, int DOT, typename VT, typename XT>
+__device__ __forceinline__ void sell_remainder(int r, int row, int n, int w, const int *cp,
+                                               const VT *vp, const XT *X, long ldx, XT *Y,
+                                               long ldy, int c0, const int *xmap,
+                                               const int *ymap, double *dpart, int slice,
+                                               int nslices, int k)
+{
+    if constexpr (R < MAXCK) {
+        if (r == R)
+            sell_chunk<R, DOT>(row, n, w, cp, vp, X, ldx, Y, ldy, c0, xmap, ymap, dpart, slice,
+                               nslices, k);
+        else
+            sell_remainder<R + 1, MAXCK, DOT>(r, row, n, w, cp, vp, X, ldx, Y, ldy, c0, xmap, ymap,
+                                              dpart, slice, nslices, k);
+    }
+}
+
+// SPLIT waves of a workgroup share one slice, each taking a contiguous part
+// of the k columns (chunks of MAXCK, then one remainder chunk); the slice's
+// entries reach the second wave from L1.  MAXCK and SPLIT are picked per
+// launch: small k does not pay the registers of wide chunks, and splitting
+// the columns gives more wavefronts to hide the gather latency.
+template <typename VT, typename XT, int MAXCK, int SPLIT, int DOT>
 __global__ __launch_bounds__(kBlock) void k_sell_spmm(int n, int nslices,
                                                      const int *__restrict__ sptr,
                                                      const int *__restrict__ sw,
@@ -144,43 +167,73 @@ __global__ __launch_bounds__(kBlock) void k_sell_spmm(int n, int nslices,
                                                      const XT *__restrict__ X, long ldx,
                                                      XT *__restrict__ Y, long ldy, int k,
                                                      const int *__restrict__ xmap,
-                                                     const int *__restrict__ ymap)
+                                                     const int *__restrict__ ymap,
+                                                     double *__restrict__ dpart)
 {
+    constexpr int kSlicesPerBlock = kBlock / kWave / SPLIT;
     const int blk = xcd_remap(blockIdx.x, gridDim.x);
-    const int slice = blk * (kBlock / kWave) + threadIdx.x / kWave;
+    const int wid = threadIdx.x / kWave;
+    const int slice = blk * kSlicesPerBlock + wid / SPLIT;
     if (slice >= nslices) return;
+    const int part = wid % SPLIT;
+    const int per = (k + SPLIT - 1) / SPLIT;
+    const int cb = part * per, ce = min(k, cb + per);
+    if (cb >= ce) return;
     const int lane = threadIdx.x % kWave;
     const int row = slice * kWave + lane;
     const int off = sptr[slice];
     const int w = sw[slice];
     const int *cp = scol + off + lane;
     const VT *vp = sval + off + lane;
-    int c0 = 0;
-    for (; c0 + MAXCK <= k; c0 += MAXCK)
-        sell_chunk<MAXCK>(row, n, w, cp, vp, X, ldx, Y, ldy, c0, xmap, ymap);
-    if (c0 < k) sell_remainder<1, MAXCK>(k - c0, row, n, w, cp, vp, X, ldx, Y, ldy, c0, xmap, ymap);
+    int c0 = cb;
+    for (; c0 + MAXCK <= ce; c0 += MAXCK)
+        sell_chunk<MAXCK, DOT>(row, n, w, cp, vp, X, ldx, Y, ldy, c0, xmap, ymap, dpart, slice,
+                               nslices, k);
+    if (c0 < ce)
+        sell_remainder<1, MAXCK, DOT>(ce - c0, row, n, w, cp, vp, X, ldx, Y, ldy, c0, xmap, ymap,
+                                      dpart, slice, nslices, k);
 }
 
-template <typename VT, typename XT>
-void launch_sell_spmm(const Mat *A, const VT *vals, const XT *X, long ldx, XT *Y, long ldy, int k,
-                      const int *d_xmap, const int *d_ymap, hipEvent_t ev0, hipEvent_t ev1)
+template <typename VT, typename XT, int MAXCK, int SPLIT, int DOT>
+void launch_sell_spmm_v(const Mat *A, const VT *vals, const XT *X, long ldx, XT *Y, long ldy, int k,
+                        const int *d_xmap, const int *d_ymap, double *dpart, hipEvent_t ev0,
+                        hipEvent_t ev1)
 {
-    const int nb = (A->nslices + kBlock / kWave - 1) / (kBlock / kWave);
-    hipStream_t st = state().stream;
+    constexpr int spb = kBlock / kWave / SPLIT;
+    const int nb = (A->nslices + spb - 1) / spb;
     // hipExtLaunchKernelGGL records the (optional) events in the dispatch packet
-    if (k <= 4)
-        hipExtLaunchKernelGGL((k_sell_spmm<VT, XT, 4>), dim3(nb), dim3(kBlock), 0, st, ev0, ev1, 0, A->nrows,
-                              A->nslices, A->d_slice_ptr, A->d_slice_w, A->d_col, vals, X, ldx, Y, ldy, k,
-                              d_xmap, d_ymap);
-    else if (k <= 8)
-        hipExtLaunchKernelGGL((k_sell_spmm<VT, XT, 8>), dim3(nb), dim3(kBlock), 0, st, ev0, ev1, 0, A->nrows,
-                              A->nslices, A->d_slice_ptr, A->d_slice_w, A->d_col, vals, X, ldx, Y, ldy, k,
-                              d_xmap, d_ymap);
-    else
-        hipExtLaunchKernelGGL((k_sell_spmm<VT, XT, 16>), dim3(nb), dim3(kBlock), 0, st, ev0, ev1, 0, A->nrows,
-                              A->nslices, A->d_slice_ptr, A->d_slice_w, A->d_col, vals, X, ldx, Y, ldy, k,
-                              d_xmap, d_ymap);
+    hipExtLaunchKernelGGL((k_sell_spmm<VT, XT, MAXCK, SPLIT, DOT>), dim3(nb), dim3(kBlock), 0,
+                          state().stream, ev0, ev1, 0, A->nrows, A->nslices, A->d_slice_ptr,
+                          A->d_slice_w, A->d_col, vals, X, ldx, Y, ldy, k, d_xmap, d_ymap, dpart);
     HIP_CHECK(hipGetLastError());
+}
+
+// columns per wave after the split: 1..4 -> MAXCK 4, 5..8 -> 8, else 16;
+// GCGE_HIP_SPMM_SPLIT = 1 | 2 | 4 overrides the split (for measurement)
+template <int DOT, typename VT, typename XT>
+void launch_sell_spmm(const Mat *A, const VT *vals, const XT *X, long ldx, XT *Y, long ldy, int k,
+                      const int *d_xmap, const int *d_ymap, double *dpart, hipEvent_t ev0,
+                      hipEvent_t ev1)
+{
+    static const int forced = [] {
+        const char *e = getenv("GCGE_HIP_SPMM_SPLIT");
+        return e ? atoi(e) : 0;
+    }();
+    int split = forced;
+    // measured on MI355X (Q1 64^3, k = 1..50): one wave per slice is best up to
+    // 10 columns, two up to 20, four beyond
+    if (split != 1 && split != 2 && split != 4) split = k <= 10 ? 1 : (k <= 20 ? 2 : 4);
+    const int per = (k + split - 1) / split;
+#define GCGE_SPMM_LAUNCH(MC, SP) \
+    launch_sell_spmm_v<VT, XT, MC, SP, DOT>(A, vals, X, ldx, Y, ldy, k, d_xmap, d_ymap, dpart, ev0, ev1)
+    if (split == 1) {
+        if (per <= 4) GCGE_SPMM_LAUNCH(4, 1); else if (per <= 8) GCGE_SPMM_LAUNCH(8, 1); else GCGE_SPMM_LAUNCH(16, 1);
+    } else if (split == 2) {
+        if (per <= 4) GCGE_SPMM_LAUNCH(4, 2); else if (per <= 8) GCGE_SPMM_LAUNCH(8, 2); else GCGE_SPMM_LAUNCH(16, 2);
+    } else {
+        if (per <= 4) GCGE_SPMM_LAUNCH(4, 4); else if (per <= 8) GCGE_SPMM_LAUNCH(8, 4); else GCGE_SPMM_LAUNCH(16, 4);
+    }
+#undef GCGE_SPMM_LAUNCH
 }
 
 __global__ __launch_bounds__(kBlock) void k_to_float(size_t len, const double *__restrict__ src,
@@ -699,7 +752,8 @@ void spmm(const Mat *A, const double *X, long ldx, double *Y, long ldy, int k, c
     const double bytes = (double)A->nnz * (8 + 4) + 4.0 * (A->nrows + 1) +
                          8.0 * (double)k * (A->ncols + A->nrows);
     ProfScope prof("spmm", bytes, true);
-    launch_sell_spmm(A, A->d_val, X, ldx, Y, ldy, k, d_xmap, d_ymap, prof.start_event(), prof.stop_event());
+    launch_sell_spmm<0>(A, A->d_val, X, ldx, Y, ldy, k, d_xmap, d_ymap, nullptr, prof.start_event(),
+                        prof.stop_event());
 }
 
 void spmm(const Mat *A, const float *X, long ldx, float *Y, long ldy, int k, const int *d_xmap,
@@ -710,7 +764,42 @@ void spmm(const Mat *A, const float *X, long ldx, float *Y, long ldy, int k, con
     const double bytes = (double)A->nnz * (4 + 4) + 4.0 * (A->nrows + 1) +
                          4.0 * (double)k * (A->ncols + A->nrows);
     ProfScope prof("spmm32", bytes, true);
-    launch_sell_spmm(A, A->d_val32, X, ldx, Y, ldy, k, d_xmap, d_ymap, prof.start_event(), prof.stop_event());
+    launch_sell_spmm<0>(A, A->d_val32, X, ldx, Y, ldy, k, d_xmap, d_ymap, nullptr, prof.start_event(),
+                        prof.stop_event());
+}
+
+void spmm_dot(const Mat *A, const double *P, long ldp, double *W, long ldw, int k,
+              const int *d_pmap, const int *d_wmap, double *d_out)
+{
+    if (k <= 0 || A->nrows <= 0) return;
+    const double bytes = (double)A->nnz * (8 + 4) + 4.0 * (A->nrows + 1) +
+                         8.0 * (double)k * (A->ncols + 2.0 * A->nrows);
+    double *dpart = partial_buffer((size_t)A->nslices * k);
+    {
+        ProfScope prof("spmm", bytes, true);
+        launch_sell_spmm<1>(A, A->d_val, P, ldp, W, ldw, k, d_pmap, d_wmap, dpart, prof.start_event(),
+                            prof.stop_event());
+    }
+    hipLaunchKernelGGL(k_reduce_rows, dim3(k), dim3(kBlock), 0, state().stream, dpart, A->nslices, d_out);
+    HIP_CHECK(hipGetLastError());
+}
+
+void spmm_dot(const Mat *A, const float *P, long ldp, float *W, long ldw, int k, const int *d_pmap,
+              const int *d_wmap, double *d_out)
+{
+    if (k <= 0 || A->nrows <= 0) return;
+    if (!A->d_val32) fail_msg("float32 SpMM on a matrix without float32 values");
+    const double bytes = (double)A->nnz * (4 + 4) + 4.0 * (A->nrows + 1) +
+                         4.0 * (double)k * (A->ncols + 2.0 * A->nrows);
+    double *dpart = partial_buffer((size_t)A->nslices * 2 * k);
+    {
+        ProfScope prof("spmm32", bytes, true);
+        launch_sell_spmm<2>(A, A->d_val32, P, ldp, W, ldw, k, d_pmap, d_wmap, dpart, prof.start_event(),
+                            prof.stop_event());
+    }
+    hipLaunchKernelGGL(k_reduce_rows, dim3(2 * k), dim3(kBlock), 0, state().stream, dpart, A->nslices,
+                       d_out);
+    HIP_CHECK(hipGetLastError());
 }
 
 void to_float(const double *src, float *dst, size_t len)

@@ -42,7 +42,8 @@ typedef struct GCGE_OPS_EXT_ {
     void (*BlockCG)(void *A, GCGE_INT if_shift, GCGE_DOUBLE shift, void *B,
                     void **RHS, void **V, GCGE_INT *start, GCGE_INT *end,
                     GCGE_INT max_it, GCGE_DOUBLE rate, GCGE_OPS *ops,
-                    void **CG_P, void **CG_W, GCGE_DOUBLE *dtmp, GCGE_INT *itmp);
+                    void **CG_P, void **CG_W, GCGE_DOUBLE *dtmp, GCGE_INT *itmp,
+                    GCGE_INT by_column);
     /* Wait for all queued device work (timers call it before reading the clock). */
     void (*Synchronize)(GCGE_OPS *ops);
     /* Y(:, ycols[c]) = M X(:, xcols[c]) for c < k: the column loop of
@@ -172,6 +173,11 @@ typedef struct GCGE_PARA_ {
      * directions); 0 (default) zeroes that coupling and solves only the
      * trailing block, as src/gcge_rayleighritz.c:507-805 does */
     GCGE_INT full_rr;
+    /* added: 1 keeps the block CG scalars of every column under that column
+     * (a plain CG per column); 0 (default) stores them by position in the
+     * list of unconverged columns and reads beta by column, as
+     * src/gcge_cg.c:405,467-476 does */
+    GCGE_INT bcg_by_column;
     /* added: optional per-check record; row r (r < history_cap) holds
      * num_iter, num_conv_continuous, then eval[0..nev-1] after the r-th
      * convergence check; history_len counts the rows written */
@@ -282,6 +288,12 @@ void GCGE_BCG(void *Matrix, GCGE_INT if_shift, GCGE_DOUBLE shift, void *B, void 
               void **V, GCGE_INT *start, GCGE_INT *end, GCGE_INT max_it, GCGE_DOUBLE rate,
               GCGE_OPS *ops, void *V_tmp1, void **V_tmp2, void **V_tmp3,
               GCGE_DOUBLE *subspace_dtmp, GCGE_INT *subspace_itmp);
+/* GCGE_BCG with the scalar bookkeeping selectable (by_column: see
+ * GCGE_PARA.bcg_by_column) */
+void GCGE_BlockCG(void *Matrix, GCGE_INT if_shift, GCGE_DOUBLE shift, void *B, void **RHS,
+                  void **V, GCGE_INT *start, GCGE_INT *end, GCGE_INT max_it, GCGE_DOUBLE rate,
+                  GCGE_OPS *ops, void *V_tmp1, void **V_tmp2, void **V_tmp3,
+                  GCGE_DOUBLE *subspace_dtmp, GCGE_INT *subspace_itmp, GCGE_INT by_column);
 void GCGE_Orthonormalization(void **V, GCGE_INT start, GCGE_INT *end, void *B, GCGE_OPS *ops,
                              GCGE_PARA *para, void **V_tmp, GCGE_DOUBLE *d_tmp);
 void GCGE_StableMultiOrthonormalization(void **V, GCGE_INT start, GCGE_INT *end, void *B,
