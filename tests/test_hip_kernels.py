@@ -46,7 +46,7 @@ MATS = {
 
 
 @pytest.mark.parametrize("name", sorted(MATS))
-@pytest.mark.parametrize("k", [1, 2, 3, 5, 8, 13, 17])
+@pytest.mark.parametrize("k", [1, 2, 3, 5, 8, 10, 13, 17, 33])
 def test_spmm_matches_oracle(name, k):
     A = MATS[name]()
     rng = _rng(k)
