@@ -84,6 +84,10 @@ def _declare(name: str, lib: C.CDLL) -> None:
                                                        C.c_void_p]
         lib.GCGE_HIP_MultiLinearSolve.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
                                                   C.c_int, C.c_int]
+        lib.GCGE_HIP_PatternGradient.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                                 C.c_int, C.c_int, dptr, dptr]
+        lib.GCGE_HIP_Q1ElementGradient.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_void_p,
+                                                   C.c_int, C.c_int, dptr, dptr]
         lib.GCGE_HIP_Synchronize.argtypes = []
         lib.GCGE_HIP_KernelTime.argtypes = [C.c_char_p, dptr, iptr]
         lib.GCGE_HIP_KernelTime.restype = C.c_int

@@ -235,6 +235,7 @@ Mat *create_mat(int nrows, int ncols, int nnz, const int *rowptr, const int *col
     m->sell_len = len;
     std::vector<int> scol(std::max<size_t>(len, 1));
     std::vector<double> sval(std::max<size_t>(len, 1));
+    std::vector<int> spos(std::max<size_t>(len, 1), -1);
     for (int s = 0; s < ns; ++s) {
         for (int lane = 0; lane < 64; ++lane) {
             const int r = s * 64 + lane;
@@ -244,6 +245,7 @@ Mat *create_mat(int nrows, int ncols, int nnz, const int *rowptr, const int *col
                 if (j < len_r) {
                     scol[pos] = colidx[rowptr[r] + j];
                     sval[pos] = vals[rowptr[r] + j];
+                    spos[pos] = rowptr[r] + j;
                 } else {  // padding: a zero entry in the row's own column
                     scol[pos] = r < ncols ? r : 0;
                     sval[pos] = 0.0;
@@ -256,6 +258,8 @@ Mat *create_mat(int nrows, int ncols, int nnz, const int *rowptr, const int *col
     HIP_CHECK(hipMalloc((void **)&m->d_slice_w, std::max(ns, 1) * sizeof(int)));
     HIP_CHECK(hipMalloc((void **)&m->d_col, scol.size() * sizeof(int)));
     HIP_CHECK(hipMalloc((void **)&m->d_val, sval.size() * sizeof(double)));
+    HIP_CHECK(hipMalloc((void **)&m->d_csrpos, spos.size() * sizeof(int)));
+    HIP_CHECK(hipMemcpyAsync(m->d_csrpos, spos.data(), spos.size() * sizeof(int), hipMemcpyHostToDevice, st.stream));
     HIP_CHECK(hipMemcpyAsync(m->d_slice_ptr, sptr.data(), (ns + 1) * sizeof(int), hipMemcpyHostToDevice, st.stream));
     if (ns) HIP_CHECK(hipMemcpyAsync(m->d_slice_w, sw.data(), ns * sizeof(int), hipMemcpyHostToDevice, st.stream));
     HIP_CHECK(hipMemcpyAsync(m->d_col, scol.data(), scol.size() * sizeof(int), hipMemcpyHostToDevice, st.stream));
@@ -1139,6 +1143,7 @@ void GCGE_HIP_MatDestroy(void *mat)
     HIP_CHECK(hipFree(m->d_col));
     HIP_CHECK(hipFree(m->d_val));
     if (m->d_val32) HIP_CHECK(hipFree(m->d_val32));
+    if (m->d_csrpos) HIP_CHECK(hipFree(m->d_csrpos));
     for (size_t i = 0; i < g_mat_host.size(); ++i)
         if (g_mat_host[i].first == m) {
             delete g_mat_host[i].second;
@@ -1159,6 +1164,63 @@ void GCGE_HIP_MatSetPrecision(void *mat, int fp32)
         HIP_CHECK(hipStreamSynchronize(state().stream));
     }
     m->fp32 = fp32 ? 1 : 0;
+}
+
+// ---- adjoint entry points ----------------------------------------------------------------
+static double *adjoint_buffer(size_t doubles)
+{
+    static double *buf = nullptr;
+    static size_t cap = 0;
+    if (doubles > cap || !buf) {
+        if (buf) {
+            HIP_CHECK(hipStreamSynchronize(state().stream));
+            HIP_CHECK(hipFree(buf));
+        }
+        HIP_CHECK(hipMalloc((void **)&buf, std::max<size_t>(doubles, 1) * sizeof(double)));
+        cap = doubles;
+    }
+    return buf;
+}
+
+void GCGE_HIP_PatternGradient(void *mat, void **X, int xs, void **Y, int ys, int k, const double *w,
+                              double *grad)
+{
+    require_init();
+    Mat *A = as_mat(mat);
+    MultiVec *Xm = as_mv(X), *Ym = Y ? as_mv(Y) : Xm;
+    if (!Y) ys = xs;
+    if (k <= 0 || A->nnz == 0) return;
+    check_range(Xm, xs, k, "PatternGradient X");
+    check_range(Ym, ys, k, "PatternGradient Y");
+    if (Xm->n != A->nrows || Xm->n != A->ncols || Ym->n != Xm->n)
+        fail_msg("GCGE_HIP_PatternGradient: the matrix must be square with the blocks' row count");
+    double *d_w = small_buffer((size_t)k);
+    upload_async(d_w, w, (size_t)k * sizeof(double));
+    double *d_grad = adjoint_buffer((size_t)A->nnz);
+    pattern_gradient(A, col_ptr(Xm, xs), Xm->ld, col_ptr(Ym, ys), Ym->ld, k, d_w, d_grad);
+    HIP_CHECK(hipMemcpyAsync(grad, d_grad, (size_t)A->nnz * sizeof(double), hipMemcpyDeviceToHost, state().stream));
+    HIP_CHECK(hipStreamSynchronize(state().stream));
+}
+
+void GCGE_HIP_Q1ElementGradient(int E, void **X, int xs, void **Y, int ys, int k, const double *w,
+                                double *grad)
+{
+    require_init();
+    MultiVec *Xm = as_mv(X), *Ym = Y ? as_mv(Y) : Xm;
+    if (!Y) ys = xs;
+    if (E < 2 || k <= 0) return;
+    check_range(Xm, xs, k, "Q1ElementGradient X");
+    check_range(Ym, ys, k, "Q1ElementGradient Y");
+    const long N = E - 1;
+    if ((long)Xm->n != N * N * N || Ym->n != Xm->n)
+        fail_msg("GCGE_HIP_Q1ElementGradient: the blocks must have (E-1)^3 rows");
+    const size_t ne = (size_t)E * E * E;
+    double *d_w = small_buffer((size_t)k);
+    upload_async(d_w, w, (size_t)k * sizeof(double));
+    double *d_grad = adjoint_buffer(ne);
+    q1_element_gradient(E, col_ptr(Xm, xs), Xm->ld, col_ptr(Ym, ys), Ym->ld, k, d_w, d_grad);
+    HIP_CHECK(hipMemcpyAsync(grad, d_grad, ne * sizeof(double), hipMemcpyDeviceToHost, state().stream));
+    HIP_CHECK(hipStreamSynchronize(state().stream));
 }
 
 void **GCGE_HIP_MultiVecCreate(int n, int ncols) { return (void **)create_mv(n, ncols); }

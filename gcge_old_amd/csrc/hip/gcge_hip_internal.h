@@ -83,6 +83,7 @@ struct Mat {
     float *d_val32;     // float32 copy (mixed precision), or null
     size_t sell_len;    // padded entries
     int fp32;           // use d_val32 in the inner solves
+    int *d_csrpos;      // CSR position of each sliced-ELLPACK entry, -1 for padding
 };
 constexpr uint32_t kMatMagic = 0x4d415431u;  // "MAT1"
 
@@ -168,5 +169,14 @@ void axpy_cols(int n, const double *d_alpha, const double *X, long ldx, const in
                double *Y, long ldy, const int *d_ymap, int k);
 void axpy_cols(int n, const double *d_alpha, const float *X, long ldx, const int *d_xmap,
                float *Y, long ldy, const int *d_ymap, int k);
+
+// ---- adjoint (gcge_hip_adjoint.hip) ----------------------------------------------------
+// d_grad[p] = sum_c w[c] (X(r, c) Y(q, c) + Y(r, c) X(q, c)) / 2 for the stored
+// entry (r, q) at CSR position p of A (d_w: k weights on the device)
+void pattern_gradient(const Mat *A, const double *X, long ldx, const double *Y, long ldy, int k,
+                      const double *d_w, double *d_grad);
+// Q1 elements of the E^3 mesh: d_grad[e] = (h / 36) sum_c w[c] x_c|e^T K36 y_c|e
+void q1_element_gradient(int E, const double *X, long ldx, const double *Y, long ldy, int k,
+                         const double *d_w, double *d_grad);
 
 }  // namespace gcge_hip

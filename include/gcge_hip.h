@@ -107,6 +107,24 @@ void GCGE_HIP_MultiLinearSolve(void *mat, void **b, int bs, void **x, int xs, in
  * driver GCGE_SolveWithBackend (gcge.h). */
 const void *GCGE_HIP_Backend(void);
 
+/* ---- adjoints (no reference counterpart: gradients of losses on computed
+ *      eigenpairs; reference eigenpairs are B-normalised, x^T B x = 1) ---- */
+/* grad[p] = sum_c w[c] (X(i, xs+c) Y(j, ys+c) + Y(i, ys+c) X(j, xs+c)) / 2 for
+ * the stored entry (i, j) at position p of the CSR arrays given to
+ * GCGE_HIP_MatCreateCSR (nnz host doubles).  Y NULL means Y = X.  With
+ * Y = X = eigenvectors: w = dL/dlambda gives dL/dA on the pattern, w =
+ * -lambda dL/dlambda gives dL/dB. */
+void GCGE_HIP_PatternGradient(void *mat, void **X, int xs, void **Y, int ys, int k, const double *w,
+                              double *grad);
+/* Trilinear Q1 elements of the uniform E^3 mesh of the unit cube, interior
+ * nodes numbered (ix-1) + N(iy-1) + N^2(iz-1), N = E-1 (Dirichlet nodes
+ * eliminated), element e = ex + E ey + E^2 ez with stiffness h Khat:
+ * grad[e] = sum_c w[c] x_c|e^T (h Khat) y_c|e (E^3 host doubles).  With
+ * K = sum_e kappa_e h Khat and Y = X = eigenvectors, w = dL/dlambda gives
+ * dL/dkappa. */
+void GCGE_HIP_Q1ElementGradient(int E, void **X, int xs, void **Y, int ys, int k, const double *w,
+                                double *grad);
+
 /* ---- measurement --------------------------------------------------------- */
 /* Per-kernel device time, measured with HIP events on the backend stream
  * while enabled (on = 1: every launch; on = 2: the SpMM launches only, whose
