@@ -30,7 +30,10 @@ def load(name: str) -> C.CDLL:
             raise RuntimeError(
                 f"{path} is not built; run `python -c 'import __graft_entry__ as g; g.build()'` "
                 "from the repository root")
-        _libs[name] = C.CDLL(path, mode=C.RTLD_GLOBAL)
+        # local binding: published globally, the ROCm libraries this library
+        # links (/opt/rocm) would capture the symbols of the ROCm copies a
+        # later `import torch` loads, and that import would crash
+        _libs[name] = C.CDLL(path, mode=C.RTLD_LOCAL)
         _declare(name, _libs[name])
     return _libs[name]
 

@@ -71,3 +71,18 @@ def test_core_parameters_and_block_size_rules():
     # -gcge_cg_rate is read with atoi: "0.5" becomes 0 (CG runs to cg_max_it)
     out = oracle.solve(A, None, 12, ["-gcge_cg_rate", "0.5", "-gcge_ev_max_it", "3"])
     assert np.all(np.isfinite(out["eval"]))
+
+
+def test_torch_import_after_the_libraries():
+    """Loading both libraries and then importing torch in one process must
+    work (the libraries are opened with local symbol binding; opened
+    globally, torch's own ROCm copies bind to /opt/rocm's and the import
+    crashes).  Runs in a subprocess; needs no GPU."""
+    import subprocess
+    import sys
+
+    code = ("import gcge_old_amd._native as n; n.core(); n.hip(); import torch; "
+            "print('ok', torch.__version__)")
+    p = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=600,
+                       cwd=ROOT)
+    assert p.returncode == 0 and p.stdout.startswith("ok"), (p.returncode, p.stderr[-2000:])
