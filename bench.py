@@ -59,10 +59,16 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--workload", choices=("q1-64", "q1-128-lognormal"), default="q1-64",
+    p.add_argument("--workload", choices=("q1-64", "q1-128-lognormal", "q1-64-adjoint"),
+                   default="q1-64",
                    help="q1-64: Q1 64^3, kappa = 1, nev 50 (default, float64 value); "
                         "q1-128-lognormal: Q1 128^3, lognormal kappa, nev 100, block 200 "
-                        "requested (the mixed-precision comparison on one seed)")
+                        "requested (the mixed-precision comparison on one seed); "
+                        "q1-64-adjoint: Q1 64^3 lognormal kappa, nev 20, gradient of "
+                        "sum w_k lambda_k + ||P x_1||^2 with respect to all kappa_e, "
+                        "checked by central differences on --fd elements")
+    p.add_argument("--fd", type=int, default=8, help="elements checked by finite differences "
+                                                     "(q1-64-adjoint)")
     p.add_argument("--mesh", type=int, default=None, help="elements per axis (overrides the workload)")
     p.add_argument("--nev", type=int, default=None)
     p.add_argument("--kappa-seed", type=int, default=1)
@@ -242,6 +248,91 @@ def cpu_baseline(K, M, nev, tol, iters, dev_iters):
     }
 
 
+def adjoint_workload(args):
+    """Gradient of L = sum_k w_k lambda_k + ||P x_1||^2 with respect to every
+    kappa_e of the lognormal Q1 64^3 problem (nev 20): one solve, the
+    eigenvalue adjoint (element kernel), the eigenvector adjoint (shifted
+    block CG + element kernel); central differences on a few elements."""
+    import gcge_old_amd as gcge
+    from gcge_old_amd import adjoint, problems
+
+    E = args.mesh or 64
+    nev = args.nev or 20
+    kappa = problems.lognormal_kappa(E, args.kappa_seed)
+    rng = np.random.default_rng(args.kappa_seed + 1)
+    w = rng.uniform(size=nev)
+    # tolerance 1e-9: the lognormal pencil does not reach 1e-11 in 800
+    # iterations
+    flags = ["-gcge_conv_type", "RB", "-gcge_ev_tol", str(min(args.tol, 1e-9)),
+             "-gcge_ev_max_it", "800", "-gcge_full_rr", "1", "-gcge_w_orth_type", "svqb"]
+
+    def evaluate(kv, grad):
+        K, M = problems.q1_fem_3d(E, kv)
+        dK, dM = gcge.DeviceMatrix(K), gcge.DeviceMatrix(M)
+        ts = time.perf_counter()
+        out = gcge.solve(dK, dM, nev, flags, seed=1)
+        if out["num_conv"] < nev:
+            raise RuntimeError(f"solve converged {out['num_conv']} of {nev} pairs")
+        x1 = out["evec"][:, 0]
+        mask = (np.random.default_rng(args.kappa_seed + 2).uniform(size=K.nrows) < 0.5)
+        if not grad:
+            # difference quotients want x_1 beyond the solver tolerance: one
+            # Newton (Jacobi-Davidson) step, (K - lam M) d = -r with d
+            # M-orthogonal to x_1, through the same shifted block CG
+            Ks, Ms = K.to_scipy(), M.to_scipy()
+            lam1 = float(x1 @ (Ks @ x1)) / float(x1 @ (Ms @ x1))
+            r = Ks @ x1 - lam1 * (Ms @ x1)
+            x1 = x1 + adjoint.eigvec_adjoint_solve(dK, dM, x1, lam1, -r, rate=1e-8)
+            x1 = x1 / np.sqrt(float(x1 @ (Ms @ x1)))
+        terms = (float(w @ out["eval"]), float(np.sum(x1[mask] ** 2)))
+        if not grad:
+            return terms, out, None
+        t0 = time.perf_counter()
+        g = adjoint.q1_kappa_gradient(E, out["evec"], w)
+        t1 = time.perf_counter()
+        gv, _ = adjoint.q1_eigvec_kappa_gradient(E, dK, dM, x1, out["eval"][0], 2.0 * mask * x1)
+        t2 = time.perf_counter()
+        return terms, out, (g, gv, t1 - t0, t2 - t1, t2 - ts)
+
+    # the timed span starts with the matrices resident on the device (host
+    # assembly and upload excluded) and ends with both gradients on the host
+    _, out, (g_val, g_vec, t_eval, t_vec, total) = evaluate(kappa, True)
+    print(f"# adjoint: solve {out['time']:.3f}s iters={out['num_iter']} conv={out['num_conv']}, "
+          f"eigenvalue adjoint {t_eval*1e3:.1f} ms, eigenvector adjoint {t_vec:.3f}s",
+          file=sys.stderr, flush=True)
+    # central differences of each term with the relative step 1e-3: one
+    # element moves L by ~1e-9 of its size per unit step, so smaller steps
+    # drown in the rounding of the eigenvalues (~1e-14 relative)
+    errs_val, errs_vec = [], []
+    for e in rng.choice(E ** 3, args.fd, replace=False):
+        d = 1e-3 * kappa[e]
+        kp, km = kappa.copy(), kappa.copy()
+        kp[e] += d
+        km[e] -= d
+        tp, tm = evaluate(kp, False)[0], evaluate(km, False)[0]
+        fds = [(tp[i] - tm[i]) / (2 * d) for i in range(2)]
+        errs_val.append(abs(g_val[e] - fds[0]) / max(abs(fds[0]), 1e-300))
+        errs_vec.append(abs(g_vec[e] - fds[1]) / max(abs(fds[1]), 1e-300))
+        print(f"# element {e}: eigenvalue term adjoint {g_val[e]:.10e} fd {fds[0]:.10e} "
+              f"rel {errs_val[-1]:.2e}; eigenvector term adjoint {g_vec[e]:.10e} fd {fds[1]:.10e} "
+              f"rel {errs_vec[-1]:.2e}", file=sys.stderr, flush=True)
+    homog = float(kappa @ g_val - w @ out["eval"]) / float(w @ out["eval"])
+    line = {"metric": "s for the eigenpair gradient (solve + adjoints), all kappa_e",
+            "value": round(total, 4), "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 0,
+            "ms_per_step": round(total * 1e3, 2), "higher_is_better": False, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"Q1 FEM {E}^3 lognormal kappa (seed {args.kappa_seed}), nev={nev}, "
+                                   "L = sum w_k lambda_k + ||P x_1||^2, dL/dkappa_e for all elements",
+                       "n": (E - 1) ** 3, "elements": E ** 3, "nev": nev},
+            "solve_s": round(out["time"], 4), "eigenvalue_adjoint_ms": round(t_eval * 1e3, 3),
+            "eigenvector_adjoint_s": round(t_vec, 4),
+            "fd_elements": args.fd,
+            "max_rel_err_vs_fd_eigenvalue_term": max(errs_val) if errs_val else None,
+            "max_rel_err_vs_fd_eigenvector_term": max(errs_vec) if errs_vec else None,
+            "homogeneity_rel_err": homog}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -253,6 +344,9 @@ def main():
 
     gcge.init(0)
     lib = _native.hip()
+    if args.workload == "q1-64-adjoint":
+        adjoint_workload(args)
+        return
     lognormal = args.workload == "q1-128-lognormal"
     E = args.mesh or (128 if lognormal else 64)
     if args.nev is None:
