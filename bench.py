@@ -73,6 +73,9 @@ def parse():
     p.add_argument("--nev", type=int, default=None)
     p.add_argument("--kappa-seed", type=int, default=1)
     p.add_argument("--tol", type=float, default=1e-8)
+    p.add_argument("--max-it", type=int, default=500,
+                   help="GCG iteration cap (a bounded run for profiling; solves stopped by it "
+                        "report all_converged false)")
     p.add_argument("--cpu-iters", type=int, default=3,
                    help="GCG iterations timed for the CPU baseline (0: skip)")
     p.add_argument("--mixed-steps", type=int, default=-1,
@@ -356,7 +359,7 @@ def main():
     K, M = problems.q1_fem_3d(E, kappa)
     assemble_s = time.time() - t0
     dK, dM = gcge.DeviceMatrix(K), gcge.DeviceMatrix(M)
-    flags = ["-gcge_conv_type", "RB", "-gcge_ev_tol", str(args.tol), "-gcge_ev_max_it", "500"]
+    flags = ["-gcge_conv_type", "RB", "-gcge_ev_tol", str(args.tol), "-gcge_ev_max_it", str(args.max_it)]
     if lognormal:
         flags += ["-gcge_block_size", str(2 * args.nev)]
     if not args.reference_locking:

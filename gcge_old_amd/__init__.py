@@ -37,7 +37,8 @@ def init(device: int = 0) -> None:
 
 
 class DeviceMatrix:
-    """A CSR matrix resident in device memory (sliced ELLPACK layout)."""
+    """A CSR matrix resident in device memory (sliced ELLPACK layout, plus
+    diagonal storage when its entries lie on few diagonals)."""
 
     def __init__(self, csr: problems.CSR):
         init()
@@ -58,6 +59,17 @@ class DeviceMatrix:
         block CG products in float32 (mixed-precision form)."""
         _native.hip().GCGE_HIP_MatSetPrecision(self.handle, 1 if fp32 else 0)
         self.fp32 = bool(fp32)
+
+    def set_storage(self, diagonal: bool) -> bool:
+        """Products through diagonal storage (when the matrix has it) or the
+        sliced-ELLPACK form; returns whether diagonal storage is now used."""
+        return bool(_native.hip().GCGE_HIP_MatSetStorage(self.handle, 1 if diagonal else 0))
+
+    def storage_info(self) -> dict:
+        nd, ng = np.zeros(1, dtype=np.int32), np.zeros(1, dtype=np.int32)
+        used = _native.hip().GCGE_HIP_MatStorageInfo(self.handle, _native.as_iptr(nd),
+                                                     _native.as_iptr(ng))
+        return {"diagonal": bool(used), "ndiag": int(nd[0]), "runs": int(ng[0])}
 
     def __del__(self):
         if getattr(self, "handle", None):

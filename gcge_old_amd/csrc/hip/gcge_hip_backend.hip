@@ -206,6 +206,71 @@ static MatHost *mat_host(const Mat *m)
     return nullptr;
 }
 
+// Diagonal storage for square matrices whose entries lie on few diagonals
+// (stencil and FEM matrices in their natural ordering): the sorted offsets
+// must fit kDiaMaxGroups runs {c-1, c, c+1}, rows must list their columns in
+// increasing order (the SpMM then adds a row's products in CSR order), and
+// the diagonals must be at least half full.  Otherwise the matrix keeps only
+// its sliced-ELLPACK form.
+static void build_dia(Mat *m, const int *rowptr, const int *colidx, const double *vals)
+{
+    const int n = m->nrows;
+    if (n == 0 || m->ncols != n || m->nnz == 0) return;
+    std::vector<int> offs;
+    for (int r = 0; r < n; ++r) {
+        int prev = -1;
+        size_t hint = 0;
+        for (int e = rowptr[r]; e < rowptr[r + 1]; ++e) {
+            const int c = colidx[e];
+            if (c <= prev) return;
+            prev = c;
+            const int d = c - r;
+            if (hint < offs.size() && offs[hint] == d) {
+                ++hint;
+                continue;
+            }
+            auto it = std::find(offs.begin(), offs.end(), d);
+            if (it == offs.end()) {
+                if ((int)offs.size() == 3 * kDiaMaxGroups) return;
+                offs.push_back(d);
+                hint = offs.size();
+            } else {
+                hint = (size_t)(it - offs.begin()) + 1;
+            }
+        }
+    }
+    std::sort(offs.begin(), offs.end());
+    const int nd = (int)offs.size();
+    if ((double)m->nnz < 0.5 * nd * (double)n) return;
+    DiaDesc dd{};
+    int g = 0;
+    for (int i = 0; i < nd;) {
+        if (g == kDiaMaxGroups) return;
+        const int c = offs[i] + 1;
+        dd.center[g] = c;
+        for (int p = 0; p < 3; ++p) {
+            dd.slot[g][p] = -1;
+            if (i < nd && offs[i] == c - 1 + p) dd.slot[g][p] = i++;
+        }
+        ++g;
+    }
+    dd.ngroups = g;
+    const size_t ld = (size_t)m->nslices * 64;
+    std::vector<double> dia((size_t)nd * ld, 0.0);
+    for (int r = 0; r < n; ++r)
+        for (int e = rowptr[r]; e < rowptr[r + 1]; ++e) {
+            const int s = (int)(std::lower_bound(offs.begin(), offs.end(), colidx[e] - r) - offs.begin());
+            dia[(size_t)s * ld + r] = vals[e];
+        }
+    HIP_CHECK(hipMalloc((void **)&m->d_dia, dia.size() * sizeof(double)));
+    HIP_CHECK(hipMemcpyAsync(m->d_dia, dia.data(), dia.size() * sizeof(double), hipMemcpyHostToDevice,
+                             state().stream));
+    HIP_CHECK(hipStreamSynchronize(state().stream));
+    m->dia = dd;
+    m->ndiag = nd;
+    m->use_dia = 1;
+}
+
 Mat *create_mat(int nrows, int ncols, int nnz, const int *rowptr, const int *colidx,
                 const double *vals)
 {
@@ -265,6 +330,7 @@ Mat *create_mat(int nrows, int ncols, int nnz, const int *rowptr, const int *col
     HIP_CHECK(hipMemcpyAsync(m->d_col, scol.data(), scol.size() * sizeof(int), hipMemcpyHostToDevice, st.stream));
     HIP_CHECK(hipMemcpyAsync(m->d_val, sval.data(), sval.size() * sizeof(double), hipMemcpyHostToDevice, st.stream));
     HIP_CHECK(hipStreamSynchronize(st.stream));
+    build_dia(m, rowptr, colidx, vals);
     MatHost *mh = new MatHost;
     mh->nz_count.resize(nrows);
     for (int i = 0; i < nrows; ++i) {
@@ -1144,6 +1210,8 @@ void GCGE_HIP_MatDestroy(void *mat)
     HIP_CHECK(hipFree(m->d_val));
     if (m->d_val32) HIP_CHECK(hipFree(m->d_val32));
     if (m->d_csrpos) HIP_CHECK(hipFree(m->d_csrpos));
+    if (m->d_dia) HIP_CHECK(hipFree(m->d_dia));
+    if (m->d_dia32) HIP_CHECK(hipFree(m->d_dia32));
     for (size_t i = 0; i < g_mat_host.size(); ++i)
         if (g_mat_host[i].first == m) {
             delete g_mat_host[i].second;
@@ -1163,7 +1231,29 @@ void GCGE_HIP_MatSetPrecision(void *mat, int fp32)
         to_float(m->d_val, m->d_val32, len);
         HIP_CHECK(hipStreamSynchronize(state().stream));
     }
+    if (fp32 && m->d_dia && !m->d_dia32) {
+        const size_t len = (size_t)m->ndiag * m->nslices * 64;
+        HIP_CHECK(hipMalloc((void **)&m->d_dia32, len * sizeof(float)));
+        to_float(m->d_dia, m->d_dia32, len);
+        HIP_CHECK(hipStreamSynchronize(state().stream));
+    }
     m->fp32 = fp32 ? 1 : 0;
+}
+
+int GCGE_HIP_MatSetStorage(void *mat, int diagonal)
+{
+    Mat *m = as_mat(mat);
+    HIP_CHECK(hipStreamSynchronize(state().stream));
+    m->use_dia = (diagonal && m->d_dia) ? 1 : 0;
+    return m->use_dia;
+}
+
+int GCGE_HIP_MatStorageInfo(void *mat, int *ndiag, int *ngroups)
+{
+    Mat *m = as_mat(mat);
+    if (ndiag) *ndiag = m->d_dia ? m->ndiag : 0;
+    if (ngroups) *ngroups = m->d_dia ? m->dia.ngroups : 0;
+    return m->use_dia;
 }
 
 // ---- adjoint entry points ----------------------------------------------------------------

@@ -72,6 +72,16 @@ struct ProfScope {
 };
 
 // ---- matrices --------------------------------------------------------------------
+// Diagonal storage: offsets col - row of the entries, sorted, grouped in runs
+// {c - 1, c, c + 1} around a centre c; slot[g][p] is the diagonal holding
+// offset c - 1 + p, or -1 when the matrix has no such offset.
+constexpr int kDiaMaxGroups = 9;
+struct DiaDesc {
+    int ngroups;  // 0: the matrix has no diagonal storage
+    int center[kDiaMaxGroups];
+    int slot[kDiaMaxGroups][3];
+};
+
 struct Mat {
     uint32_t magic;
     int nrows, ncols, nnz;
@@ -84,6 +94,14 @@ struct Mat {
     size_t sell_len;    // padded entries
     int fp32;           // use d_val32 in the inner solves
     int *d_csrpos;      // CSR position of each sliced-ELLPACK entry, -1 for padding
+    // diagonal storage (square matrices whose offsets fit kDiaMaxGroups runs
+    // of three): d_dia[s * nslices * 64 + row] = A(row, row + offset s), zero
+    // where the row has no such entry; used by the SpMM when use_dia is set
+    DiaDesc dia;
+    int ndiag;
+    double *d_dia;
+    float *d_dia32;
+    int use_dia;
 };
 constexpr uint32_t kMatMagic = 0x4d415431u;  // "MAT1"
 

@@ -154,10 +154,14 @@ __device__ __forceinline__ void sell_remainder(int r, int row, int n, int w, con
 }
 
 // SPLIT waves of a workgroup share one slice, each taking a contiguous part
-// of the k columns (chunks of MAXCK, then one remainder chunk); the slice's
-// entries reach the second wave from L1.  MAXCK and SPLIT are picked per
-// launch: small k does not pay the registers of wide chunks, and splitting
-// the columns gives more wavefronts to hide the gather latency.
+// of the group's columns (chunks of MAXCK, then one remainder chunk); the
+// slice's entries reach the second wave from L1.  MAXCK and SPLIT are picked
+// per launch: small k does not pay the registers of wide chunks, and
+// splitting the columns gives more wavefronts to hide the gather latency.
+// Wide blocks are cut into groups of gcols columns, blockIdx.y the group:
+// the dispatcher runs the groups as successive sweeps over the rows, so the
+// rows of X that the stencil re-reads stay cached for gcols columns at a
+// time instead of all k.
 template <typename VT, typename XT, int MAXCK, int SPLIT, int DOT>
 __global__ __launch_bounds__(kBlock) void k_sell_spmm(int n, int nslices,
                                                      const int *__restrict__ sptr,
@@ -166,6 +170,7 @@ __global__ __launch_bounds__(kBlock) void k_sell_spmm(int n, int nslices,
                                                      const VT *__restrict__ sval,
                                                      const XT *__restrict__ X, long ldx,
                                                      XT *__restrict__ Y, long ldy, int k,
+                                                     int gcols,
                                                      const int *__restrict__ xmap,
                                                      const int *__restrict__ ymap,
                                                      double *__restrict__ dpart)
@@ -176,8 +181,9 @@ __global__ __launch_bounds__(kBlock) void k_sell_spmm(int n, int nslices,
     const int slice = blk * kSlicesPerBlock + wid / SPLIT;
     if (slice >= nslices) return;
     const int part = wid % SPLIT;
-    const int per = (k + SPLIT - 1) / SPLIT;
-    const int cb = part * per, ce = min(k, cb + per);
+    const int gb = blockIdx.y * gcols, ge = min(k, gb + gcols);
+    const int per = (ge - gb + SPLIT - 1) / SPLIT;
+    const int cb = gb + part * per, ce = min(ge, cb + per);
     if (cb >= ce) return;
     const int lane = threadIdx.x % kWave;
     const int row = slice * kWave + lane;
@@ -196,15 +202,21 @@ __global__ __launch_bounds__(kBlock) void k_sell_spmm(int n, int nslices,
 
 template <typename VT, typename XT, int MAXCK, int SPLIT, int DOT>
 void launch_sell_spmm_v(const Mat *A, const VT *vals, const XT *X, long ldx, XT *Y, long ldy, int k,
-                        const int *d_xmap, const int *d_ymap, double *dpart, hipEvent_t ev0,
-                        hipEvent_t ev1)
+                        int gcols, const int *d_xmap, const int *d_ymap, double *dpart,
+                        hipEvent_t ev0, hipEvent_t ev1)
 {
     constexpr int spb = kBlock / kWave / SPLIT;
-    const int nb = (A->nslices + spb - 1) / spb;
+    const int ng = (k + gcols - 1) / gcols;
+    int nb = (A->nslices + spb - 1) / spb;
+    // workgroup (x, y) goes to XCD (y * nb + x) % 8: with nb a multiple of 8
+    // that is x % 8 in every group, as the remap assumes (extra workgroups
+    // find no slice and return)
+    if (ng > 1) nb = (nb + kXcds - 1) / kXcds * kXcds;
     // hipExtLaunchKernelGGL records the (optional) events in the dispatch packet
-    hipExtLaunchKernelGGL((k_sell_spmm<VT, XT, MAXCK, SPLIT, DOT>), dim3(nb), dim3(kBlock), 0,
+    hipExtLaunchKernelGGL((k_sell_spmm<VT, XT, MAXCK, SPLIT, DOT>), dim3(nb, ng), dim3(kBlock), 0,
                           state().stream, ev0, ev1, 0, A->nrows, A->nslices, A->d_slice_ptr,
-                          A->d_slice_w, A->d_col, vals, X, ldx, Y, ldy, k, d_xmap, d_ymap, dpart);
+                          A->d_slice_w, A->d_col, vals, X, ldx, Y, ldy, k, gcols, d_xmap, d_ymap,
+                          dpart);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -219,13 +231,20 @@ void launch_sell_spmm(const Mat *A, const VT *vals, const XT *X, long ldx, XT *Y
         const char *e = getenv("GCGE_HIP_SPMM_SPLIT");
         return e ? atoi(e) : 0;
     }();
+    static const int forced_group = [] {
+        const char *e = getenv("GCGE_HIP_SPMM_GROUP");
+        return e ? atoi(e) : 0;
+    }();
+    const int gcols = forced_group > 0 ? std::min(forced_group, k) : k;
+    const int kg = gcols;
     int split = forced;
     // measured on MI355X (Q1 64^3, k = 1..50): one wave per slice is best up to
     // 10 columns, two up to 20, four beyond
-    if (split != 1 && split != 2 && split != 4) split = k <= 10 ? 1 : (k <= 20 ? 2 : 4);
-    const int per = (k + split - 1) / split;
+    if (split != 1 && split != 2 && split != 4) split = kg <= 10 ? 1 : (kg <= 20 ? 2 : 4);
+    const int per = (kg + split - 1) / split;
 #define GCGE_SPMM_LAUNCH(MC, SP) \
-    launch_sell_spmm_v<VT, XT, MC, SP, DOT>(A, vals, X, ldx, Y, ldy, k, d_xmap, d_ymap, dpart, ev0, ev1)
+    launch_sell_spmm_v<VT, XT, MC, SP, DOT>(A, vals, X, ldx, Y, ldy, k, gcols, d_xmap, d_ymap, dpart, \
+                                            ev0, ev1)
     if (split == 1) {
         if (per <= 4) GCGE_SPMM_LAUNCH(4, 1); else if (per <= 8) GCGE_SPMM_LAUNCH(8, 1); else GCGE_SPMM_LAUNCH(16, 1);
     } else if (split == 2) {
@@ -234,6 +253,196 @@ void launch_sell_spmm(const Mat *A, const VT *vals, const XT *X, long ldx, XT *Y
         if (per <= 4) GCGE_SPMM_LAUNCH(4, 4); else if (per <= 8) GCGE_SPMM_LAUNCH(8, 4); else GCGE_SPMM_LAUNCH(16, 4);
     }
 #undef GCGE_SPMM_LAUNCH
+}
+
+// ---- SpMM on diagonal storage ----------------------------------------------------------
+// A wavefront owns a block of 62 consecutive rows: lane L (1..62) computes
+// row r0 + L - 1, lanes 0 and 63 hold the rows just outside the block.  For a
+// run of offsets {c-1, c, c+1} every lane loads x[r0 + L - 1 + c] (one
+// coalesced access per column); lane L takes x[row + c - 1] and
+// x[row + c + 1] from lanes L - 1 and L + 1 through DPP wavefront shifts.  A
+// 27-point stencil thus issues 9 loads per column and row block instead of 27
+// gathers, and reads no column indices.  The row's values stay in registers
+// for all columns of the wave; SPLIT waves of a workgroup share a row block
+// (each takes part of the columns, the values reach the later waves from L1).
+// Products are added in increasing column order, as the CSR rows list them.
+// Indices outside 0..n-1 are clamped: the matrix has no entry there (zero
+// value), so the clamped load only keeps the access in bounds.
+constexpr int kDiaRows = kWave - 2;
+
+__device__ __forceinline__ int dpp_prev(int src)  // lane i <- lane i - 1
+{
+    return __builtin_amdgcn_update_dpp(0, src, 0x138, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int dpp_next(int src)  // lane i <- lane i + 1
+{
+    return __builtin_amdgcn_update_dpp(0, src, 0x130, 0xf, 0xf, false);
+}
+template <bool NEXT>
+__device__ __forceinline__ double lane_shift(double src)
+{
+    const long long s = __double_as_longlong(src);
+    const int lo = (int)(s & 0xffffffffLL), hi = (int)(s >> 32);
+    const int rl = NEXT ? dpp_next(lo) : dpp_prev(lo);
+    const int rh = NEXT ? dpp_next(hi) : dpp_prev(hi);
+    return __longlong_as_double(((long long)rh << 32) | (unsigned int)rl);
+}
+template <bool NEXT>
+__device__ __forceinline__ float lane_shift(float src)
+{
+    const int i = __float_as_int(src);
+    return __int_as_float(NEXT ? dpp_next(i) : dpp_prev(i));
+}
+
+// columns [c0, c0 + cn) of the wave's row block, cn <= CK
+template <int CK, int DOT, typename XT>
+__device__ __forceinline__ void dia_chunk(const XT (&v)[kDiaMaxGroups][3], const DiaDesc &dd,
+                                          int n, int row, bool own, int lane, int rb, int nblocks,
+                                          const XT *__restrict__ X, long ldx, XT *__restrict__ Y,
+                                          long ldy, int k, int c0, int cn,
+                                          const int *__restrict__ xmap,
+                                          const int *__restrict__ ymap, double *__restrict__ dpart)
+{
+    const XT *xc[CK];
+#pragma unroll
+    for (int cc = 0; cc < CK; ++cc) {
+        const int c = c0 + min(cc, cn - 1);
+        xc[cc] = X + (long)(xmap ? xmap[c] : c) * ldx;
+    }
+    // all loads of the chunk first (9 * CK in flight), then the products
+    XT xv[kDiaMaxGroups][CK];
+#pragma unroll
+    for (int g = 0; g < kDiaMaxGroups; ++g) {
+        const int ic = min(max(row + (g < dd.ngroups ? dd.center[g] : 0), 0), n - 1);
+#pragma unroll
+        for (int cc = 0; cc < CK; ++cc) xv[g][cc] = (g < dd.ngroups && cc < cn) ? xc[cc][ic] : XT(0);
+    }
+    XT acc[CK];
+#pragma unroll
+    for (int cc = 0; cc < CK; ++cc) acc[cc] = XT(0);
+#pragma unroll
+    for (int g = 0; g < kDiaMaxGroups; ++g) {
+        if (g < dd.ngroups) {
+            const bool lo = dd.slot[g][0] >= 0, mid = dd.slot[g][1] >= 0, hi = dd.slot[g][2] >= 0;
+#pragma unroll
+            for (int cc = 0; cc < CK; ++cc) {
+                if (lo) acc[cc] += v[g][0] * lane_shift<false>(xv[g][cc]);
+                if (mid) acc[cc] += v[g][1] * xv[g][cc];
+                if (hi) acc[cc] += v[g][2] * lane_shift<true>(xv[g][cc]);
+            }
+        }
+    }
+    if (own) {
+#pragma unroll
+        for (int cc = 0; cc < CK; ++cc)
+            if (cc < cn) Y[(long)(ymap ? ymap[c0 + cc] : c0 + cc) * ldy + row] = acc[cc];
+    }
+    if constexpr (DOT > 0) {
+#pragma unroll
+        for (int cc = 0; cc < CK; ++cc) {
+            if (cc < cn) {
+                const double pv = own ? (double)xc[cc][row] : 0.0;
+                const double pw = wave_sum(pv * (own ? (double)acc[cc] : 0.0));
+                double pp = 0.0;
+                if constexpr (DOT > 1) pp = wave_sum(pv * pv);
+                if (lane == 0) {
+                    dpart[(long)(c0 + cc) * nblocks + rb] = pw;
+                    if constexpr (DOT > 1) dpart[(long)(k + c0 + cc) * nblocks + rb] = pp;
+                }
+            }
+        }
+    }
+}
+
+template <typename VT, typename XT, int CK, int SPLIT, int DOT>
+__global__ __launch_bounds__(kBlock) void k_dia_spmm(int n, int nblocks, DiaDesc dd,
+                                                    const VT *__restrict__ dval, long ldd,
+                                                    const XT *__restrict__ X, long ldx,
+                                                    XT *__restrict__ Y, long ldy, int k, int gcols,
+                                                    const int *__restrict__ xmap,
+                                                    const int *__restrict__ ymap,
+                                                    double *__restrict__ dpart)
+{
+    constexpr int kBlocksPerGroup = kBlock / kWave / SPLIT;
+    const int blk = xcd_remap(blockIdx.x, gridDim.x);
+    const int wid = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int rb = blk * kBlocksPerGroup + wid / SPLIT;
+    if (rb >= nblocks) return;
+    const int part = wid % SPLIT;
+    const int gb = blockIdx.y * gcols, ge = min(k, gb + gcols);
+    const int per = (ge - gb + SPLIT - 1) / SPLIT;
+    const int cb = gb + part * per, ce = min(ge, cb + per);
+    if (cb >= ce) return;
+    const int row = rb * kDiaRows + lane - 1;
+    const bool own = lane >= 1 && lane <= kDiaRows && row < n;
+    XT v[kDiaMaxGroups][3];
+#pragma unroll
+    for (int g = 0; g < kDiaMaxGroups; ++g)
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+            const int s = g < dd.ngroups ? dd.slot[g][p] : -1;
+            v[g][p] = (s >= 0 && own) ? (XT)dval[(long)s * ldd + row] : XT(0);
+        }
+    for (int c0 = cb; c0 < ce; c0 += CK)
+        dia_chunk<CK, DOT>(v, dd, n, row, own, lane, rb, nblocks, X, ldx, Y, ldy, k, c0,
+                           min(CK, ce - c0), xmap, ymap, dpart);
+}
+
+inline int dia_blocks(const Mat *A) { return (A->nrows + kDiaRows - 1) / kDiaRows; }
+
+// Column groups (blockIdx.y) of gcols columns, SPLIT waves per row block, CK
+// columns per register chunk.  GCGE_HIP_DIA_GROUP / GCGE_HIP_DIA_SPLIT
+// override the group width and the split (for measurement).
+template <int DOT, typename VT, typename XT>
+void launch_dia_spmm(const Mat *A, const VT *vals, const XT *X, long ldx, XT *Y, long ldy, int k,
+                     const int *d_xmap, const int *d_ymap, double *dpart, hipEvent_t ev0,
+                     hipEvent_t ev1)
+{
+    static const int forced_group = [] {
+        const char *e = getenv("GCGE_HIP_DIA_GROUP");
+        return e ? atoi(e) : 0;
+    }();
+    static const int forced_split = [] {
+        const char *e = getenv("GCGE_HIP_DIA_SPLIT");
+        return e ? atoi(e) : 0;
+    }();
+    static const int forced_ck = [] {
+        const char *e = getenv("GCGE_HIP_DIA_CK");
+        return e ? atoi(e) : 0;
+    }();
+    const int gcols = forced_group > 0 ? std::min(forced_group, k) : k;
+    // measured on MI355X (Q1 64^3 and 128^3, k = 1..100): one wave per row
+    // block and chunks of 2 float64 columns (18 loads in flight at four
+    // waves per SIMD) beat wider chunks, column splits and column groups
+    int split = forced_split;
+    if (split != 1 && split != 2 && split != 4) split = 1;
+    const int per = (gcols + split - 1) / split;
+    const int ng = (k + gcols - 1) / gcols;
+    const int nblocks = dia_blocks(A);
+    const int bpg = kBlock / kWave / split;
+    int nb = (nblocks + bpg - 1) / bpg;
+    if (ng > 1) nb = (nb + kXcds - 1) / kXcds * kXcds;
+    const long ldd = (long)A->nslices * kWave;
+#define GCGE_DIA_LAUNCH(CK, SP)                                                                      \
+    hipExtLaunchKernelGGL((k_dia_spmm<VT, XT, CK, SP, DOT>), dim3(nb, ng), dim3(kBlock), 0,          \
+                          state().stream, ev0, ev1, 0, A->nrows, nblocks, A->dia, vals, ldd, X, ldx,  \
+                          Y, ldy, k, gcols, d_xmap, d_ymap, dpart)
+    const int ck_auto = sizeof(XT) == 8 ? 2 : (per <= 2 ? 2 : 4);
+    const int ckw = forced_ck == 2 || forced_ck == 4 || forced_ck == 8 ? forced_ck : ck_auto;
+#define GCGE_DIA_CK(SP)                                     \
+    if (ckw == 2) GCGE_DIA_LAUNCH(2, SP);                   \
+    else if (ckw == 4) GCGE_DIA_LAUNCH(4, SP);              \
+    else GCGE_DIA_LAUNCH(8, SP);
+    if (split == 1) {
+        GCGE_DIA_CK(1)
+    } else if (split == 2) {
+        GCGE_DIA_CK(2)
+    } else {
+        GCGE_DIA_CK(4)
+    }
+#undef GCGE_DIA_CK
+#undef GCGE_DIA_LAUNCH
+    HIP_CHECK(hipGetLastError());
 }
 
 __global__ __launch_bounds__(kBlock) void k_to_float(size_t len, const double *__restrict__ src,
@@ -752,8 +961,12 @@ void spmm(const Mat *A, const double *X, long ldx, double *Y, long ldy, int k, c
     const double bytes = (double)A->nnz * (8 + 4) + 4.0 * (A->nrows + 1) +
                          8.0 * (double)k * (A->ncols + A->nrows);
     ProfScope prof("spmm", bytes, true);
-    launch_sell_spmm<0>(A, A->d_val, X, ldx, Y, ldy, k, d_xmap, d_ymap, nullptr, prof.start_event(),
-                        prof.stop_event());
+    if (A->use_dia)
+        launch_dia_spmm<0>(A, A->d_dia, X, ldx, Y, ldy, k, d_xmap, d_ymap, nullptr, prof.start_event(),
+                           prof.stop_event());
+    else
+        launch_sell_spmm<0>(A, A->d_val, X, ldx, Y, ldy, k, d_xmap, d_ymap, nullptr, prof.start_event(),
+                            prof.stop_event());
 }
 
 void spmm(const Mat *A, const float *X, long ldx, float *Y, long ldy, int k, const int *d_xmap,
@@ -764,8 +977,12 @@ void spmm(const Mat *A, const float *X, long ldx, float *Y, long ldy, int k, con
     const double bytes = (double)A->nnz * (4 + 4) + 4.0 * (A->nrows + 1) +
                          4.0 * (double)k * (A->ncols + A->nrows);
     ProfScope prof("spmm32", bytes, true);
-    launch_sell_spmm<0>(A, A->d_val32, X, ldx, Y, ldy, k, d_xmap, d_ymap, nullptr, prof.start_event(),
-                        prof.stop_event());
+    if (A->use_dia && A->d_dia32)
+        launch_dia_spmm<0>(A, A->d_dia32, X, ldx, Y, ldy, k, d_xmap, d_ymap, nullptr,
+                           prof.start_event(), prof.stop_event());
+    else
+        launch_sell_spmm<0>(A, A->d_val32, X, ldx, Y, ldy, k, d_xmap, d_ymap, nullptr,
+                            prof.start_event(), prof.stop_event());
 }
 
 void spmm_dot(const Mat *A, const double *P, long ldp, double *W, long ldw, int k,
@@ -774,13 +991,18 @@ void spmm_dot(const Mat *A, const double *P, long ldp, double *W, long ldw, int 
     if (k <= 0 || A->nrows <= 0) return;
     const double bytes = (double)A->nnz * (8 + 4) + 4.0 * (A->nrows + 1) +
                          8.0 * (double)k * (A->ncols + 2.0 * A->nrows);
-    double *dpart = partial_buffer((size_t)A->nslices * k);
+    const int nparts = A->use_dia ? dia_blocks(A) : A->nslices;
+    double *dpart = partial_buffer((size_t)nparts * k);
     {
         ProfScope prof("spmm", bytes, true);
-        launch_sell_spmm<1>(A, A->d_val, P, ldp, W, ldw, k, d_pmap, d_wmap, dpart, prof.start_event(),
-                            prof.stop_event());
+        if (A->use_dia)
+            launch_dia_spmm<1>(A, A->d_dia, P, ldp, W, ldw, k, d_pmap, d_wmap, dpart,
+                               prof.start_event(), prof.stop_event());
+        else
+            launch_sell_spmm<1>(A, A->d_val, P, ldp, W, ldw, k, d_pmap, d_wmap, dpart,
+                                prof.start_event(), prof.stop_event());
     }
-    hipLaunchKernelGGL(k_reduce_rows, dim3(k), dim3(kBlock), 0, state().stream, dpart, A->nslices, d_out);
+    hipLaunchKernelGGL(k_reduce_rows, dim3(k), dim3(kBlock), 0, state().stream, dpart, nparts, d_out);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -791,13 +1013,19 @@ void spmm_dot(const Mat *A, const float *P, long ldp, float *W, long ldw, int k,
     if (!A->d_val32) fail_msg("float32 SpMM on a matrix without float32 values");
     const double bytes = (double)A->nnz * (4 + 4) + 4.0 * (A->nrows + 1) +
                          4.0 * (double)k * (A->ncols + 2.0 * A->nrows);
-    double *dpart = partial_buffer((size_t)A->nslices * 2 * k);
+    const bool dia = A->use_dia && A->d_dia32;
+    const int nparts = dia ? dia_blocks(A) : A->nslices;
+    double *dpart = partial_buffer((size_t)nparts * 2 * k);
     {
         ProfScope prof("spmm32", bytes, true);
-        launch_sell_spmm<2>(A, A->d_val32, P, ldp, W, ldw, k, d_pmap, d_wmap, dpart, prof.start_event(),
-                            prof.stop_event());
+        if (dia)
+            launch_dia_spmm<2>(A, A->d_dia32, P, ldp, W, ldw, k, d_pmap, d_wmap, dpart,
+                               prof.start_event(), prof.stop_event());
+        else
+            launch_sell_spmm<2>(A, A->d_val32, P, ldp, W, ldw, k, d_pmap, d_wmap, dpart,
+                                prof.start_event(), prof.stop_event());
     }
-    hipLaunchKernelGGL(k_reduce_rows, dim3(2 * k), dim3(kBlock), 0, state().stream, dpart, A->nslices,
+    hipLaunchKernelGGL(k_reduce_rows, dim3(2 * k), dim3(kBlock), 0, state().stream, dpart, nparts,
                        d_out);
     HIP_CHECK(hipGetLastError());
 }

@@ -59,6 +59,20 @@ void GCGE_HIP_MatDestroy(void *mat);
 /* fp32 != 0: keep a float32 copy of the values and use it in the inner block
  * CG (mixed-precision form); fp32 == 0: float64 everywhere (default). */
 void GCGE_HIP_MatSetPrecision(void *mat, int fp32);
+/* Storage used by the products of this matrix (no counterpart in the
+ * reference, whose CSR products are per-vector loops).  At creation a square
+ * matrix whose entries lie on few diagonals (sorted offsets col - row
+ * covered by at most 9 runs of three consecutive offsets, rows with
+ * increasing columns, diagonals at least half full: stencil and FEM matrices
+ * in their natural ordering) also gets diagonal storage, and its products
+ * use it.  diagonal == 0 switches to the sliced-ELLPACK form, which every
+ * matrix has; diagonal != 0 switches back when diagonal storage exists.
+ * Returns 1 when the products now use diagonal storage.  Results of the two
+ * forms agree to rounding (same products, same order, fused multiply-adds). */
+int GCGE_HIP_MatSetStorage(void *mat, int diagonal);
+/* number of stored diagonals and of runs (0, 0 without diagonal storage);
+ * returns 1 when the products use diagonal storage */
+int GCGE_HIP_MatStorageInfo(void *mat, int *ndiag, int *ngroups);
 
 /* ---- multivecs (replace the default MultiVecCreateByMat of per-vector
  *      allocations, reference src/gcge_ops.c:72-80) ----------------------- */

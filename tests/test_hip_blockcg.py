@@ -19,10 +19,15 @@ from test_oracle import BCG_FIXTURES, bcg_matrices
 pytestmark = pytest.mark.gpu
 
 
-def device_block_cg(A, B, rhs, x0, max_it, rate, if_shift=0, shift=0.0, fp32=False):
+def device_block_cg(A, B, rhs, x0, max_it, rate, if_shift=0, shift=0.0, fp32=False,
+                    diagonal=True):
     lib = _native.hip()
     Ad = DeviceMatrix(A)
     Bd = DeviceMatrix(B) if B is not None else None
+    if not diagonal:
+        for d in (Ad, Bd):
+            if d is not None:
+                d.set_storage(False)
     Ad.set_precision(fp32)
     if Bd is not None:
         Bd.set_precision(fp32)
@@ -55,24 +60,27 @@ def test_device_block_cg_matches_reference_fixture(case, golden_dir):
     _close(r, g[case + "/r"], 1e-9)
 
 
+@pytest.mark.parametrize("diagonal", [True, False])
 @pytest.mark.parametrize("shifted", [False, True])
-def test_device_block_cg_matches_restatement(shifted):
+def test_device_block_cg_matches_restatement(shifted, diagonal):
     """Q1 12^3 stiffness (and mass for the shifted solve), 8 columns of mixed
-    scale so columns stop at different iterations."""
+    scale so columns stop at different iterations; products through the
+    diagonal storage of the matrices and through sliced ELLPACK."""
     K, M = problems.q1_fem_3d(12)
     rng = np.random.default_rng(31)
     b = rng.standard_normal((K.nrows, 8)) * np.logspace(0, -2, 8)
     x0 = 1e-3 * rng.standard_normal((K.nrows, 8))
     args = (20, 1e-2, 1, 40.0) if shifted else (20, 1e-2, 0, 0.0)
-    print(f"block CG restatement shifted={shifted}", flush=True)
-    x, r = device_block_cg(K, M, b, x0, *args)
+    print(f"block CG restatement shifted={shifted} diagonal={diagonal}", flush=True)
+    x, r = device_block_cg(K, M, b, x0, *args, diagonal=diagonal)
     xo, ro = oracle.block_cg(K, M, b, x0, *args)
     _close(x, xo, 1e-9)
     _close(r, ro, 1e-8)
 
 
+@pytest.mark.parametrize("diagonal", [True, False])
 @pytest.mark.parametrize("case", ["fd64", "q1e10_shift_B"])
-def test_device_block_cg_fp32_directions(case):
+def test_device_block_cg_fp32_directions(case, diagonal):
     """14 CG steps with float32 directions and products against the float64
     restatement: solution within float32 rounding, true residual as small."""
     if case == "fd64":
@@ -83,8 +91,8 @@ def test_device_block_cg_fp32_directions(case):
     rng = np.random.default_rng(32)
     b = rng.standard_normal((A.nrows, 6))
     x0 = np.zeros_like(b)
-    print(f"block CG float32 {case}", flush=True)
-    x32, _ = device_block_cg(A, B, b, x0, *args, fp32=True)
+    print(f"block CG float32 {case} diagonal={diagonal}", flush=True)
+    x32, _ = device_block_cg(A, B, b, x0, *args, fp32=True, diagonal=diagonal)
     x64, _ = oracle.block_cg(A, B, b, x0, *args)
     rel = np.linalg.norm(x32 - x64, axis=0) / np.linalg.norm(x64, axis=0)
     assert np.all(rel < 1e-4), rel

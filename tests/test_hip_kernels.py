@@ -34,8 +34,29 @@ def _random_csr(n, m, density, seed, long_rows=(), empty_rows=()):
     return problems.CSR.from_scipy(A.tocsr())
 
 
+def _banded_csr(n, offsets, keep, seed):
+    """random entries on the given diagonals, each kept with probability
+    `keep` (ragged rows, diagonals partly empty)"""
+    import scipy.sparse as sp
+
+    rng = _rng(seed)
+    rows, cols = [], []
+    for d in offsets:
+        r = np.arange(max(0, -d), min(n, n - d))
+        r = r[rng.uniform(size=len(r)) < keep]
+        rows.append(r)
+        cols.append(r + d)
+    r, c = np.concatenate(rows), np.concatenate(cols)
+    A = sp.csr_matrix((rng.standard_normal(len(r)), (r, c)), shape=(n, n))
+    A.sort_indices()
+    return problems.CSR.from_scipy(A)
+
+
 MATS = {
     "fd45": lambda: problems.fd_laplacian_2d(45),
+    "q1_lognormal_e8": lambda: problems.q1_fem_3d(8, problems.lognormal_kappa(8, 3))[0],
+    "band130": lambda: _banded_csr(130, (-70, -69, -3, -1, 0, 1, 2, 5, 63, 64, 66), 0.8, 4),
+    "band65": lambda: _banded_csr(65, (-64, -2, 0, 1, 64), 0.9, 5),
     "q1_e10": lambda: problems.q1_fem_3d(10)[0],
     "q1_mass_e9": lambda: problems.q1_fem_3d(9)[1],
     "ragged": lambda: _random_csr(1000, 1000, 0.01, 3, long_rows=(5, 700), empty_rows=(0, 64, 999)),
@@ -45,13 +66,17 @@ MATS = {
 }
 
 
+@pytest.mark.parametrize("storage", ["default", "sliced"])
 @pytest.mark.parametrize("name", sorted(MATS))
 @pytest.mark.parametrize("k", [1, 2, 3, 5, 8, 10, 13, 17, 33])
-def test_spmm_matches_oracle(name, k):
+def test_spmm_matches_oracle(name, k, storage):
+    """Both storages: diagonal (where the matrix has it) and sliced ELLPACK."""
     A = MATS[name]()
     rng = _rng(k)
     X = rng.standard_normal((A.ncols, k + 3))
     dA = DeviceMatrix(A)
+    if storage == "sliced":
+        assert not dA.set_storage(False)
     dX = DeviceMultiVec.from_host(X)
     dY = DeviceMultiVec(A.nrows, k + 2)
     _native.hip().GCGE_HIP_SpMM(dA.handle, dX.handle, 2, dY.handle, 1, k)
@@ -59,6 +84,60 @@ def test_spmm_matches_oracle(name, k):
     want = oracle.spmm(oracle.Matrix(A), X[:, 2:2 + k])
     bound = abs(A.to_scipy()) @ np.abs(X[:, 2:2 + k])
     assert np.all(np.abs(got - want) <= 64 * EPS * bound + 1e-300)
+
+
+DIAGONAL_LAYOUT = {
+    # matrix: (stored diagonals, runs of three offsets), (0, 0) = none
+    "fd45": (5, 3),            # -45 | -1 0 1 | 45
+    "q1_e10": (21, 9),         # kappa = 1: the +-1, +-N, +-N^2 couplings vanish
+    "q1_mass_e9": (27, 9),
+    "q1_lognormal_e8": (21, 9),  # those couplings are zero in every element
+    "band130": (11, 6),        # -70 -69 | -3 -1 | 0 1 2 | 5 | 63 64 | 66 (runs start
+                               # at the smallest offset not yet covered)
+    "band65": (5, 4),          # -64 | -2 | 0 1 | 64
+    "tiny": (3, 2),            # -2 0 | 2
+    "ragged": (0, 0),          # random pattern: too many diagonals
+}
+
+
+@pytest.mark.parametrize("name", sorted(DIAGONAL_LAYOUT))
+def test_diagonal_storage_layout(name):
+    info = DeviceMatrix(MATS[name]()).storage_info()
+    want = DIAGONAL_LAYOUT[name]
+    assert (info["ndiag"], info["runs"]) == want
+    assert info["diagonal"] == (want[0] > 0)
+
+
+def test_no_diagonal_storage_for_permuted_or_unsorted_rows():
+    K = problems.q1_fem_3d(6)[0]
+    perm = _rng(9).permutation(K.nrows)
+    Kp = problems.CSR.from_scipy(K.to_scipy()[perm][:, perm].tocsr())
+    assert not DeviceMatrix(Kp).storage_info()["diagonal"]
+    # same entries, one row's columns listed in decreasing order
+    rp, ci, v = K.rowptr.copy(), K.colidx.copy(), K.vals.copy()
+    ci[rp[7]:rp[8]] = ci[rp[7]:rp[8]][::-1].copy()
+    v[rp[7]:rp[8]] = v[rp[7]:rp[8]][::-1].copy()
+    assert not DeviceMatrix(problems.CSR(K.nrows, K.ncols, rp, ci, v)).storage_info()["diagonal"]
+
+
+@pytest.mark.parametrize("name", ["q1_e10", "q1_lognormal_e8", "band130"])
+def test_spmm_storages_agree(name):
+    """Same products in the same order: the two storages agree to a few ulps
+    of the absolute-value product (different fused multiply-add grouping
+    only where the compiler contracts differently)."""
+    A = MATS[name]()
+    k = 11
+    X = _rng(7).standard_normal((A.ncols, k))
+    dA = DeviceMatrix(A)
+    dX = DeviceMultiVec.from_host(X)
+    dY = DeviceMultiVec(A.nrows, k)
+    _native.hip().GCGE_HIP_SpMM(dA.handle, dX.handle, 0, dY.handle, 0, k)
+    y_dia = dY.get()
+    dA.set_storage(False)
+    _native.hip().GCGE_HIP_SpMM(dA.handle, dX.handle, 0, dY.handle, 0, k)
+    y_sell = dY.get()
+    bound = abs(A.to_scipy()) @ np.abs(X)
+    assert np.all(np.abs(y_dia - y_sell) <= 4 * EPS * bound + 1e-300)
 
 
 @pytest.mark.parametrize("n,m,k", [(1, 1, 1), (63, 16, 16), (64, 17, 1), (65, 1, 17),
