@@ -151,6 +151,13 @@ void upload_async(void *dst, const void *src, size_t bytes)
     HIP_CHECK(hipEventRecord(s.ring_event[slot], s.stream));
 }
 
+// environment switch: set and not "0"
+static bool getenv_flag(const char *name)
+{
+    const char *e = getenv(name);
+    return e && *e && strcmp(e, "0") != 0;
+}
+
 // ---- profiling ------------------------------------------------------------------------------
 struct ProfRecord {
     const char *name;
@@ -182,6 +189,11 @@ ProfScope::ProfScope(const char *name, double bytes, bool pkt) : slot(-1), packe
     if (!packet) HIP_CHECK(hipEventRecord(r.start, s.stream));
     slot = (int)g_prof.size();
     g_prof.push_back(r);
+}
+
+void prof_set_bytes(int slot, double bytes)
+{
+    if (slot >= 0 && slot < (int)g_prof.size()) g_prof[slot].bytes = bytes;
 }
 
 hipEvent_t ProfScope::start_event() const { return slot < 0 ? nullptr : g_prof[slot].start; }
@@ -788,6 +800,92 @@ static void bcg_iterate(Mat *A, Mat *B, int if_shift, double shift, int n, int l
     }
 }
 
+// The same iteration with the bookkeeping on the device (k_bcg_control):
+// every step is enqueued without waiting for the previous one; a step whose
+// list is empty does nothing.  The host scalars (rho1 of earlier calls
+// included: the reference reads positions this call never writes) go up
+// once and come back once, so dtmp/itmp end as after the host loop.
+// d_rr0 holds the initial r.r of the len columns.
+template <typename PT>
+static void bcg_iterate_dev(Mat *A, int n, int len, double *Rd, long ldr, int r0, double *Xd,
+                            long ldx, int x0, PT *Pd, long ldp, int p0, PT *Wd, long ldw, int w0,
+                            int max_it, double rate, double *rho1, double *rho2, double *err,
+                            double *last_err, int *unlock, int *num_unlock, double *d_st,
+                            int *d_it, int by_column)
+{
+    constexpr bool scaled = std::is_same<PT, float>::value;
+    State &s = state();
+    BcgControl c{};
+    c.rho1 = d_st;
+    c.rho2 = d_st + len;
+    c.err = d_st + 2 * len;
+    c.last_err = d_st + 3 * len;
+    c.s_col = d_st + 4 * len;
+    c.pn_col = d_st + 5 * len;
+    c.beta = d_st + 6 * len;
+    c.alpha_num = d_st + 7 * len;
+    c.s_old = d_st + 8 * len;
+    c.s_new = d_st + 9 * len;
+    double *d_ptw = d_st + 10 * len;  // 2 len
+    double *d_rr = d_st + 12 * len;   // initial r.r on entry
+    c.ptw = d_ptw;
+    c.rr = d_rr;
+    c.unlock = d_it;
+    c.count = d_it + len;
+    c.maps = d_it + len + 1;
+    c.hist = d_it + 5 * len + 1;
+    c.len = len;
+    c.p0 = p0;
+    c.r0 = r0;
+    c.x0 = x0;
+    c.w0 = w0;
+    c.by_column = by_column;
+    c.scaled = scaled ? 1 : 0;
+    c.rate = rate;
+    const int *pm = c.maps, *rm = c.maps + len, *xm = c.maps + 2 * len, *wm = c.maps + 3 * len;
+    upload_async(c.rho1, rho1, (size_t)len * sizeof(double));
+    const int steps = std::max(0, max_it - 1);
+    bcg_control(c, 1, 0, steps > 0 ? 1 : 0);
+    // fixed and per-column algorithmic bytes of each product, for the profile
+    const double fixed = (double)A->nnz * (scaled ? 8 : 12) + 4.0 * (A->nrows + 1);
+    const double per_col = (scaled ? 4.0 : 8.0) * (A->ncols + 2.0 * A->nrows);
+    std::vector<int> slots(steps, -1);
+    for (int it = 1; it <= steps; ++it) {
+        if constexpr (scaled)
+            bcg_p_update(n, Pd, ldp, pm, Rd, ldr, rm, c.beta, c.s_old, c.s_new, it == 1, len, c.count);
+        else
+            bcg_p_update(n, Pd, ldp, pm, Rd, ldr, rm, c.beta, it == 1, len, c.count);
+        spmm_dot(A, Pd, ldp, Wd, ldw, len, pm, wm, d_ptw, c.count, &slots[it - 1]);
+        if constexpr (scaled)
+            bcg_xr_update(n, Xd, ldx, xm, Rd, ldr, rm, Pd, ldp, pm, Wd, ldw, wm, c.alpha_num, d_ptw,
+                          c.s_new, len, d_rr, c.count);
+        else
+            bcg_xr_update(n, Xd, ldx, xm, Rd, ldr, rm, Pd, ldp, pm, Wd, ldw, wm, c.alpha_num, d_ptw,
+                          len, d_rr, c.count);
+        bcg_control(c, 0, it, it + 1 <= steps ? it + 1 : 0);
+    }
+    // state back: rho1, rho2, err, last_err; unlock, count, step sizes
+    double *h = result_buffer((size_t)4 * len + (len + 1 + steps + 2) / 2 + 1);
+    int *hi = (int *)(h + 4 * len);
+    HIP_CHECK(hipMemcpyAsync(h, d_st, (size_t)4 * len * sizeof(double), hipMemcpyDeviceToHost, s.stream));
+    HIP_CHECK(hipMemcpyAsync(hi, d_it, (size_t)(len + 1) * sizeof(int), hipMemcpyDeviceToHost, s.stream));
+    if (steps > 0)
+        HIP_CHECK(hipMemcpyAsync(hi + len + 1, c.hist, (size_t)steps * sizeof(int), hipMemcpyDeviceToHost,
+                                 s.stream));
+    HIP_CHECK(hipStreamSynchronize(s.stream));
+    memcpy(rho1, h, (size_t)len * sizeof(double));
+    memcpy(rho2, h + len, (size_t)len * sizeof(double));
+    memcpy(err, h + 2 * len, (size_t)len * sizeof(double));
+    memcpy(last_err, h + 3 * len, (size_t)len * sizeof(double));
+    *num_unlock = hi[len];
+    memcpy(unlock, hi, (size_t)*num_unlock * sizeof(int));
+    for (int it = 0; it < steps; ++it)
+        if (slots[it] >= 0) {
+            const int cols = hi[len + 1 + it];
+            prof_set_bytes(slots[it], cols > 0 ? fixed + per_col * cols : 0.0);
+        }
+}
+
 static void block_cg(void *Amat, int if_shift, double shift, void *Bmat, void **RHS, void **V,
                      int *start, int *end, int max_it, double rate, void **CG_P, void **CG_W,
                      double *dtmp, int *itmp, int by_column = 0)
@@ -859,6 +957,29 @@ static void block_cg(void *Amat, int if_shift, double shift, void *Bmat, void **
         }
     }
     axpby(n, -1.0, Wd + (long)w0 * ldw, ldw, 1.0, Rd + (long)start[0] * ldr, ldr, len);
+    // bookkeeping on the device for the unshifted solve (the GCG inner
+    // solve); the shifted form keeps the host loop
+    const bool dev = if_shift != 1 && len <= kBcgControlMax && !getenv_flag("GCGE_HIP_BCG_HOST");
+    if (dev) {
+        double *d_st = grow_device(s.d_bcg, s.bcg_cap, (size_t)13 * len);
+        int *d_it = grow_device(s.d_bcgi, s.bcgi_cap, (size_t)5 * len + 1 + std::max(max_it, 1));
+        coldots(n, Rd + (long)start[0] * ldr, ldr, nullptr, Rd + (long)start[0] * ldr, ldr, nullptr, len,
+                d_st + 12 * len);
+        int num_unlock = 0;
+        if (f32) {
+            long lpf, lwf;
+            float *Pf = scratch_f32(0, n, len, &lpf);
+            float *Wf = scratch_f32(1, n, len, &lwf);
+            bcg_iterate_dev<float>(A, n, len, Rd, ldr, start[0], Xd, ldx, start[1], Pf, lpf, 0, Wf,
+                                   lwf, 0, max_it, rate, rho1, rho2, err, last_err, unlock,
+                                   &num_unlock, d_st, d_it, by_column);
+        } else {
+            bcg_iterate_dev<double>(A, n, len, Rd, ldr, start[0], Xd, ldx, start[1], Pd, ldp, p0, Wd,
+                                    ldw, w0, max_it, rate, rho1, rho2, err, last_err, unlock,
+                                    &num_unlock, d_st, d_it, by_column);
+        }
+        return;
+    }
     double *d_scal = small_buffer2((size_t)4 * len + 8);
     coldots(n, Rd + (long)start[0] * ldr, ldr, nullptr, Rd + (long)start[0] * ldr, ldr, nullptr, len, d_scal);
     double *h = result_buffer((size_t)len);
@@ -1164,6 +1285,8 @@ void GCGE_HIP_Finalize(void)
     if (s.d_small) HIP_CHECK(hipFree(s.d_small));
     if (s.d_small2) HIP_CHECK(hipFree(s.d_small2));
     if (s.d_index) HIP_CHECK(hipFree(s.d_index));
+    if (s.d_bcg) HIP_CHECK(hipFree(s.d_bcg));
+    if (s.d_bcgi) HIP_CHECK(hipFree(s.d_bcgi));
     if (s.h_result) HIP_CHECK(hipHostFree(s.h_result));
     for (int i = 0; i < State::kRing; ++i) {
         if (s.h_ring[i]) HIP_CHECK(hipHostFree(s.h_ring[i]));

@@ -30,6 +30,10 @@ struct State {
     size_t small2_cap = 0;
     int *d_index = nullptr;       // column index lists
     size_t index_cap = 0;         // ints
+    double *d_bcg = nullptr;      // block-CG bookkeeping state
+    size_t bcg_cap = 0;           // doubles
+    int *d_bcgi = nullptr;
+    size_t bcgi_cap = 0;          // ints
     // pinned host staging
     double *h_result = nullptr;
     size_t result_cap = 0;
@@ -62,6 +66,8 @@ void upload_async(void *dst, const void *src, size_t bytes);
 // hipExtLaunchKernelGGL, which records them in the dispatch packet itself
 // (no separate event packets around the kernel, so the interval is the
 // kernel's own execution time).
+// set the algorithmic bytes of a recorded launch (slot from ProfScope)
+void prof_set_bytes(int slot, double bytes);
 struct ProfScope {
     int slot;
     bool packet;
@@ -139,10 +145,15 @@ void to_float(const double *src, float *dst, size_t len);
 // block-CG product: W(:, wmap[c]) = A P(:, pmap[c]) and, from the same pass,
 // d_out[c] = P(:, pmap[c]) . W(:, wmap[c]); the float32 form also gives
 // d_out[k + c] = P(:, pmap[c]) . P(:, pmap[c]) (sums in float64)
+// d_count (optional): the launch covers columns c < min(k, *d_count), the
+// list length kept on the device (block CG); prof_slot (optional) receives
+// the profile record of the product for a later byte correction
 void spmm_dot(const Mat *A, const double *P, long ldp, double *W, long ldw, int k,
-              const int *d_pmap, const int *d_wmap, double *d_out);
+              const int *d_pmap, const int *d_wmap, double *d_out,
+              const int *d_count = nullptr, int *prof_slot = nullptr);
 void spmm_dot(const Mat *A, const float *P, long ldp, float *W, long ldw, int k, const int *d_pmap,
-              const int *d_wmap, double *d_out);
+              const int *d_wmap, double *d_out, const int *d_count = nullptr,
+              int *prof_slot = nullptr);
 // host G(m x k, ldg) = V^T W
 void gram(int n, const double *V, long ldv, int m, const double *W, long ldw, int k, double *G,
           int ldg, bool sync_result = true);
@@ -168,21 +179,45 @@ void residual_sq(int n, const double *AX, long ldax, const double *BX, long ldbx
 // directions are stored scaled, p = s p_stored with per-column s (d_sold,
 // d_snew, d_scale: one entry per listed column).
 void bcg_p_update(int n, double *P, long ldp, const int *d_pmap, const double *R, long ldr,
-                  const int *d_rmap, const double *d_beta, int first, int k);
+                  const int *d_rmap, const double *d_beta, int first, int k,
+                  const int *d_count = nullptr);
 void bcg_p_update(int n, float *P, long ldp, const int *d_pmap, const double *R, long ldr,
                   const int *d_rmap, const double *d_beta, const double *d_sold,
-                  const double *d_snew, int first, int k);
+                  const double *d_snew, int first, int k, const int *d_count = nullptr);
 // d_out[c] = p_c . w_c, d_out[k + c] = p_c . p_c (float32 blocks, float64 sums)
 void coldots_pw(int n, const float *P, long ldp, const int *d_pmap, const float *W, long ldw,
                 const int *d_wmap, int k, double *d_out);
 void bcg_xr_update(int n, double *X, long ldx, const int *d_xmap, double *R, long ldr,
                    const int *d_rmap, const double *P, long ldp, const int *d_pmap,
                    const double *W, long ldw, const int *d_wmap, const double *d_rho2,
-                   const double *d_ptw, int k, double *d_rr_out);
+                   const double *d_ptw, int k, double *d_rr_out, const int *d_count = nullptr);
 void bcg_xr_update(int n, double *X, long ldx, const int *d_xmap, double *R, long ldr,
                    const int *d_rmap, const float *P, long ldp, const int *d_pmap, const float *W,
                    long ldw, const int *d_wmap, const double *d_rho2, const double *d_ptw,
-                   const double *d_scale, int k, double *d_rr_out);
+                   const double *d_scale, int k, double *d_rr_out, const int *d_count = nullptr);
+// Block-CG bookkeeping on the device (one workgroup): the per-column scalars
+// and the list of unconverged columns of the reference loop
+// (src/gcge_cg.c:395-490), kept in device memory between steps so that the
+// steps need no host round trip.  rho1/rho2 are indexed as the reference
+// indexes them (by position in the list, or by column with by_column).
+struct BcgControl {
+    double *rho1, *rho2, *err, *last_err, *s_col, *pn_col;  // len each
+    double *beta, *alpha_num, *s_old, *s_new;               // by position, read by the steps
+    const double *ptw;   // p.w of the step [len], p.p [len + id] (scaled directions)
+    const double *rr;    // r.r of the step, or the initial r.r (init)
+    int *unlock;         // unconverged columns [len]
+    int *count;          // number of entries of unlock
+    int *maps;           // p, r, x, w column lists [4 len]
+    int *hist;           // columns of each step (for the profile byte counts)
+    int len, p0, r0, x0, w0, by_column, scaled;
+    double rate;
+};
+constexpr int kBcgControlMax = 1024;  // columns handled by the device bookkeeping
+// init: rho2 = rr, err = last_err = sqrt(rr), unlock = columns with err >= 1e-15;
+// finished_step > 0: take the r.r of that step and drop converged columns;
+// prepare_step > 0: lists and scalars of that step
+void bcg_control(const BcgControl &c, int init, int finished_step, int prepare_step);
+
 void axpy_cols(int n, const double *d_alpha, const double *X, long ldx, const int *d_xmap,
                double *Y, long ldy, const int *d_ymap, int k);
 void axpy_cols(int n, const double *d_alpha, const float *X, long ldx, const int *d_xmap,

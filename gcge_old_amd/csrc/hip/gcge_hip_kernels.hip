@@ -173,7 +173,8 @@ __global__ __launch_bounds__(kBlock) void k_sell_spmm(int n, int nslices,
                                                      int gcols,
                                                      const int *__restrict__ xmap,
                                                      const int *__restrict__ ymap,
-                                                     double *__restrict__ dpart)
+                                                     double *__restrict__ dpart,
+                                                     const int *__restrict__ count)
 {
     constexpr int kSlicesPerBlock = kBlock / kWave / SPLIT;
     const int blk = xcd_remap(blockIdx.x, gridDim.x);
@@ -181,7 +182,9 @@ __global__ __launch_bounds__(kBlock) void k_sell_spmm(int n, int nslices,
     const int slice = blk * kSlicesPerBlock + wid / SPLIT;
     if (slice >= nslices) return;
     const int part = wid % SPLIT;
-    const int gb = blockIdx.y * gcols, ge = min(k, gb + gcols);
+    const int kc = count ? min(k, *count) : k;  // columns listed on the device
+    const int gb = blockIdx.y * gcols, ge = min(kc, gb + gcols);
+    if (gb >= ge) return;
     const int per = (ge - gb + SPLIT - 1) / SPLIT;
     const int cb = gb + part * per, ce = min(ge, cb + per);
     if (cb >= ce) return;
@@ -203,7 +206,7 @@ __global__ __launch_bounds__(kBlock) void k_sell_spmm(int n, int nslices,
 template <typename VT, typename XT, int MAXCK, int SPLIT, int DOT>
 void launch_sell_spmm_v(const Mat *A, const VT *vals, const XT *X, long ldx, XT *Y, long ldy, int k,
                         int gcols, const int *d_xmap, const int *d_ymap, double *dpart,
-                        hipEvent_t ev0, hipEvent_t ev1)
+                        const int *d_count, hipEvent_t ev0, hipEvent_t ev1)
 {
     constexpr int spb = kBlock / kWave / SPLIT;
     const int ng = (k + gcols - 1) / gcols;
@@ -216,7 +219,7 @@ void launch_sell_spmm_v(const Mat *A, const VT *vals, const XT *X, long ldx, XT 
     hipExtLaunchKernelGGL((k_sell_spmm<VT, XT, MAXCK, SPLIT, DOT>), dim3(nb, ng), dim3(kBlock), 0,
                           state().stream, ev0, ev1, 0, A->nrows, A->nslices, A->d_slice_ptr,
                           A->d_slice_w, A->d_col, vals, X, ldx, Y, ldy, k, gcols, d_xmap, d_ymap,
-                          dpart);
+                          dpart, d_count);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -224,8 +227,8 @@ void launch_sell_spmm_v(const Mat *A, const VT *vals, const XT *X, long ldx, XT 
 // GCGE_HIP_SPMM_SPLIT = 1 | 2 | 4 overrides the split (for measurement)
 template <int DOT, typename VT, typename XT>
 void launch_sell_spmm(const Mat *A, const VT *vals, const XT *X, long ldx, XT *Y, long ldy, int k,
-                      const int *d_xmap, const int *d_ymap, double *dpart, hipEvent_t ev0,
-                      hipEvent_t ev1)
+                      const int *d_xmap, const int *d_ymap, double *dpart, const int *d_count,
+                      hipEvent_t ev0, hipEvent_t ev1)
 {
     static const int forced = [] {
         const char *e = getenv("GCGE_HIP_SPMM_SPLIT");
@@ -235,7 +238,10 @@ void launch_sell_spmm(const Mat *A, const VT *vals, const XT *X, long ldx, XT *Y
         const char *e = getenv("GCGE_HIP_SPMM_GROUP");
         return e ? atoi(e) : 0;
     }();
-    const int gcols = forced_group > 0 ? std::min(forced_group, k) : k;
+    // measured on MI355X (Q1 128^3, k = 50..200): groups of 32 columns beat
+    // one sweep over all columns from about 64 columns up (k = 100: 3.4 ms
+    // against 4.3 ms on K)
+    const int gcols = forced_group > 0 ? std::min(forced_group, k) : (k > 48 ? 32 : k);
     const int kg = gcols;
     int split = forced;
     // measured on MI355X (Q1 64^3, k = 1..50): one wave per slice is best up to
@@ -244,7 +250,7 @@ void launch_sell_spmm(const Mat *A, const VT *vals, const XT *X, long ldx, XT *Y
     const int per = (kg + split - 1) / split;
 #define GCGE_SPMM_LAUNCH(MC, SP) \
     launch_sell_spmm_v<VT, XT, MC, SP, DOT>(A, vals, X, ldx, Y, ldy, k, gcols, d_xmap, d_ymap, dpart, \
-                                            ev0, ev1)
+                                            d_count, ev0, ev1)
     if (split == 1) {
         if (per <= 4) GCGE_SPMM_LAUNCH(4, 1); else if (per <= 8) GCGE_SPMM_LAUNCH(8, 1); else GCGE_SPMM_LAUNCH(16, 1);
     } else if (split == 2) {
@@ -361,7 +367,8 @@ __global__ __launch_bounds__(kBlock) void k_dia_spmm(int n, int nblocks, DiaDesc
                                                     XT *__restrict__ Y, long ldy, int k, int gcols,
                                                     const int *__restrict__ xmap,
                                                     const int *__restrict__ ymap,
-                                                    double *__restrict__ dpart)
+                                                    double *__restrict__ dpart,
+                                                    const int *__restrict__ count)
 {
     constexpr int kBlocksPerGroup = kBlock / kWave / SPLIT;
     const int blk = xcd_remap(blockIdx.x, gridDim.x);
@@ -369,7 +376,9 @@ __global__ __launch_bounds__(kBlock) void k_dia_spmm(int n, int nblocks, DiaDesc
     const int rb = blk * kBlocksPerGroup + wid / SPLIT;
     if (rb >= nblocks) return;
     const int part = wid % SPLIT;
-    const int gb = blockIdx.y * gcols, ge = min(k, gb + gcols);
+    const int kc = count ? min(k, *count) : k;  // columns listed on the device
+    const int gb = blockIdx.y * gcols, ge = min(kc, gb + gcols);
+    if (gb >= ge) return;
     const int per = (ge - gb + SPLIT - 1) / SPLIT;
     const int cb = gb + part * per, ce = min(ge, cb + per);
     if (cb >= ce) return;
@@ -395,8 +404,8 @@ inline int dia_blocks(const Mat *A) { return (A->nrows + kDiaRows - 1) / kDiaRow
 // override the group width and the split (for measurement).
 template <int DOT, typename VT, typename XT>
 void launch_dia_spmm(const Mat *A, const VT *vals, const XT *X, long ldx, XT *Y, long ldy, int k,
-                     const int *d_xmap, const int *d_ymap, double *dpart, hipEvent_t ev0,
-                     hipEvent_t ev1)
+                     const int *d_xmap, const int *d_ymap, double *dpart, const int *d_count,
+                     hipEvent_t ev0, hipEvent_t ev1)
 {
     static const int forced_group = [] {
         const char *e = getenv("GCGE_HIP_DIA_GROUP");
@@ -426,7 +435,7 @@ void launch_dia_spmm(const Mat *A, const VT *vals, const XT *X, long ldx, XT *Y,
 #define GCGE_DIA_LAUNCH(CK, SP)                                                                      \
     hipExtLaunchKernelGGL((k_dia_spmm<VT, XT, CK, SP, DOT>), dim3(nb, ng), dim3(kBlock), 0,          \
                           state().stream, ev0, ev1, 0, A->nrows, nblocks, A->dia, vals, ldd, X, ldx,  \
-                          Y, ldy, k, gcols, d_xmap, d_ymap, dpart)
+                          Y, ldy, k, gcols, d_xmap, d_ymap, dpart, d_count)
     const int ck_auto = sizeof(XT) == 8 ? 2 : (per <= 2 ? 2 : 4);
     const int ckw = forced_ck == 2 || forced_ck == 4 || forced_ck == 8 ? forced_ck : ck_auto;
 #define GCGE_DIA_CK(SP)                                     \
@@ -827,11 +836,12 @@ __global__ __launch_bounds__(kBlock) void k_bcg_p(int n, PT *__restrict__ P, lon
                                                  const int *__restrict__ rmap,
                                                  const double *__restrict__ beta,
                                                  const double *__restrict__ s_old,
-                                                 const double *__restrict__ s_new, int first)
+                                                 const double *__restrict__ s_new, int first,
+                                                 const int *__restrict__ count)
 {
     const int row = blockIdx.x * kBlock + threadIdx.x;
     const int c = blockIdx.y;
-    if (row >= n) return;
+    if (row >= n || (count && c >= *count)) return;
     PT *p = P + (long)pmap[c] * ldp + row;
     const double r = R[(long)rmap[c] * ldr + row];
     if (!s_new) {
@@ -887,10 +897,12 @@ __global__ __launch_bounds__(kBlock) void k_bcg_xr(int n, int rows_per_block,
                                                   const double *__restrict__ rho2,
                                                   const double *__restrict__ ptw,
                                                   const double *__restrict__ scale,
-                                                  double *__restrict__ partial)
+                                                  double *__restrict__ partial,
+                                                  const int *__restrict__ count)
 {
     __shared__ double lds[kBlock / kWave];
     const int bx = blockIdx.x, c = blockIdx.y;
+    if (count && c >= *count) return;
     const double alpha = scale ? rho2[c] / (scale[c] * ptw[c]) : rho2[c] / ptw[c];
     double *x = X + (long)xmap[c] * ldx;
     double *r = R + (long)rmap[c] * ldr;
@@ -919,6 +931,114 @@ __global__ __launch_bounds__(kBlock) void k_axpy_cols(int n, const double *__res
     const int c = blockIdx.y;
     if (row >= n) return;
     Y[(long)(ymap ? ymap[c] : c) * ldy + row] += (T)alpha[c] * X[(long)(xmap ? xmap[c] : c) * ldx + row];
+}
+
+// Block-CG bookkeeping (reference src/gcge_cg.c:395-490 order of
+// operations), one workgroup; the state lives in LDS during the launch.
+__global__ __launch_bounds__(kBlock) void k_bcg_control(BcgControl d, int init, int finished_step,
+                                                       int prepare_step)
+{
+    // the host form of this arithmetic has no fused multiply-adds: the same
+    // roundings here keep both forms bit-identical
+#pragma clang fp contract(off)
+    __shared__ double s_rho1[kBcgControlMax], s_rho2[kBcgControlMax], s_err[kBcgControlMax],
+        s_lerr[kBcgControlMax], s_scol[kBcgControlMax], s_pn[kBcgControlMax];
+    __shared__ int s_unlock[kBcgControlMax];
+    __shared__ int s_count;
+    const int t = threadIdx.x, len = d.len;
+    for (int i = t; i < len; i += kBlock) {
+        s_rho1[i] = d.rho1[i];
+        if (init) {
+            const double r = d.rr[i];
+            s_rho2[i] = r;
+            s_err[i] = sqrt(r);
+            s_lerr[i] = s_err[i];
+            s_scol[i] = 1.0;
+            s_pn[i] = 0.0;
+            s_unlock[i] = i;
+        } else {
+            s_rho2[i] = d.rho2[i];
+            s_err[i] = d.err[i];
+            s_lerr[i] = d.last_err[i];
+            s_scol[i] = d.s_col[i];
+            s_pn[i] = d.pn_col[i];
+            s_unlock[i] = d.unlock[i];
+        }
+    }
+    if (t == 0) s_count = init ? len : *d.count;
+    __syncthreads();
+    const int nu = s_count;
+    if (finished_step > 0) {
+        // rho1 <- rho2, rho2 <- r.r, |p| for the scaled directions, then
+        // last_err from the new rho2 (each position touches its own entries)
+        for (int id = t; id < nu; id += kBlock) {
+            const int idx = s_unlock[id];
+            const int q = d.by_column ? idx : id;
+            s_rho1[q] = s_rho2[q];
+            s_rho2[q] = d.rr[id];
+            if (d.scaled) s_pn[idx] = sqrt(d.ptw[len + id]);
+            s_lerr[idx] = sqrt(fabs(s_rho2[q]));
+        }
+        if (t == 0) d.hist[finished_step - 1] = nu;
+        __syncthreads();
+    }
+    if (init || finished_step > 0) {
+        // keep the columns that have not reached the rate (wave 0, in order)
+        if (t < kWave) {
+            int out = 0;
+            const double rate = d.rate;
+            for (int base = 0; base < nu; base += kWave) {
+                const int id = base + t;
+                int idx = 0;
+                bool keep = false;
+                if (id < nu) {
+                    idx = s_unlock[id];
+                    keep = init ? s_err[idx] >= 1e-15
+                                : (s_lerr[idx] >= rate * s_err[idx] && s_lerr[idx] >= 1e-15);
+                }
+                const unsigned long long m = __ballot(keep);
+                const int pos = out + __popcll(m & ((1ull << t) - 1ull));
+                if (keep) s_unlock[pos] = idx;
+                out += __popcll(m);
+            }
+            if (t == 0) s_count = out;
+        }
+        __syncthreads();
+    }
+    const int nn = s_count;
+    if (prepare_step > 0) {
+        for (int id = t; id < nn; id += kBlock) {
+            const int idx = s_unlock[id];
+            d.maps[id] = d.p0 + idx;
+            d.maps[len + id] = d.r0 + idx;
+            d.maps[2 * len + id] = d.x0 + idx;
+            d.maps[3 * len + id] = d.w0 + idx;
+            const double beta = prepare_step == 1 ? 0.0 : s_rho2[idx] / s_rho1[idx];  // by column
+            d.beta[id] = beta;
+            d.alpha_num[id] = s_rho2[d.by_column ? idx : id];  // reference: by position
+            if (d.scaled) {
+                double snew = prepare_step == 1 ? s_lerr[idx]
+                                                : s_lerr[idx] + fabs(beta) * s_scol[idx] * s_pn[idx];
+                if (!(snew > 0.0) && !isnan(snew)) snew = 1.0;
+                d.s_old[id] = s_scol[idx];
+                d.s_new[id] = snew;
+            }
+        }
+        __syncthreads();
+        if (d.scaled)
+            for (int id = t; id < nn; id += kBlock) s_scol[s_unlock[id]] = d.s_new[id];
+        __syncthreads();
+    }
+    for (int i = t; i < len; i += kBlock) {
+        d.rho1[i] = s_rho1[i];
+        d.rho2[i] = s_rho2[i];
+        d.err[i] = s_err[i];
+        d.last_err[i] = s_lerr[i];
+        d.s_col[i] = s_scol[i];
+        d.pn_col[i] = s_pn[i];
+        d.unlock[i] = s_unlock[i];
+    }
+    if (t == 0) *d.count = nn;
 }
 
 inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
@@ -962,11 +1082,11 @@ void spmm(const Mat *A, const double *X, long ldx, double *Y, long ldy, int k, c
                          8.0 * (double)k * (A->ncols + A->nrows);
     ProfScope prof("spmm", bytes, true);
     if (A->use_dia)
-        launch_dia_spmm<0>(A, A->d_dia, X, ldx, Y, ldy, k, d_xmap, d_ymap, nullptr, prof.start_event(),
-                           prof.stop_event());
+        launch_dia_spmm<0>(A, A->d_dia, X, ldx, Y, ldy, k, d_xmap, d_ymap, nullptr, nullptr,
+                           prof.start_event(), prof.stop_event());
     else
-        launch_sell_spmm<0>(A, A->d_val, X, ldx, Y, ldy, k, d_xmap, d_ymap, nullptr, prof.start_event(),
-                            prof.stop_event());
+        launch_sell_spmm<0>(A, A->d_val, X, ldx, Y, ldy, k, d_xmap, d_ymap, nullptr, nullptr,
+                            prof.start_event(), prof.stop_event());
 }
 
 void spmm(const Mat *A, const float *X, long ldx, float *Y, long ldy, int k, const int *d_xmap,
@@ -978,15 +1098,16 @@ void spmm(const Mat *A, const float *X, long ldx, float *Y, long ldy, int k, con
                          4.0 * (double)k * (A->ncols + A->nrows);
     ProfScope prof("spmm32", bytes, true);
     if (A->use_dia && A->d_dia32)
-        launch_dia_spmm<0>(A, A->d_dia32, X, ldx, Y, ldy, k, d_xmap, d_ymap, nullptr,
+        launch_dia_spmm<0>(A, A->d_dia32, X, ldx, Y, ldy, k, d_xmap, d_ymap, nullptr, nullptr,
                            prof.start_event(), prof.stop_event());
     else
-        launch_sell_spmm<0>(A, A->d_val32, X, ldx, Y, ldy, k, d_xmap, d_ymap, nullptr,
+        launch_sell_spmm<0>(A, A->d_val32, X, ldx, Y, ldy, k, d_xmap, d_ymap, nullptr, nullptr,
                             prof.start_event(), prof.stop_event());
 }
 
 void spmm_dot(const Mat *A, const double *P, long ldp, double *W, long ldw, int k,
-              const int *d_pmap, const int *d_wmap, double *d_out)
+              const int *d_pmap, const int *d_wmap, double *d_out, const int *d_count,
+              int *prof_slot)
 {
     if (k <= 0 || A->nrows <= 0) return;
     const double bytes = (double)A->nnz * (8 + 4) + 4.0 * (A->nrows + 1) +
@@ -995,11 +1116,12 @@ void spmm_dot(const Mat *A, const double *P, long ldp, double *W, long ldw, int 
     double *dpart = partial_buffer((size_t)nparts * k);
     {
         ProfScope prof("spmm", bytes, true);
+        if (prof_slot) *prof_slot = prof.slot;
         if (A->use_dia)
-            launch_dia_spmm<1>(A, A->d_dia, P, ldp, W, ldw, k, d_pmap, d_wmap, dpart,
+            launch_dia_spmm<1>(A, A->d_dia, P, ldp, W, ldw, k, d_pmap, d_wmap, dpart, d_count,
                                prof.start_event(), prof.stop_event());
         else
-            launch_sell_spmm<1>(A, A->d_val, P, ldp, W, ldw, k, d_pmap, d_wmap, dpart,
+            launch_sell_spmm<1>(A, A->d_val, P, ldp, W, ldw, k, d_pmap, d_wmap, dpart, d_count,
                                 prof.start_event(), prof.stop_event());
     }
     hipLaunchKernelGGL(k_reduce_rows, dim3(k), dim3(kBlock), 0, state().stream, dpart, nparts, d_out);
@@ -1007,7 +1129,7 @@ void spmm_dot(const Mat *A, const double *P, long ldp, double *W, long ldw, int 
 }
 
 void spmm_dot(const Mat *A, const float *P, long ldp, float *W, long ldw, int k, const int *d_pmap,
-              const int *d_wmap, double *d_out)
+              const int *d_wmap, double *d_out, const int *d_count, int *prof_slot)
 {
     if (k <= 0 || A->nrows <= 0) return;
     if (!A->d_val32) fail_msg("float32 SpMM on a matrix without float32 values");
@@ -1018,15 +1140,24 @@ void spmm_dot(const Mat *A, const float *P, long ldp, float *W, long ldw, int k,
     double *dpart = partial_buffer((size_t)nparts * 2 * k);
     {
         ProfScope prof("spmm32", bytes, true);
+        if (prof_slot) *prof_slot = prof.slot;
         if (dia)
-            launch_dia_spmm<2>(A, A->d_dia32, P, ldp, W, ldw, k, d_pmap, d_wmap, dpart,
+            launch_dia_spmm<2>(A, A->d_dia32, P, ldp, W, ldw, k, d_pmap, d_wmap, dpart, d_count,
                                prof.start_event(), prof.stop_event());
         else
-            launch_sell_spmm<2>(A, A->d_val32, P, ldp, W, ldw, k, d_pmap, d_wmap, dpart,
+            launch_sell_spmm<2>(A, A->d_val32, P, ldp, W, ldw, k, d_pmap, d_wmap, dpart, d_count,
                                 prof.start_event(), prof.stop_event());
     }
     hipLaunchKernelGGL(k_reduce_rows, dim3(2 * k), dim3(kBlock), 0, state().stream, dpart, nparts,
                        d_out);
+    HIP_CHECK(hipGetLastError());
+}
+
+void bcg_control(const BcgControl &c, int init, int finished_step, int prepare_step)
+{
+    if (c.len > kBcgControlMax) fail_msg("block CG bookkeeping: too many columns for one workgroup");
+    hipLaunchKernelGGL(k_bcg_control, dim3(1), dim3(kBlock), 0, state().stream, c, init,
+                       finished_step, prepare_step);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -1249,28 +1380,28 @@ void residual_sq(int n, const double *AX, long ldax, const double *BX, long ldbx
 template <typename PT>
 static void bcg_p_t(int n, PT *P, long ldp, const int *d_pmap, const double *R, long ldr,
                     const int *d_rmap, const double *d_beta, const double *d_sold,
-                    const double *d_snew, int first, int k)
+                    const double *d_snew, int first, int k, const int *d_count)
 {
     if (k <= 0) return;
     ProfScope prof("bcg_p");
     hipLaunchKernelGGL((k_bcg_p<PT>), dim3(ceil_div(n, kBlock), k), dim3(kBlock), 0,
                        state().stream, n, P, ldp, d_pmap, R, ldr, d_rmap, d_beta, d_sold, d_snew,
-                       first);
+                       first, d_count);
     HIP_CHECK(hipGetLastError());
 }
 
 void bcg_p_update(int n, double *P, long ldp, const int *d_pmap, const double *R, long ldr,
-                  const int *d_rmap, const double *d_beta, int first, int k)
+                  const int *d_rmap, const double *d_beta, int first, int k, const int *d_count)
 {
     bcg_p_t(n, P, ldp, d_pmap, R, ldr, d_rmap, d_beta, (const double *)nullptr,
-            (const double *)nullptr, first, k);
+            (const double *)nullptr, first, k, d_count);
 }
 
 void bcg_p_update(int n, float *P, long ldp, const int *d_pmap, const double *R, long ldr,
                   const int *d_rmap, const double *d_beta, const double *d_sold,
-                  const double *d_snew, int first, int k)
+                  const double *d_snew, int first, int k, const int *d_count)
 {
-    bcg_p_t(n, P, ldp, d_pmap, R, ldr, d_rmap, d_beta, d_sold, d_snew, first, k);
+    bcg_p_t(n, P, ldp, d_pmap, R, ldr, d_rmap, d_beta, d_sold, d_snew, first, k, d_count);
 }
 
 void coldots_pw(int n, const float *P, long ldp, const int *d_pmap, const float *W, long ldw,
@@ -1292,7 +1423,7 @@ template <typename PT>
 static void bcg_xr_t(int n, double *X, long ldx, const int *d_xmap, double *R, long ldr,
                      const int *d_rmap, const PT *P, long ldp, const int *d_pmap, const PT *W,
                      long ldw, const int *d_wmap, const double *d_rho2, const double *d_ptw,
-                     const double *d_scale, int k, double *d_rr_out)
+                     const double *d_scale, int k, double *d_rr_out, const int *d_count)
 {
     if (k <= 0) return;
     const int rows = reduction_rows(n, k);
@@ -1301,7 +1432,7 @@ static void bcg_xr_t(int n, double *X, long ldx, const int *d_xmap, double *R, l
     ProfScope prof("bcg_xr");
     hipLaunchKernelGGL((k_bcg_xr<PT>), dim3(nbx, k), dim3(kBlock), 0, state().stream, n, rows, X,
                        ldx, d_xmap, R, ldr, d_rmap, P, ldp, d_pmap, W, ldw, d_wmap, d_rho2, d_ptw,
-                       d_scale, partial);
+                       d_scale, partial, d_count);
     hipLaunchKernelGGL(k_reduce_rows, dim3(k), dim3(kBlock), 0, state().stream, partial, nbx,
                        d_rr_out);
     HIP_CHECK(hipGetLastError());
@@ -1310,19 +1441,19 @@ static void bcg_xr_t(int n, double *X, long ldx, const int *d_xmap, double *R, l
 void bcg_xr_update(int n, double *X, long ldx, const int *d_xmap, double *R, long ldr,
                    const int *d_rmap, const double *P, long ldp, const int *d_pmap,
                    const double *W, long ldw, const int *d_wmap, const double *d_rho2,
-                   const double *d_ptw, int k, double *d_rr_out)
+                   const double *d_ptw, int k, double *d_rr_out, const int *d_count)
 {
     bcg_xr_t(n, X, ldx, d_xmap, R, ldr, d_rmap, P, ldp, d_pmap, W, ldw, d_wmap, d_rho2, d_ptw,
-             (const double *)nullptr, k, d_rr_out);
+             (const double *)nullptr, k, d_rr_out, d_count);
 }
 
 void bcg_xr_update(int n, double *X, long ldx, const int *d_xmap, double *R, long ldr,
                    const int *d_rmap, const float *P, long ldp, const int *d_pmap, const float *W,
                    long ldw, const int *d_wmap, const double *d_rho2, const double *d_ptw,
-                   const double *d_scale, int k, double *d_rr_out)
+                   const double *d_scale, int k, double *d_rr_out, const int *d_count)
 {
     bcg_xr_t(n, X, ldx, d_xmap, R, ldr, d_rmap, P, ldp, d_pmap, W, ldw, d_wmap, d_rho2, d_ptw,
-             d_scale, k, d_rr_out);
+             d_scale, k, d_rr_out, d_count);
 }
 
 template <typename T>

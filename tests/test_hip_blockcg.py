@@ -60,18 +60,23 @@ def test_device_block_cg_matches_reference_fixture(case, golden_dir):
     _close(r, g[case + "/r"], 1e-9)
 
 
+@pytest.mark.parametrize("bookkeeping", ["device", "host"])
 @pytest.mark.parametrize("diagonal", [True, False])
 @pytest.mark.parametrize("shifted", [False, True])
-def test_device_block_cg_matches_restatement(shifted, diagonal):
+def test_device_block_cg_matches_restatement(shifted, diagonal, bookkeeping, monkeypatch):
     """Q1 12^3 stiffness (and mass for the shifted solve), 8 columns of mixed
     scale so columns stop at different iterations; products through the
-    diagonal storage of the matrices and through sliced ELLPACK."""
+    diagonal storage of the matrices and through sliced ELLPACK; the column
+    lists and CG scalars kept on the device (unshifted solves) or on the host
+    (GCGE_HIP_BCG_HOST=1, and every shifted solve)."""
+    if bookkeeping == "host":
+        monkeypatch.setenv("GCGE_HIP_BCG_HOST", "1")
     K, M = problems.q1_fem_3d(12)
     rng = np.random.default_rng(31)
     b = rng.standard_normal((K.nrows, 8)) * np.logspace(0, -2, 8)
     x0 = 1e-3 * rng.standard_normal((K.nrows, 8))
     args = (20, 1e-2, 1, 40.0) if shifted else (20, 1e-2, 0, 0.0)
-    print(f"block CG restatement shifted={shifted} diagonal={diagonal}", flush=True)
+    print(f"block CG restatement shifted={shifted} diagonal={diagonal} {bookkeeping}", flush=True)
     x, r = device_block_cg(K, M, b, x0, *args, diagonal=diagonal)
     xo, ro = oracle.block_cg(K, M, b, x0, *args)
     _close(x, xo, 1e-9)
@@ -101,6 +106,28 @@ def test_device_block_cg_fp32_directions(case, diagonal):
     t32 = np.linalg.norm(b - Op(x32), axis=0)
     t64 = np.linalg.norm(b - Op(x64), axis=0)
     assert np.all(t32 <= 1.5 * t64 + 1e-5 * np.linalg.norm(b, axis=0))
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_device_bookkeeping_matches_host_bookkeeping(fp32, monkeypatch):
+    """Same solve twice, lists and scalars on the device and on the host:
+    identical arithmetic, so identical results (NaN pattern included: the
+    third column starts converged, so later columns read rho1 entries that
+    this call never wrote -- zeros here -- as the reference does)."""
+    K, _ = problems.q1_fem_3d(9)
+    rng = np.random.default_rng(34)
+    b = rng.standard_normal((K.nrows, 7)) * np.logspace(0, -6, 7)
+    b[:, 2] = 0.0
+    x0 = 1e-2 * rng.standard_normal((K.nrows, 7))
+    x0[:, 2] = 0.0
+    out = {}
+    for mode in ("device", "host"):
+        if mode == "host":
+            monkeypatch.setenv("GCGE_HIP_BCG_HOST", "1")
+        print(f"block CG bookkeeping {mode} fp32={fp32}", flush=True)
+        out[mode] = device_block_cg(K, None, b, x0, 20, 1e-3, fp32=fp32)
+    for a, h in zip(out["device"], out["host"]):
+        np.testing.assert_array_equal(a, h)
 
 
 def test_device_block_cg_fp32_large_beta():
