@@ -148,7 +148,9 @@ def kernel_sweep(K, M):
     Y = gcge.DeviceMultiVec(n, 96)
     lib.GCGE_HIP_RandomFill(X.handle, 0, 96)
 
-    def timed(name, fn, reps=20):
+    def timed(name, fn, reps=20, extra=()):
+        """us per launch and GB/s of kernel `name`; `extra` kernels of the
+        same calls (a second reduction pass) are added to the time"""
         lib.GCGE_HIP_ProfileReset()
         lib.GCGE_HIP_ProfileEnable(1)
         for _ in range(reps):
@@ -156,8 +158,13 @@ def kernel_sweep(K, M):
         lib.GCGE_HIP_ProfileEnable(0)
         ms, cnt, byt = C.c_double(0.0), C.c_int(0), C.c_double(0.0)
         lib.GCGE_HIP_KernelStats(name.encode(), C.byref(ms), C.byref(cnt), C.byref(byt))
-        us = ms.value * 1e3 / max(cnt.value, 1)
-        gbs = byt.value / (ms.value * 1e-3) / 1e9 if ms.value > 0 else 0.0
+        total_ms = ms.value
+        for e in extra:
+            ms2, cnt2 = C.c_double(0.0), C.c_int(0)
+            lib.GCGE_HIP_KernelStats(e.encode(), C.byref(ms2), C.byref(cnt2), None)
+            total_ms += ms2.value
+        us = total_ms * 1e3 / max(cnt.value, 1)
+        gbs = byt.value / (total_ms * 1e-3) / 1e9 if total_ms > 0 else 0.0
         return us, gbs
 
     for mat, label in ((dK, "K"), (dM, "M")):
@@ -167,8 +174,10 @@ def kernel_sweep(K, M):
     G = np.zeros((96, 96), order="F")
     for m, k in ((70, 10), (60, 60), (90, 10), (10, 10), (1, 1)):
         us, gbs = timed("gram", lambda: lib.GCGE_HIP_Gram(X.handle, 0, m, Y.handle, 0, k,
-                                                          _native.as_dptr(G), 96))
-        print(f"# kernel gram m={m} k={k}: {us:.1f} us/launch, {gbs:.0f} GB/s", file=sys.stderr)
+                                                          _native.as_dptr(G), 96),
+                        extra=("gram_reduce",))
+        print(f"# kernel gram m={m} k={k} (with its reduction pass): {us:.1f} us/call, {gbs:.0f} GB/s",
+              file=sys.stderr)
     Cm = np.asfortranarray(np.random.default_rng(0).standard_normal((96, 96)))
     for m, k in ((90, 50), (80, 30), (70, 10)):
         us, gbs = timed("lincomb", lambda: lib.GCGE_HIP_LinearComb(X.handle, 0, m, _native.as_dptr(Cm),

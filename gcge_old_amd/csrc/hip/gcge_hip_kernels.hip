@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstring>
 
 #include "gcge_hip_internal.h"
 
@@ -598,7 +599,25 @@ __global__ __launch_bounds__(kBlock) void k_gram_cols(int n, int rows_per_block,
     const double *vcol[MR];
 #pragma unroll
     for (int a = 0; a < MR; ++a) vcol[a] = V + (long)min(i0 + a, m - 1) * ldv;
-    for (int r = r0 + lane; r < r1; r += kWave) {
+    // two rows per lane per step: the loads of both are in flight together
+    int r = r0 + lane;
+    for (; r + kWave < r1; r += 2 * kWave) {
+        double v[2][MR], w[2][KR];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+#pragma unroll
+            for (int a = 0; a < MR; ++a) v[h][a] = (i0 + a < m) ? vcol[a][r + h * kWave] : 0.0;
+#pragma unroll
+            for (int b = 0; b < KR; ++b) w[h][b] = b < k ? W[(long)b * ldw + r + h * kWave] : 0.0;
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int a = 0; a < MR; ++a)
+#pragma unroll
+                for (int b = 0; b < KR; ++b) acc[a][b] += v[h][a] * w[h][b];
+    }
+    if (r < r1) {
         double v[MR], w[KR];
 #pragma unroll
         for (int a = 0; a < MR; ++a) v[a] = (i0 + a < m) ? vcol[a][r] : 0.0;
@@ -617,6 +636,100 @@ __global__ __launch_bounds__(kBlock) void k_gram_cols(int n, int rows_per_block,
             const double t = wave_sum(acc[a][b]);
             if (lane == 0 && i0 + a < m && b < k) out[(long)b * m + i0 + a] = t;
         }
+}
+
+// V^T W on the matrix cores (v_mfma_f64_16x16x4f64).  Workgroup (bx, by, bz)
+// takes rows [bx * rows_per_block, ...), V columns [128 by, +128) and W
+// columns [128 bz, +128).  Per stage, 16 rows of those columns go to LDS
+// (row stride MP + KP + 1 doubles: V in the first MP columns, W after),
+// 16 consecutive threads reading one column's 16 consecutive rows; every
+// thread issues all its loads of a stage at once, and the next stage's loads
+// are in flight while the matrix cores work on the current one.  The
+// (16 x 16) output tiles are shared round-robin by the four waves; each MFMA
+// takes one element of V and one of W per lane from LDS (A: lane l holds
+// Vt[l % 16][l / 16] of the 16 x 4 block, B: W[l / 16][l % 16]), and lane l,
+// component v of the accumulator holds D[l / 16 + 4 v][l % 16] (layout checked
+// against the oracle Gram on the device).  Partials as k_gram.
+typedef double gcge_double4 __attribute__((ext_vector_type(4)));
+constexpr int kGmRows = 16;
+constexpr int kGmCols = 128;
+
+template <int PW, int EPT>
+__global__ __launch_bounds__(kBlock) void k_gram_mfma(int n, int rows_per_block,
+                                                     const double *__restrict__ V, long ldv,
+                                                     int m, const double *__restrict__ W,
+                                                     long ldw, int k,
+                                                     double *__restrict__ partial)
+{
+    extern __shared__ double lds_gm[];
+    const int i_base = blockIdx.y * kGmCols, j_base = blockIdx.z * kGmCols;
+    const int mb = min(kGmCols, m - i_base), kb = min(kGmCols, k - j_base);
+    const int ti = (mb + 15) / 16, tj = (kb + 15) / 16;
+    const int MP = ti * 16, KP = tj * 16, npairs = ti * tj;
+    const int ld = MP + KP + 1;
+    const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+    const int r0 = blockIdx.x * rows_per_block, r1 = min(n, r0 + rows_per_block);
+    const int ncols = MP + KP;
+    // this thread's stage elements: row q, columns c0 + 16 t (16 = kBlock / kGmRows)
+    constexpr int kColStep = kBlock / kGmRows;
+    const int q = threadIdx.x % kGmRows, c0 = threadIdx.x / kGmRows;
+    double buf[EPT];
+    auto load_stage = [&](int rs) {
+        const int r = rs + q;
+#pragma unroll
+        for (int t = 0; t < EPT; ++t) {
+            const int c = c0 + kColStep * t;
+            double v = 0.0;
+            if (r < r1) {
+                if (c < MP) {
+                    if (c < mb) v = V[(long)(i_base + c) * ldv + r];
+                } else if (c - MP < kb) {
+                    v = W[(long)(j_base + c - MP) * ldw + r];
+                }
+            }
+            buf[t] = v;
+        }
+    };
+    gcge_double4 acc[PW];
+#pragma unroll
+    for (int p = 0; p < PW; ++p) acc[p] = gcge_double4{0.0, 0.0, 0.0, 0.0};
+    if (r0 < r1) load_stage(r0);
+    for (int rs = r0; rs < r1; rs += kGmRows) {
+#pragma unroll
+        for (int t = 0; t < EPT; ++t) {
+            const int c = c0 + kColStep * t;
+            if (c < ncols) lds_gm[q * ld + c] = buf[t];
+        }
+        __syncthreads();
+        if (rs + kGmRows < r1) load_stage(rs + kGmRows);
+#pragma unroll
+        for (int kk = 0; kk < kGmRows / 4; ++kk) {
+            const int qa = kk * 4 + lane / 16;
+#pragma unroll
+            for (int p = 0; p < PW; ++p) {
+                const int pair = wid + 4 * p;
+                if (pair < npairs) {
+                    const double a = lds_gm[qa * ld + (pair / tj) * 16 + lane % 16];
+                    const double b = lds_gm[qa * ld + MP + (pair % tj) * 16 + lane % 16];
+                    acc[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[p], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    double *out = partial + (long)blockIdx.x * m * k;
+#pragma unroll
+    for (int p = 0; p < PW; ++p) {
+        const int pair = wid + 4 * p;
+        if (pair < npairs) {
+            const int j = j_base + (pair % tj) * 16 + lane % 16;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = i_base + (pair / tj) * 16 + lane / 16 + 4 * v;
+                if (i < i_base + mb && j < j_base + kb) out[(long)j * m + i] = acc[p][v];
+            }
+        }
+    }
 }
 
 // out[e] = sum_b partial[b * stride + e], e < count.  A block owns OUTS
@@ -1051,8 +1164,12 @@ void reduce_partials(const double *partial, long stride, int nparts, long count,
         hipLaunchKernelGGL(k_reduce_partials<1>, dim3(1), dim3(kBlock), 0, st, partial, stride, nparts, count, out);
     else if (count <= 4)
         hipLaunchKernelGGL(k_reduce_partials<4>, dim3(1), dim3(kBlock), 0, st, partial, stride, nparts, count, out);
-    else if (count <= 16)
-        hipLaunchKernelGGL(k_reduce_partials<16>, dim3(1), dim3(kBlock), 0, st, partial, stride, nparts, count, out);
+    else if (count <= 4096)
+        // 16 threads per output: at least count / 16 workgroups share the
+        // partials (64 outputs per workgroup left a 10 x 70 block of 200
+        // partials to 11 workgroups, 15 us of mostly latency)
+        hipLaunchKernelGGL(k_reduce_partials<16>, dim3(ceil_div(count, 16)), dim3(kBlock), 0, st, partial, stride,
+                           nparts, count, out);
     else
         hipLaunchKernelGGL(k_reduce_partials<64>, dim3(ceil_div(count, 64)), dim3(kBlock), 0, st, partial, stride,
                            nparts, count, out);
@@ -1200,6 +1317,67 @@ void gram(int n, const double *V, long ldv, int m, const double *W, long ldw, in
             GCGE_GRAM_SMALL(4, 1) GCGE_GRAM_SMALL(4, 2) GCGE_GRAM_SMALL(4, 4) GCGE_GRAM_SMALL(4, 8)
             GCGE_GRAM_SMALL(8, 1) GCGE_GRAM_SMALL(8, 2) GCGE_GRAM_SMALL(8, 4) GCGE_GRAM_SMALL(8, 8)
 #undef GCGE_GRAM_SMALL
+            HIP_CHECK(hipGetLastError());
+        }
+        {
+            ProfScope prof("gram_reduce");
+            reduce_partials(partial, mk, nbx, mk, d_out);
+        }
+        double *h = result_buffer((size_t)mk);
+        HIP_CHECK(hipMemcpyAsync(h, d_out, mk * sizeof(double), hipMemcpyDeviceToHost, s.stream));
+        if (!sync_result) return;
+        HIP_CHECK(hipStreamSynchronize(s.stream));
+        for (int j = 0; j < k; ++j)
+            for (int i = 0; i < m; ++i) G[(long)j * ldg + i] = h[(long)j * m + i];
+        return;
+    }
+    static const int gram_mode = [] {
+        const char *e = getenv("GCGE_HIP_GRAM");  // "fma": the vector-FMA kernels (measurement)
+        return (e && strcmp(e, "fma") == 0) ? 1 : 0;
+    }();
+    if (gram_mode == 0) {
+        // matrix cores: 128 x 128 output blocks (grid y, z), tile pairs shared by four waves
+        const int gy = ceil_div(m, kGmCols), gz = ceil_div(k, kGmCols);
+        const int ti = ceil_div(std::min(m, kGmCols), 16), tj = ceil_div(std::min(k, kGmCols), 16);
+        const int pw = ceil_div(ti * tj, kBlock / kWave);
+        const int ncols = 16 * (ti + tj);
+        const int ept = ceil_div(kGmRows * ncols, kBlock);
+        const size_t lds = (size_t)kGmRows * (ncols + 1) * sizeof(double);
+        // about four workgroups per CU in all, at least two stages each, and
+        // partials of at most a tenth of the input bytes (the reduction pass
+        // reads them all)
+        const long cap_parts = std::max(1L, (long)n * (m + k) / (10L * mk));
+        int nbx = std::max(1, std::min(ceil_div(n, 2 * kGmRows), std::max(1, 1024 / (gy * gz))));
+        nbx = (int)std::min<long>(nbx, cap_parts);
+        int rows = ceil_div(n, nbx);
+        rows = (rows + kGmRows - 1) / kGmRows * kGmRows;
+        nbx = std::max(1, ceil_div(n, rows));
+        double *partial = partial_buffer((size_t)nbx * mk);
+        double *d_out = small_buffer2((size_t)mk);
+        {
+            ProfScope prof("gram", 8.0 * (double)n * (m + k));
+            dim3 grid(nbx, gy, gz);
+#define GCGE_GRAM_MFMA(P, E)                                                                       \
+    hipLaunchKernelGGL((k_gram_mfma<P, E>), grid, dim3(kBlock), lds, s.stream, n, rows, V, ldv, m, W, ldw, \
+                       k, partial)
+#define GCGE_GRAM_EPT(P)                     \
+    if (ept <= 2) GCGE_GRAM_MFMA(P, 2);      \
+    else if (ept <= 4) GCGE_GRAM_MFMA(P, 4);  \
+    else if (ept <= 8) GCGE_GRAM_MFMA(P, 8); \
+    else GCGE_GRAM_MFMA(P, 16);
+            if (pw <= 1) {
+                GCGE_GRAM_EPT(1)
+            } else if (pw <= 2) {
+                GCGE_GRAM_EPT(2)
+            } else if (pw <= 8) {
+                // (the 4-pair instance keeps its accumulators in VGPRs and
+                // drops to one or two waves per SIMD; the 8-pair one does not)
+                GCGE_GRAM_EPT(8)
+            } else {
+                GCGE_GRAM_EPT(16)
+            }
+#undef GCGE_GRAM_EPT
+#undef GCGE_GRAM_MFMA
             HIP_CHECK(hipGetLastError());
         }
         {
