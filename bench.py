@@ -426,8 +426,9 @@ def main():
                     "traffic": read_traffic(),
                     "traffic_note": "HBM bytes per SpMM launch on K with 10 columns (the block-CG "
                                     "shape), rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a copy "
-                                    "calibration; algorithmic bytes of that launch: "
-                                    f"{K.nnz * 12 + 4 * (K.nrows + 1) + 160 * K.nrows}",
+                                    "calibration; algorithmic bytes of that launch (CSR formula): "
+                                    f"{K.nnz * 12 + 4 * (K.nrows + 1) + 160 * K.nrows}; the "
+                                    "diagonal-storage kernel reads no column indices",
                     "kernel": "spmm", "launches": sp["launches"],
                     "avg_launch_us": round(sp["ms"] * 1e3 / sp["launches"], 2),
                     "algorithmic_bytes_per_launch": round(sp["bytes"] / sp["launches"])}
