@@ -434,7 +434,8 @@ def main():
         achieved = sp["bytes"] / (sp["ms"] * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": read_traffic(),
+                    # the counter summary covers the 64^3 K product only
+                    "traffic": read_traffic() if (E == 64 and not lognormal) else None,
                     "traffic_note": "HBM bytes per SpMM launch on K with 10 columns (the block-CG "
                                     "shape), rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a copy "
                                     "calibration; algorithmic bytes of that launch (CSR formula): "

@@ -696,9 +696,11 @@ struct CGParams {
 };
 static CGParams g_cg;
 
-// CG iterations of the block CG after the initial residual: P and W hold
-// float64 (the caller's blocks or backend scratch) or float32 (backend
-// scratch; the products A p then use the float32 values).
+// CG iterations of the block CG after the initial residual: P, W and the
+// residual block R hold float64 (the caller's blocks or backend scratch) or
+// float32 (backend scratch; the products A p then use the float32 values;
+// the float64 initial residual is rounded to float32 once and the final one
+// written back); x, the scalars and every dot product stay float64.
 //
 // The float32 directions are stored scaled, p = s p_stored, with one scale
 // per column chosen so that |p_stored| <= 1: s_new = |r| + |beta| s_old
@@ -709,7 +711,7 @@ static CGParams g_cg;
 // by column, src/gcge_cg.c:405,467-476) gives a large beta, which float64
 // absorbs and unscaled float32 directions would overflow.
 template <typename PT>
-static void bcg_iterate(Mat *A, Mat *B, int if_shift, double shift, int n, int len, double *Rd,
+static void bcg_iterate(Mat *A, Mat *B, int if_shift, double shift, int n, int len, PT *Rd,
                         long ldr, int r0, double *Xd, long ldx, int x0, PT *Pd, long ldp, int p0,
                         PT *Wd, long ldw, int w0, PT *T, long ldt, int max_it, double rate,
                         double *rho1, double *rho2, const double *err, double *last_err,
@@ -800,6 +802,26 @@ static void bcg_iterate(Mat *A, Mat *B, int if_shift, double shift, int n, int l
     }
 }
 
+// float32 copy of a column block and back (same row count; one launch when
+// the leading dimensions agree)
+static void block_to_float(const double *src, long lds, float *dst, long ldd, int n, int k)
+{
+    if (lds == ldd) {
+        to_float(src, dst, (size_t)k * lds);
+        return;
+    }
+    for (int c = 0; c < k; ++c) to_float(src + (long)c * lds, dst + (long)c * ldd, (size_t)n);
+}
+
+static void block_to_double(const float *src, long lds, double *dst, long ldd, int n, int k)
+{
+    if (lds == ldd) {
+        to_double(src, dst, (size_t)k * lds);
+        return;
+    }
+    for (int c = 0; c < k; ++c) to_double(src + (long)c * lds, dst + (long)c * ldd, (size_t)n);
+}
+
 // The same iteration with the bookkeeping on the device (k_bcg_control):
 // every step is enqueued without waiting for the previous one; a step whose
 // list is empty does nothing.  The host scalars (rho1 of earlier calls
@@ -807,7 +829,7 @@ static void bcg_iterate(Mat *A, Mat *B, int if_shift, double shift, int n, int l
 // once and come back once, so dtmp/itmp end as after the host loop.
 // d_rr0 holds the initial r.r of the len columns.
 template <typename PT>
-static void bcg_iterate_dev(Mat *A, int n, int len, double *Rd, long ldr, int r0, double *Xd,
+static void bcg_iterate_dev(Mat *A, int n, int len, PT *Rd, long ldr, int r0, double *Xd,
                             long ldx, int x0, PT *Pd, long ldp, int p0, PT *Wd, long ldw, int w0,
                             int max_it, double rate, double *rho1, double *rho2, double *err,
                             double *last_err, int *unlock, int *num_unlock, double *d_st,
@@ -967,12 +989,15 @@ static void block_cg(void *Amat, int if_shift, double shift, void *Bmat, void **
                 d_st + 12 * len);
         int num_unlock = 0;
         if (f32) {
-            long lpf, lwf;
+            long lpf, lwf, lrf;
             float *Pf = scratch_f32(0, n, len, &lpf);
             float *Wf = scratch_f32(1, n, len, &lwf);
-            bcg_iterate_dev<float>(A, n, len, Rd, ldr, start[0], Xd, ldx, start[1], Pf, lpf, 0, Wf,
-                                   lwf, 0, max_it, rate, rho1, rho2, err, last_err, unlock,
-                                   &num_unlock, d_st, d_it, by_column);
+            float *Rf = scratch_f32(3, n, len, &lrf);
+            block_to_float(Rd + (long)start[0] * ldr, ldr, Rf, lrf, n, len);
+            bcg_iterate_dev<float>(A, n, len, Rf, lrf, 0, Xd, ldx, start[1], Pf, lpf, 0, Wf, lwf, 0,
+                                   max_it, rate, rho1, rho2, err, last_err, unlock, &num_unlock,
+                                   d_st, d_it, by_column);
+            block_to_double(Rf, lrf, Rd + (long)start[0] * ldr, ldr, n, len);
         } else {
             bcg_iterate_dev<double>(A, n, len, Rd, ldr, start[0], Xd, ldx, start[1], Pd, ldp, p0, Wd,
                                     ldw, w0, max_it, rate, rho1, rho2, err, last_err, unlock,
@@ -993,13 +1018,16 @@ static void block_cg(void *Amat, int if_shift, double shift, void *Bmat, void **
         if (err[i] >= 1e-15) unlock[num_unlock++] = i;
     }
     if (f32) {
-        long lpf, lwf, ltf = 0;
+        long lpf, lwf, ltf = 0, lrf;
         float *Pf = scratch_f32(0, n, len, &lpf);
         float *Wf = scratch_f32(1, n, len, &lwf);
         float *Tf = (if_shift == 1 && B) ? scratch_f32(2, n, len, &ltf) : nullptr;
-        bcg_iterate<float>(A, B, if_shift, shift, n, len, Rd, ldr, start[0], Xd, ldx, start[1], Pf,
-                           lpf, 0, Wf, lwf, 0, Tf, ltf, max_it, rate, rho1, rho2, err, last_err,
-                           unlock, num_unlock, by_column);
+        float *Rf = scratch_f32(3, n, len, &lrf);
+        block_to_float(Rd + (long)start[0] * ldr, ldr, Rf, lrf, n, len);
+        bcg_iterate<float>(A, B, if_shift, shift, n, len, Rf, lrf, 0, Xd, ldx, start[1], Pf, lpf,
+                           0, Wf, lwf, 0, Tf, ltf, max_it, rate, rho1, rho2, err, last_err, unlock,
+                           num_unlock, by_column);
+        block_to_double(Rf, lrf, Rd + (long)start[0] * ldr, ldr, n, len);
     } else {
         bcg_iterate<double>(A, B, if_shift, shift, n, len, Rd, ldr, start[0], Xd, ldx, start[1], Pd,
                             ldp, p0, Wd, ldw, w0, T, ldt, max_it, rate, rho1, rho2, err, last_err,

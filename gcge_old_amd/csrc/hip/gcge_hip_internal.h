@@ -142,6 +142,7 @@ void spmm(const Mat *A, const double *X, long ldx, double *Y, long ldy, int k,
 void spmm(const Mat *A, const float *X, long ldx, float *Y, long ldy, int k,
           const int *d_xmap = nullptr, const int *d_ymap = nullptr);
 void to_float(const double *src, float *dst, size_t len);
+void to_double(const float *src, double *dst, size_t len);
 // block-CG product: W(:, wmap[c]) = A P(:, pmap[c]) and, from the same pass,
 // d_out[c] = P(:, pmap[c]) . W(:, wmap[c]); the float32 form also gives
 // d_out[k + c] = P(:, pmap[c]) . P(:, pmap[c]) (sums in float64)
@@ -174,14 +175,14 @@ void coldots(int n, const double *A, long lda, const int *d_amap, const double *
 // residual norms of columns c < k: |AX - lam BX|^2, |X|^2, |BX|^2 -> d_out[3k]
 void residual_sq(int n, const double *AX, long ldax, const double *BX, long ldbx, const double *X,
                  long ldx, const double *d_lam, int k, double *d_out);
-// block CG pieces (column lists on the device); the direction block P and
-// its product W are float64 or float32, x and r stay float64.  The float32
+// block CG pieces (column lists on the device); the direction block P, its
+// product W and the residual block R are float64 or float32, x stays float64.  The float32
 // directions are stored scaled, p = s p_stored with per-column s (d_sold,
 // d_snew, d_scale: one entry per listed column).
 void bcg_p_update(int n, double *P, long ldp, const int *d_pmap, const double *R, long ldr,
                   const int *d_rmap, const double *d_beta, int first, int k,
                   const int *d_count = nullptr);
-void bcg_p_update(int n, float *P, long ldp, const int *d_pmap, const double *R, long ldr,
+void bcg_p_update(int n, float *P, long ldp, const int *d_pmap, const float *R, long ldr,
                   const int *d_rmap, const double *d_beta, const double *d_sold,
                   const double *d_snew, int first, int k, const int *d_count = nullptr);
 // d_out[c] = p_c . w_c, d_out[k + c] = p_c . p_c (float32 blocks, float64 sums)
@@ -191,7 +192,7 @@ void bcg_xr_update(int n, double *X, long ldx, const int *d_xmap, double *R, lon
                    const int *d_rmap, const double *P, long ldp, const int *d_pmap,
                    const double *W, long ldw, const int *d_wmap, const double *d_rho2,
                    const double *d_ptw, int k, double *d_rr_out, const int *d_count = nullptr);
-void bcg_xr_update(int n, double *X, long ldx, const int *d_xmap, double *R, long ldr,
+void bcg_xr_update(int n, double *X, long ldx, const int *d_xmap, float *R, long ldr,
                    const int *d_rmap, const float *P, long ldp, const int *d_pmap, const float *W,
                    long ldw, const int *d_wmap, const double *d_rho2, const double *d_ptw,
                    const double *d_scale, int k, double *d_rr_out, const int *d_count = nullptr);

@@ -46,10 +46,34 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb)
     return (xcd < r) ? xcd * (q + 1) + loc : r * (q + 1) + (xcd - r) * q + loc;
 }
 
+// DPP move of a double (two 32-bit halves); lanes without a source, or in
+// rows outside ROW_MASK, get 0
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_move(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, ROW_MASK, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over the 64 lanes, returned to every lane (all lanes must be active).
+// Within each row of 16 lanes, row_shr:1/2/4/8 steps leave the row sum in the
+// row's last lane; row_bcast:15 adds rows 0 and 2 into rows 1 and 3,
+// row_bcast:31 adds lane 31 into rows 2 and 3, so lane 63 holds the total.
+// Data moves in the VALU, without the LDS crossbar of lane shuffles.
 __device__ __forceinline__ double wave_sum(double v)
 {
-    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
-    return v;
+    v += dpp_move<0x111, 0xf>(v);  // row_shr:1
+    v += dpp_move<0x112, 0xf>(v);  // row_shr:2
+    v += dpp_move<0x114, 0xf>(v);  // row_shr:4
+    v += dpp_move<0x118, 0xf>(v);  // row_shr:8
+    v += dpp_move<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v += dpp_move<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), kWave - 1);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), kWave - 1);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
 // sum over a 256-thread block; result valid in thread 0
@@ -460,6 +484,13 @@ __global__ __launch_bounds__(kBlock) void k_to_float(size_t len, const double *_
 {
     const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
     if (i < len) dst[i] = (float)src[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_to_double(size_t len, const float *__restrict__ src,
+                                                     double *__restrict__ dst)
+{
+    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < len) dst[i] = (double)src[i];
 }
 
 // ---- Gram -----------------------------------------------------------------------
@@ -941,11 +972,11 @@ __global__ __launch_bounds__(kBlock) void k_residual(int n, int rows_per_block,
 // p = r (first) or p = r + beta p, column lists.  float64 P: unscaled.
 // float32 P holds p / s with a per-column scale: p_new = (r + beta s_old p) /
 // s_new, s_new chosen by the host so that |p_new| <= 1 (no overflow when
-// beta is large).
-template <typename PT>
+// beta is large).  R is float64, or float32 with the float32 directions.
+template <typename PT, typename RT>
 __global__ __launch_bounds__(kBlock) void k_bcg_p(int n, PT *__restrict__ P, long ldp,
                                                  const int *__restrict__ pmap,
-                                                 const double *__restrict__ R, long ldr,
+                                                 const RT *__restrict__ R, long ldr,
                                                  const int *__restrict__ rmap,
                                                  const double *__restrict__ beta,
                                                  const double *__restrict__ s_old,
@@ -956,7 +987,7 @@ __global__ __launch_bounds__(kBlock) void k_bcg_p(int n, PT *__restrict__ P, lon
     const int c = blockIdx.y;
     if (row >= n || (count && c >= *count)) return;
     PT *p = P + (long)pmap[c] * ldp + row;
-    const double r = R[(long)rmap[c] * ldr + row];
+    const double r = (double)R[(long)rmap[c] * ldr + row];
     if (!s_new) {
         *p = (PT)(first ? r : r + beta[c] * (double)(*p));
     } else {
@@ -996,12 +1027,13 @@ __global__ __launch_bounds__(kBlock) void k_coldots_pw(int n, int rows_per_block
 
 // x += alpha p; r -= alpha w; partial r.r, with alpha = rho2[c] / ptw[c]
 // (float64 P), or alpha = rho2[c] / (scale[c] ptw[c]) for the scaled float32
-// directions (p = scale p_stored, ptw the stored p . A p_stored); x, r float64
-template <typename PT>
+// directions (p = scale p_stored, ptw the stored p . A p_stored); x float64,
+// r float64 or float32 (with float32 directions); arithmetic and r.r in float64
+template <typename PT, typename RT>
 __global__ __launch_bounds__(kBlock) void k_bcg_xr(int n, int rows_per_block,
                                                   double *__restrict__ X, long ldx,
                                                   const int *__restrict__ xmap,
-                                                  double *__restrict__ R, long ldr,
+                                                  RT *__restrict__ R, long ldr,
                                                   const int *__restrict__ rmap,
                                                   const PT *__restrict__ P, long ldp,
                                                   const int *__restrict__ pmap,
@@ -1018,16 +1050,16 @@ __global__ __launch_bounds__(kBlock) void k_bcg_xr(int n, int rows_per_block,
     if (count && c >= *count) return;
     const double alpha = scale ? rho2[c] / (scale[c] * ptw[c]) : rho2[c] / ptw[c];
     double *x = X + (long)xmap[c] * ldx;
-    double *r = R + (long)rmap[c] * ldr;
+    RT *r = R + (long)rmap[c] * ldr;
     const PT *p = P + (long)pmap[c] * ldp;
     const PT *w = W + (long)wmap[c] * ldw;
     const int r0 = bx * rows_per_block, r1 = min(n, r0 + rows_per_block);
     double s = 0.0;
     for (int i = r0 + threadIdx.x; i < r1; i += kBlock) {
         x[i] = alpha * (double)p[i] + x[i];
-        const double rn = -alpha * (double)w[i] + r[i];
+        const RT rn = (RT)(-alpha * (double)w[i] + (double)r[i]);
         r[i] = rn;
-        s += rn * rn;
+        s += (double)rn * (double)rn;
     }
     s = block_sum(s, lds);
     if (threadIdx.x == 0) partial[(long)c * gridDim.x + bx] = s;
@@ -1282,6 +1314,14 @@ void to_float(const double *src, float *dst, size_t len)
 {
     if (!len) return;
     hipLaunchKernelGGL(k_to_float, dim3(ceil_div((long)len, kBlock)), dim3(kBlock), 0,
+                       state().stream, len, src, dst);
+    HIP_CHECK(hipGetLastError());
+}
+
+void to_double(const float *src, double *dst, size_t len)
+{
+    if (!len) return;
+    hipLaunchKernelGGL(k_to_double, dim3(ceil_div((long)len, kBlock)), dim3(kBlock), 0,
                        state().stream, len, src, dst);
     HIP_CHECK(hipGetLastError());
 }
@@ -1555,14 +1595,14 @@ void residual_sq(int n, const double *AX, long ldax, const double *BX, long ldbx
     HIP_CHECK(hipGetLastError());
 }
 
-template <typename PT>
-static void bcg_p_t(int n, PT *P, long ldp, const int *d_pmap, const double *R, long ldr,
+template <typename PT, typename RT>
+static void bcg_p_t(int n, PT *P, long ldp, const int *d_pmap, const RT *R, long ldr,
                     const int *d_rmap, const double *d_beta, const double *d_sold,
                     const double *d_snew, int first, int k, const int *d_count)
 {
     if (k <= 0) return;
     ProfScope prof("bcg_p");
-    hipLaunchKernelGGL((k_bcg_p<PT>), dim3(ceil_div(n, kBlock), k), dim3(kBlock), 0,
+    hipLaunchKernelGGL((k_bcg_p<PT, RT>), dim3(ceil_div(n, kBlock), k), dim3(kBlock), 0,
                        state().stream, n, P, ldp, d_pmap, R, ldr, d_rmap, d_beta, d_sold, d_snew,
                        first, d_count);
     HIP_CHECK(hipGetLastError());
@@ -1575,7 +1615,7 @@ void bcg_p_update(int n, double *P, long ldp, const int *d_pmap, const double *R
             (const double *)nullptr, first, k, d_count);
 }
 
-void bcg_p_update(int n, float *P, long ldp, const int *d_pmap, const double *R, long ldr,
+void bcg_p_update(int n, float *P, long ldp, const int *d_pmap, const float *R, long ldr,
                   const int *d_rmap, const double *d_beta, const double *d_sold,
                   const double *d_snew, int first, int k, const int *d_count)
 {
@@ -1597,8 +1637,8 @@ void coldots_pw(int n, const float *P, long ldp, const int *d_pmap, const float 
     HIP_CHECK(hipGetLastError());
 }
 
-template <typename PT>
-static void bcg_xr_t(int n, double *X, long ldx, const int *d_xmap, double *R, long ldr,
+template <typename PT, typename RT>
+static void bcg_xr_t(int n, double *X, long ldx, const int *d_xmap, RT *R, long ldr,
                      const int *d_rmap, const PT *P, long ldp, const int *d_pmap, const PT *W,
                      long ldw, const int *d_wmap, const double *d_rho2, const double *d_ptw,
                      const double *d_scale, int k, double *d_rr_out, const int *d_count)
@@ -1608,7 +1648,7 @@ static void bcg_xr_t(int n, double *X, long ldx, const int *d_xmap, double *R, l
     const int nbx = std::max(1, ceil_div(n, rows));
     double *partial = partial_buffer((size_t)nbx * k);
     ProfScope prof("bcg_xr");
-    hipLaunchKernelGGL((k_bcg_xr<PT>), dim3(nbx, k), dim3(kBlock), 0, state().stream, n, rows, X,
+    hipLaunchKernelGGL((k_bcg_xr<PT, RT>), dim3(nbx, k), dim3(kBlock), 0, state().stream, n, rows, X,
                        ldx, d_xmap, R, ldr, d_rmap, P, ldp, d_pmap, W, ldw, d_wmap, d_rho2, d_ptw,
                        d_scale, partial, d_count);
     hipLaunchKernelGGL(k_reduce_rows, dim3(k), dim3(kBlock), 0, state().stream, partial, nbx,
@@ -1625,7 +1665,7 @@ void bcg_xr_update(int n, double *X, long ldx, const int *d_xmap, double *R, lon
              (const double *)nullptr, k, d_rr_out, d_count);
 }
 
-void bcg_xr_update(int n, double *X, long ldx, const int *d_xmap, double *R, long ldr,
+void bcg_xr_update(int n, double *X, long ldx, const int *d_xmap, float *R, long ldr,
                    const int *d_rmap, const float *P, long ldp, const int *d_pmap, const float *W,
                    long ldw, const int *d_wmap, const double *d_rho2, const double *d_ptw,
                    const double *d_scale, int k, double *d_rr_out, const int *d_count)
