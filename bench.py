@@ -59,21 +59,27 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--workload", choices=("q1-64", "q1-128-lognormal", "q1-64-adjoint"),
+    p.add_argument("--workload", choices=("q1-64", "q1-128-lognormal", "q1-64-adjoint",
+                                          "q1-192-permuted"),
                    default="q1-64",
                    help="q1-64: Q1 64^3, kappa = 1, nev 50 (default, float64 value); "
                         "q1-128-lognormal: Q1 128^3, lognormal kappa, nev 100, block 200 "
                         "requested (the mixed-precision comparison on one seed); "
                         "q1-64-adjoint: Q1 64^3 lognormal kappa, nev 20, gradient of "
                         "sum w_k lambda_k + ||P x_1||^2 with respect to all kappa_e, "
-                        "checked by fourth-order central differences on --fd elements")
+                        "checked by fourth-order central differences on --fd elements; "
+                        "q1-192-permuted: Q1 192^3 lognormal kappa, rows and columns "
+                        "symmetrically permuted by a seeded random permutation, nev 200, block "
+                        "400 requested, a bounded number of GCG iterations (--max-it, default "
+                        "2 here) in float64 and mixed precision, then the eigenvalue adjoint on "
+                        "all elements")
     p.add_argument("--fd", type=int, default=8, help="elements checked by finite differences "
                                                      "(q1-64-adjoint)")
     p.add_argument("--mesh", type=int, default=None, help="elements per axis (overrides the workload)")
     p.add_argument("--nev", type=int, default=None)
     p.add_argument("--kappa-seed", type=int, default=1)
     p.add_argument("--tol", type=float, default=1e-8)
-    p.add_argument("--max-it", type=int, default=500,
+    p.add_argument("--max-it", type=int, default=None,
                    help="GCG iteration cap (a bounded run for profiling; solves stopped by it "
                         "report all_converged false)")
     p.add_argument("--cpu-iters", type=int, default=3,
@@ -356,6 +362,94 @@ def adjoint_workload(args):
     print(json.dumps(line), flush=True)
 
 
+def permuted_workload(args):
+    """Scale and irregular access: the lognormal Q1 pencil on the 192^3 mesh
+    (n = 191^3), rows and columns permuted by a seeded random permutation, so
+    every product gathers X at random rows.  GCG iterations are bounded
+    (the per-iteration cost is the measured quantity; a full solve takes
+    hundreds of iterations at this size): seconds per iteration in float64
+    and in the mixed form, the products' rate, then dL/dkappa_e of
+    L = sum_c w_c lambda_c for all E^3 elements on the device through the
+    node -> row map, checked by the identity sum_e kappa_e g_e =
+    sum_c w_c x_c^T K x_c (exact for any vectors, K linear in kappa)."""
+    import ctypes as C
+
+    import gcge_old_amd as gcge
+    from gcge_old_amd import _native, adjoint, problems
+
+    lib = _native.hip()
+    E = args.mesh or 192
+    nev = args.nev or 200
+    iters = args.max_it or 2
+    t0 = time.time()
+    kappa = problems.lognormal_kappa(E, args.kappa_seed)
+    K, M = problems.q1_fem_3d(E, kappa)
+    n = K.nrows
+    perm = np.random.Generator(np.random.PCG64(args.kappa_seed + 1000)).permutation(n)
+    Kp = problems.symmetric_permutation(K, perm, sort_rows=False)
+    Mp = problems.symmetric_permutation(M, perm, sort_rows=False)
+    del K, M
+    t_prep = time.time() - t0
+    dK, dM = gcge.DeviceMatrix(Kp), gcge.DeviceMatrix(Mp)
+    print(f"# workload: Q1 {E}^3 lognormal kappa, randomly permuted, n={n} nnz(K)={Kp.nnz} "
+          f"nnz(M)={Mp.nnz} nev={nev} (assembly + permutation {t_prep:.1f}s, storage "
+          f"{dK.storage_info()})", file=sys.stderr, flush=True)
+    flags = ["-gcge_conv_type", "RB", "-gcge_ev_tol", str(args.tol), "-gcge_ev_max_it", str(iters),
+             "-gcge_block_size", str(2 * nev), "-gcge_full_rr", "1", "-gcge_w_orth_type", "svqb"]
+    legs = {}
+    out = None
+    for prec in ("fp64", "mixed"):
+        lib.GCGE_HIP_ProfileReset()
+        lib.GCGE_HIP_ProfileEnable(2)
+        out = gcge.solve(dK, dM, nev, flags, seed=1, precision=prec)
+        lib.GCGE_HIP_ProfileEnable(0)
+        k = kernel_stats(lib)
+        legs[prec] = {"s_per_iteration": round(out["time"] / max(out["num_iter"], 1), 4),
+                      "iterations": out["num_iter"], "block_size": out["block_size"]}
+        for name in ("spmm", "spmm32"):  # float64 products; float32 inner-CG products
+            if name in k and k[name]["ms"] > 0:
+                sp = k[name]
+                legs[prec][name + "_gbs"] = round(sp["bytes"] / (sp["ms"] * 1e-3) / 1e9, 1)
+                legs[prec][name + "_avg_launch_ms"] = round(sp["ms"] / sp["launches"], 3)
+        print(f"# {prec}: {out['time']:.2f}s for {out['num_iter']} iterations, {legs[prec]}",
+              file=sys.stderr, flush=True)
+        if prec == "fp64":
+            X = out["evec"]
+    dK.set_precision(False)
+    dM.set_precision(False)
+    # eigenvalue adjoint over all elements on the (float64 leg's) vectors
+    w = np.random.default_rng(args.kappa_seed + 1).uniform(size=nev)
+    Xd = gcge.DeviceMultiVec.from_host(X)
+    del X
+    inv = np.empty(n, dtype=np.int64)
+    inv[perm] = np.arange(n)
+    lib.GCGE_HIP_Synchronize()
+    t = time.perf_counter()
+    g = adjoint.q1_kappa_gradient(E, Xd, w, node_row=inv)
+    t_adj = time.perf_counter() - t
+    Yd = gcge.DeviceMultiVec(n, nev)
+    lib.GCGE_HIP_SpMM(dK.handle, Xd.handle, 0, Yd.handle, 0, nev)
+    G = np.zeros((nev, nev), order="F")
+    lib.GCGE_HIP_Gram(Xd.handle, 0, nev, Yd.handle, 0, nev, _native.as_dptr(G), nev)
+    xkx = float(w @ np.diag(G))
+    ident = abs(float(kappa @ g) - xkx) / abs(xkx)
+    fp64, mixed = legs["fp64"]["s_per_iteration"], legs["mixed"]["s_per_iteration"]
+    line = {"metric": "s per GCG iteration (bounded run) on the permuted 192^3 pencil: FP64 vs FP32-inner",
+            "value": fp64, "unit": "s", "n_gpus": 1, "steps": iters, "warmup": 0,
+            "ms_per_step": round(fp64 * 1e3, 2), "higher_is_better": False, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"Q1 FEM {E}^3 lognormal kappa (seed {args.kappa_seed}), rows and "
+                                   "columns randomly permuted, nev=200, block 400 requested",
+                       "n": n, "nnz_K": Kp.nnz, "nnz_M": Mp.nnz, "nev": nev,
+                       "rayleigh_ritz": "full (-gcge_full_rr 1)", "w_orth": "svqb"},
+            "fp64": legs["fp64"], "mixed": legs["mixed"],
+            "mixed_speedup_per_iteration": round(fp64 / mixed, 3),
+            "eigenvalue_adjoint_all_elements_ms": round(t_adj * 1e3, 2),
+            "adjoint_identity_rel_err": ident,
+            "adjoint_identity_sides": [float(kappa @ g), xkx], "elements": E ** 3}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -370,6 +464,11 @@ def main():
     if args.workload == "q1-64-adjoint":
         adjoint_workload(args)
         return
+    if args.workload == "q1-192-permuted":
+        permuted_workload(args)
+        return
+    if args.max_it is None:
+        args.max_it = 500
     lognormal = args.workload == "q1-128-lognormal"
     E = args.mesh or (128 if lognormal else 64)
     if args.nev is None:

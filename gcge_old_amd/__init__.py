@@ -60,16 +60,25 @@ class DeviceMatrix:
         _native.hip().GCGE_HIP_MatSetPrecision(self.handle, 1 if fp32 else 0)
         self.fp32 = bool(fp32)
 
-    def set_storage(self, diagonal: bool) -> bool:
-        """Products through diagonal storage (when the matrix has it) or the
-        sliced-ELLPACK form; returns whether diagonal storage is now used."""
-        return bool(_native.hip().GCGE_HIP_MatSetStorage(self.handle, 1 if diagonal else 0))
+    STORAGES = ("sliced", "diagonal", "rows")
+
+    def set_storage(self, storage) -> bool:
+        """Products through "diagonal" storage (when the matrix has it),
+        "sliced" ELLPACK with element gathers, or sliced ELLPACK with "rows"
+        gathers from a row-major copy of the column block (from 16 columns);
+        True/False mean "diagonal"/"sliced".  Returns whether diagonal storage
+        is now used."""
+        if storage is True or storage is False:
+            storage = "diagonal" if storage else "sliced"
+        used = _native.hip().GCGE_HIP_MatSetStorage(self.handle, self.STORAGES.index(storage))
+        return used == 1
 
     def storage_info(self) -> dict:
         nd, ng = np.zeros(1, dtype=np.int32), np.zeros(1, dtype=np.int32)
         used = _native.hip().GCGE_HIP_MatStorageInfo(self.handle, _native.as_iptr(nd),
                                                      _native.as_iptr(ng))
-        return {"diagonal": bool(used), "ndiag": int(nd[0]), "runs": int(ng[0])}
+        return {"diagonal": used == 1, "storage": self.STORAGES[used], "ndiag": int(nd[0]),
+                "runs": int(ng[0])}
 
     def __del__(self):
         if getattr(self, "handle", None):

@@ -95,6 +95,8 @@ def _declare(name: str, lib: C.CDLL) -> None:
                                                  C.c_int, C.c_int, dptr, dptr]
         lib.GCGE_HIP_Q1ElementGradient.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_void_p,
                                                    C.c_int, C.c_int, dptr, dptr]
+        lib.GCGE_HIP_Q1ElementGradientRows.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_void_p,
+                                                       C.c_int, C.c_int, dptr, iptr, dptr]
         lib.GCGE_HIP_Synchronize.argtypes = []
         lib.GCGE_HIP_KernelTime.argtypes = [C.c_char_p, dptr, iptr]
         lib.GCGE_HIP_KernelTime.restype = C.c_int

@@ -52,16 +52,25 @@ def pattern_gradients(A: DeviceMatrix, B: DeviceMatrix | None, X, lam, gbar):
     return gA, gB
 
 
-def q1_kappa_gradient(E: int, X, gbar, Y=None) -> np.ndarray:
+def q1_kappa_gradient(E: int, X, gbar, Y=None, node_row=None) -> np.ndarray:
     """sum_c gbar_c x_c|e^T (h Khat) y_c|e for every element of the E^3 mesh
-    (Y = X when None): dL/dkappa of an eigenvalue loss."""
+    (Y = X when None): dL/dkappa of an eigenvalue loss.  node_row: for blocks
+    of a symmetrically permuted pencil, the row that holds each node (the
+    inverse of the permutation of problems.symmetric_permutation)."""
     Xd = _as_dev_mv(X)
     Yd = None if Y is None else _as_dev_mv(Y)
     k = len(gbar)
     w = np.ascontiguousarray(gbar, dtype=np.float64)
     g = np.zeros(E ** 3)
-    _native.hip().GCGE_HIP_Q1ElementGradient(E, Xd.handle, 0, None if Yd is None else Yd.handle, 0,
-                                             k, _native.as_dptr(w), _native.as_dptr(g))
+    lib = _native.hip()
+    if node_row is None:
+        lib.GCGE_HIP_Q1ElementGradient(E, Xd.handle, 0, None if Yd is None else Yd.handle, 0,
+                                       k, _native.as_dptr(w), _native.as_dptr(g))
+    else:
+        rows = np.ascontiguousarray(node_row, dtype=np.int32)
+        lib.GCGE_HIP_Q1ElementGradientRows(E, Xd.handle, 0, None if Yd is None else Yd.handle, 0,
+                                           k, _native.as_dptr(w), _native.as_iptr(rows),
+                                           _native.as_dptr(g))
     return g
 
 

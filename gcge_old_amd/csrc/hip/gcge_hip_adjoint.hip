@@ -64,6 +64,7 @@ __global__ __launch_bounds__(kBlock) void k_q1_elem_grad(int E, double scale,
                                                         const double *__restrict__ X, long ldx,
                                                         const double *__restrict__ Y, long ldy,
                                                         int k, const double *__restrict__ w,
+                                                        const int *__restrict__ node_row,
                                                         double *__restrict__ grad)
 {
     const long e = (long)blockIdx.x * kBlock + threadIdx.x;
@@ -77,6 +78,7 @@ __global__ __launch_bounds__(kBlock) void k_q1_elem_grad(int E, double scale,
         const int ix = ex + ((p >> 2) & 1), iy = ey + ((p >> 1) & 1), iz = ez + (p & 1);
         const bool interior = ix >= 1 && ix <= N && iy >= 1 && iy <= N && iz >= 1 && iz <= N;
         node[p] = interior ? (long)(ix - 1) + (long)N * (iy - 1) + (long)N * N * (iz - 1) : -1;
+        if (node_row && node[p] >= 0) node[p] = node_row[node[p]];  // row of a permuted block
     }
     double total = 0.0;
     for (int c = 0; c < k; ++c) {
@@ -115,7 +117,7 @@ void pattern_gradient(const Mat *A, const double *X, long ldx, const double *Y, 
 }
 
 void q1_element_gradient(int E, const double *X, long ldx, const double *Y, long ldy, int k,
-                         const double *d_w, double *d_grad)
+                         const double *d_w, double *d_grad, const int *d_node_row)
 {
     if (E < 2 || k <= 0) return;
     static bool uploaded = false;
@@ -139,7 +141,7 @@ void q1_element_gradient(int E, const double *X, long ldx, const double *Y, long
     const double h = 1.0 / E;
     ProfScope prof("q1_elem_grad", 16.0 * k * (double)(E - 1) * (E - 1) * (E - 1) + 8.0 * ne);
     hipLaunchKernelGGL(k_q1_elem_grad, dim3((unsigned)((ne + kBlock - 1) / kBlock)), dim3(kBlock), 0,
-                       state().stream, E, h / 36.0, X, ldx, Y, ldy, k, d_w, d_grad);
+                       state().stream, E, h / 36.0, X, ldx, Y, ldy, k, d_w, d_node_row, d_grad);
     HIP_CHECK(hipGetLastError());
 }
 

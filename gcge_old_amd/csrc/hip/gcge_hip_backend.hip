@@ -116,6 +116,13 @@ double *partial_buffer(size_t d) { return grow_device(state().d_partial, state()
 double *small_buffer(size_t d) { return grow_device(state().d_small, state().small_cap, d); }
 double *small_buffer2(size_t d) { return grow_device(state().d_small2, state().small2_cap, d); }
 int *index_buffer(size_t d) { return grow_device(state().d_index, state().index_cap, d); }
+void *rowblock_buffer(size_t bytes)
+{
+    char *p = (char *)state().d_rowblk;
+    p = grow_device(p, state().rowblk_cap, bytes);
+    state().d_rowblk = p;
+    return p;
+}
 
 double *result_buffer(size_t d)
 {
@@ -342,6 +349,26 @@ Mat *create_mat(int nrows, int ncols, int nnz, const int *rowptr, const int *col
     HIP_CHECK(hipMemcpyAsync(m->d_col, scol.data(), scol.size() * sizeof(int), hipMemcpyHostToDevice, st.stream));
     HIP_CHECK(hipMemcpyAsync(m->d_val, sval.data(), sval.size() * sizeof(double), hipMemcpyHostToDevice, st.stream));
     HIP_CHECK(hipStreamSynchronize(st.stream));
+    {
+        // column spread of each entry row of a slice, weighted by its stored
+        // entries: scattered when most entries sit in rows spanning >= 256
+        long tight = 0, total = 0;
+        for (int sl = 0; sl < ns; ++sl)
+            for (int j = 0; j < sw[sl]; ++j) {
+                int lo = INT32_MAX, hi = -1, cnt = 0;
+                for (int lane = 0; lane < 64; ++lane) {
+                    const size_t pos = (size_t)sptr[sl] + (size_t)j * 64 + lane;
+                    if (spos[pos] < 0) continue;
+                    lo = std::min(lo, scol[pos]);
+                    hi = std::max(hi, scol[pos]);
+                    ++cnt;
+                }
+                total += cnt;
+                if (cnt && hi - lo < 256) tight += cnt;
+            }
+        m->irregular = (total > 0 && 2 * tight < total) ? 1 : 0;
+        m->use_rows = m->irregular;
+    }
     build_dia(m, rowptr, colidx, vals);
     MatHost *mh = new MatHost;
     mh->nz_count.resize(nrows);
@@ -1314,6 +1341,7 @@ void GCGE_HIP_Finalize(void)
     if (s.d_small2) HIP_CHECK(hipFree(s.d_small2));
     if (s.d_index) HIP_CHECK(hipFree(s.d_index));
     if (s.d_bcg) HIP_CHECK(hipFree(s.d_bcg));
+    if (s.d_rowblk) HIP_CHECK(hipFree(s.d_rowblk));
     if (s.d_bcgi) HIP_CHECK(hipFree(s.d_bcgi));
     if (s.h_result) HIP_CHECK(hipHostFree(s.h_result));
     for (int i = 0; i < State::kRing; ++i) {
@@ -1391,12 +1419,13 @@ void GCGE_HIP_MatSetPrecision(void *mat, int fp32)
     m->fp32 = fp32 ? 1 : 0;
 }
 
-int GCGE_HIP_MatSetStorage(void *mat, int diagonal)
+int GCGE_HIP_MatSetStorage(void *mat, int storage)
 {
     Mat *m = as_mat(mat);
     HIP_CHECK(hipStreamSynchronize(state().stream));
-    m->use_dia = (diagonal && m->d_dia) ? 1 : 0;
-    return m->use_dia;
+    m->use_dia = (storage == 1 && m->d_dia) ? 1 : 0;
+    m->use_rows = storage == 2 ? 1 : 0;
+    return m->use_dia ? 1 : (m->use_rows ? 2 : 0);
 }
 
 int GCGE_HIP_MatStorageInfo(void *mat, int *ndiag, int *ngroups)
@@ -1404,7 +1433,7 @@ int GCGE_HIP_MatStorageInfo(void *mat, int *ndiag, int *ngroups)
     Mat *m = as_mat(mat);
     if (ndiag) *ndiag = m->d_dia ? m->ndiag : 0;
     if (ngroups) *ngroups = m->d_dia ? m->dia.ngroups : 0;
-    return m->use_dia;
+    return m->use_dia ? 1 : (m->use_rows ? 2 : 0);
 }
 
 // ---- adjoint entry points ----------------------------------------------------------------
@@ -1443,8 +1472,8 @@ void GCGE_HIP_PatternGradient(void *mat, void **X, int xs, void **Y, int ys, int
     HIP_CHECK(hipStreamSynchronize(state().stream));
 }
 
-void GCGE_HIP_Q1ElementGradient(int E, void **X, int xs, void **Y, int ys, int k, const double *w,
-                                double *grad)
+void GCGE_HIP_Q1ElementGradientRows(int E, void **X, int xs, void **Y, int ys, int k,
+                                    const double *w, const int *node_row, double *grad)
 {
     require_init();
     MultiVec *Xm = as_mv(X), *Ym = Y ? as_mv(Y) : Xm;
@@ -1455,13 +1484,28 @@ void GCGE_HIP_Q1ElementGradient(int E, void **X, int xs, void **Y, int ys, int k
     const long N = E - 1;
     if ((long)Xm->n != N * N * N || Ym->n != Xm->n)
         fail_msg("GCGE_HIP_Q1ElementGradient: the blocks must have (E-1)^3 rows");
+    int *d_rows = nullptr;
+    if (node_row) {
+        const long n = N * N * N;
+        for (long i = 0; i < n; ++i)
+            if (node_row[i] < 0 || node_row[i] >= n)
+                fail_msg("GCGE_HIP_Q1ElementGradientRows: node_row entry out of range");
+        d_rows = index_buffer((size_t)n);
+        upload_async(d_rows, node_row, (size_t)n * sizeof(int));
+    }
     const size_t ne = (size_t)E * E * E;
     double *d_w = small_buffer((size_t)k);
     upload_async(d_w, w, (size_t)k * sizeof(double));
     double *d_grad = adjoint_buffer(ne);
-    q1_element_gradient(E, col_ptr(Xm, xs), Xm->ld, col_ptr(Ym, ys), Ym->ld, k, d_w, d_grad);
+    q1_element_gradient(E, col_ptr(Xm, xs), Xm->ld, col_ptr(Ym, ys), Ym->ld, k, d_w, d_grad, d_rows);
     HIP_CHECK(hipMemcpyAsync(grad, d_grad, ne * sizeof(double), hipMemcpyDeviceToHost, state().stream));
     HIP_CHECK(hipStreamSynchronize(state().stream));
+}
+
+void GCGE_HIP_Q1ElementGradient(int E, void **X, int xs, void **Y, int ys, int k, const double *w,
+                                double *grad)
+{
+    GCGE_HIP_Q1ElementGradientRows(E, X, xs, Y, ys, k, w, nullptr, grad);
 }
 
 void **GCGE_HIP_MultiVecCreate(int n, int ncols) { return (void **)create_mv(n, ncols); }

@@ -30,6 +30,8 @@ struct State {
     size_t small2_cap = 0;
     int *d_index = nullptr;       // column index lists
     size_t index_cap = 0;         // ints
+    void *d_rowblk = nullptr;     // row-major copy of a column block (irregular products)
+    size_t rowblk_cap = 0;        // bytes
     double *d_bcg = nullptr;      // block-CG bookkeeping state
     size_t bcg_cap = 0;           // doubles
     int *d_bcgi = nullptr;
@@ -54,6 +56,7 @@ double *partial_buffer(size_t doubles);
 double *small_buffer(size_t doubles);
 double *small_buffer2(size_t doubles);
 int *index_buffer(size_t ints);
+void *rowblock_buffer(size_t bytes);
 double *result_buffer(size_t doubles);
 // copy host data into a device buffer through the pinned ring (async, safe
 // against reuse of the staging slot)
@@ -108,6 +111,12 @@ struct Mat {
     double *d_dia;
     float *d_dia32;
     int use_dia;
+    // scattered columns (fewer than half of the stored entries sit in
+    // sliced-ELLPACK entry rows spanning under 256 columns): wide products
+    // gather whole rows of a row-major copy of X instead of single elements
+    // (use_rows)
+    int irregular;
+    int use_rows;
 };
 constexpr uint32_t kMatMagic = 0x4d415431u;  // "MAT1"
 
@@ -229,8 +238,9 @@ void axpy_cols(int n, const double *d_alpha, const float *X, long ldx, const int
 // entry (r, q) at CSR position p of A (d_w: k weights on the device)
 void pattern_gradient(const Mat *A, const double *X, long ldx, const double *Y, long ldy, int k,
                       const double *d_w, double *d_grad);
-// Q1 elements of the E^3 mesh: d_grad[e] = (h / 36) sum_c w[c] x_c|e^T K36 y_c|e
+// Q1 elements of the E^3 mesh: d_grad[e] = (h / 36) sum_c w[c] x_c|e^T K36 y_c|e;
+// d_node_row (optional): the row of X and Y that holds each interior node
 void q1_element_gradient(int E, const double *X, long ldx, const double *Y, long ldy, int k,
-                         const double *d_w, double *d_grad);
+                         const double *d_w, double *d_grad, const int *d_node_row = nullptr);
 
 }  // namespace gcge_hip

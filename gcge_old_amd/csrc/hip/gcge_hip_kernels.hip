@@ -479,6 +479,145 @@ void launch_dia_spmm(const Mat *A, const VT *vals, const XT *X, long ldx, XT *Y,
     HIP_CHECK(hipGetLastError());
 }
 
+// ---- SpMM with whole-row gathers (matrices with scattered columns) ------------------
+// The column block X (column-major, optional column map) is first copied to a
+// row-major block XT (row i: the k values of row i, row stride kp), through
+// 64 x 64 LDS tiles so that both sides are coalesced.  The product then runs
+// with lanes along the columns: a wavefront owns 16 rows of a sliced-ELLPACK
+// slice and 64 columns (grid y: column passes); for every entry of those rows
+// the column index and value are wave-uniform, and the gather of the matching
+// row of XT is one contiguous access of 64 values, where the column-major form
+// takes a separate cache line per element.  Each lane writes its 16 outputs
+// of a column as one contiguous run.  DOT: p.w (and p.p) per column, summed
+// over the workgroup's rows through LDS, one partial per workgroup.
+constexpr int kRowsPerWave = 16;
+
+// (only the first min(k, *count) columns: entries of a device column list
+// beyond its length are not defined)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cols_to_rows(int n, int k, const T *__restrict__ X, long ldx,
+                                                        const int *__restrict__ xmap,
+                                                        T *__restrict__ XT, int kp,
+                                                        const int *__restrict__ count)
+{
+    __shared__ T tile[64][65];
+    const int kc = count ? min(k, *count) : k;
+    const int r0 = blockIdx.x * 64, c0 = blockIdx.y * 64;
+    if (c0 >= kc) return;  // the whole workgroup
+    const int t = threadIdx.x % 64, g = threadIdx.x / 64;
+    for (int cc = g; cc < 64; cc += kBlock / 64) {
+        const int c = c0 + cc, r = r0 + t;
+        tile[cc][t] = (c < kc && r < n) ? X[(long)(xmap ? xmap[c] : c) * ldx + r] : T(0);
+    }
+    __syncthreads();
+    for (int rr = g; rr < 64; rr += kBlock / 64) {
+        const int r = r0 + rr, c = c0 + t;
+        if (r < n && c < kc) XT[(long)r * kp + c] = tile[t][rr];
+    }
+}
+
+template <typename VT, typename XT, int DOT>
+__global__ __launch_bounds__(kBlock) void k_sell_rows_spmm(int n, int nslices, int slices_per_block,
+                                                          const int *__restrict__ sptr,
+                                                          const int *__restrict__ sw,
+                                                          const int *__restrict__ scol,
+                                                          const VT *__restrict__ sval,
+                                                          const XT *__restrict__ XTm, int kp,
+                                                          XT *__restrict__ Y, long ldy, int k,
+                                                          const int *__restrict__ ymap,
+                                                          double *__restrict__ dpart, int nparts,
+                                                          const int *__restrict__ count)
+{
+    __shared__ double red[DOT > 0 ? 2 * kBlock : 1];
+    const int wid = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int kc = count ? min(k, *count) : k;
+    const int c0 = blockIdx.y * kWave;
+    if (c0 >= kc) return;  // the whole workgroup: same column pass
+    const int c = c0 + lane;
+    const bool live = c < kc;
+    const int s0 = blockIdx.x * slices_per_block, s1 = min(nslices, s0 + slices_per_block);
+    double pw = 0.0, pp = 0.0;
+    for (int sl = s0; sl < s1; ++sl) {
+        const int w = sw[sl];
+        const int base = sptr[sl] + wid * kRowsPerWave;
+        XT acc[kRowsPerWave];
+#pragma unroll
+        for (int i = 0; i < kRowsPerWave; ++i) acc[i] = XT(0);
+        for (int j = 0; j < w; ++j) {
+            const int e0 = base + j * kWave;
+            XT xv[kRowsPerWave];
+#pragma unroll
+            for (int i = 0; i < kRowsPerWave; ++i) {
+                const int col = scol[e0 + i];
+                xv[i] = live ? XTm[(long)col * kp + c] : XT(0);
+            }
+#pragma unroll
+            for (int i = 0; i < kRowsPerWave; ++i) acc[i] += (XT)sval[e0 + i] * xv[i];
+        }
+        const int rbase = sl * kWave + wid * kRowsPerWave;
+        if (live) {
+            XT *y = Y + (long)(ymap ? ymap[c] : c) * ldy;
+#pragma unroll
+            for (int i = 0; i < kRowsPerWave; ++i)
+                if (rbase + i < n) y[rbase + i] = acc[i];
+        }
+        if constexpr (DOT > 0) {
+#pragma unroll
+            for (int i = 0; i < kRowsPerWave; ++i) {
+                const int r = rbase + i;
+                const double pv = (live && r < n) ? (double)XTm[(long)r * kp + c] : 0.0;
+                pw += pv * (double)acc[i];
+                if constexpr (DOT > 1) pp += pv * pv;
+            }
+        }
+    }
+    if constexpr (DOT > 0) {
+        red[threadIdx.x] = pw;
+        red[kBlock + threadIdx.x] = pp;
+        __syncthreads();
+        if (wid == 0 && live) {
+            double a = 0.0, b = 0.0;
+            for (int q = 0; q < kBlock / kWave; ++q) {
+                a += red[q * kWave + lane];
+                b += red[kBlock + q * kWave + lane];
+            }
+            dpart[(long)c * nparts + blockIdx.x] = a;
+            if constexpr (DOT > 1) dpart[(long)(k + c) * nparts + blockIdx.x] = b;
+        }
+    }
+}
+
+// slices per workgroup of the row-gather product: about 2048 workgroups per
+// column pass
+inline int rows_slices_per_block(const Mat *A) { return std::max(1, (A->nslices + 2047) / 2048); }
+inline int rows_parts(const Mat *A)
+{
+    const int spb = rows_slices_per_block(A);
+    return (A->nslices + spb - 1) / spb;
+}
+
+template <int DOT, typename VT, typename XT>
+void launch_rows_spmm(const Mat *A, const VT *vals, const XT *X, long ldx, XT *Y, long ldy, int k,
+                      const int *d_xmap, const int *d_ymap, double *dpart, const int *d_count,
+                      hipEvent_t ev0, hipEvent_t ev1)
+{
+    const int n = A->nrows;
+    const int kp = (k + 7) / 8 * 8;
+    XT *XTm = (XT *)rowblock_buffer((size_t)A->ncols * kp * sizeof(XT));
+    hipLaunchKernelGGL((k_cols_to_rows<XT>), dim3((A->ncols + 63) / 64, (k + 63) / 64), dim3(kBlock), 0,
+                       state().stream, A->ncols, k, X, ldx, d_xmap, XTm, kp, d_count);
+    const int spb = rows_slices_per_block(A);
+    const int nparts = rows_parts(A);
+    hipExtLaunchKernelGGL((k_sell_rows_spmm<VT, XT, DOT>), dim3(nparts, (k + kWave - 1) / kWave),
+                          dim3(kBlock), 0, state().stream, ev0, ev1, 0, n, A->nslices, spb,
+                          A->d_slice_ptr, A->d_slice_w, A->d_col, vals, XTm, kp, Y, ldy, k, d_ymap,
+                          dpart, nparts, d_count);
+    HIP_CHECK(hipGetLastError());
+}
+
+// the row-gather form from this many columns (fewer leave most lanes idle)
+constexpr int kRowsMinCols = 16;
+
 __global__ __launch_bounds__(kBlock) void k_to_float(size_t len, const double *__restrict__ src,
                                                     float *__restrict__ dst)
 {
@@ -1233,6 +1372,9 @@ void spmm(const Mat *A, const double *X, long ldx, double *Y, long ldy, int k, c
     if (A->use_dia)
         launch_dia_spmm<0>(A, A->d_dia, X, ldx, Y, ldy, k, d_xmap, d_ymap, nullptr, nullptr,
                            prof.start_event(), prof.stop_event());
+    else if (A->use_rows && k >= kRowsMinCols)
+        launch_rows_spmm<0>(A, A->d_val, X, ldx, Y, ldy, k, d_xmap, d_ymap, nullptr, nullptr,
+                            prof.start_event(), prof.stop_event());
     else
         launch_sell_spmm<0>(A, A->d_val, X, ldx, Y, ldy, k, d_xmap, d_ymap, nullptr, nullptr,
                             prof.start_event(), prof.stop_event());
@@ -1249,6 +1391,9 @@ void spmm(const Mat *A, const float *X, long ldx, float *Y, long ldy, int k, con
     if (A->use_dia && A->d_dia32)
         launch_dia_spmm<0>(A, A->d_dia32, X, ldx, Y, ldy, k, d_xmap, d_ymap, nullptr, nullptr,
                            prof.start_event(), prof.stop_event());
+    else if (A->use_rows && k >= kRowsMinCols)
+        launch_rows_spmm<0>(A, A->d_val32, X, ldx, Y, ldy, k, d_xmap, d_ymap, nullptr, nullptr,
+                            prof.start_event(), prof.stop_event());
     else
         launch_sell_spmm<0>(A, A->d_val32, X, ldx, Y, ldy, k, d_xmap, d_ymap, nullptr, nullptr,
                             prof.start_event(), prof.stop_event());
@@ -1261,7 +1406,8 @@ void spmm_dot(const Mat *A, const double *P, long ldp, double *W, long ldw, int 
     if (k <= 0 || A->nrows <= 0) return;
     const double bytes = (double)A->nnz * (8 + 4) + 4.0 * (A->nrows + 1) +
                          8.0 * (double)k * (A->ncols + 2.0 * A->nrows);
-    const int nparts = A->use_dia ? dia_blocks(A) : A->nslices;
+    const bool rows = !A->use_dia && A->use_rows && k >= kRowsMinCols;
+    const int nparts = A->use_dia ? dia_blocks(A) : (rows ? rows_parts(A) : A->nslices);
     double *dpart = partial_buffer((size_t)nparts * k);
     {
         ProfScope prof("spmm", bytes, true);
@@ -1269,6 +1415,9 @@ void spmm_dot(const Mat *A, const double *P, long ldp, double *W, long ldw, int 
         if (A->use_dia)
             launch_dia_spmm<1>(A, A->d_dia, P, ldp, W, ldw, k, d_pmap, d_wmap, dpart, d_count,
                                prof.start_event(), prof.stop_event());
+        else if (rows)
+            launch_rows_spmm<1>(A, A->d_val, P, ldp, W, ldw, k, d_pmap, d_wmap, dpart, d_count,
+                                prof.start_event(), prof.stop_event());
         else
             launch_sell_spmm<1>(A, A->d_val, P, ldp, W, ldw, k, d_pmap, d_wmap, dpart, d_count,
                                 prof.start_event(), prof.stop_event());
@@ -1285,7 +1434,8 @@ void spmm_dot(const Mat *A, const float *P, long ldp, float *W, long ldw, int k,
     const double bytes = (double)A->nnz * (4 + 4) + 4.0 * (A->nrows + 1) +
                          4.0 * (double)k * (A->ncols + 2.0 * A->nrows);
     const bool dia = A->use_dia && A->d_dia32;
-    const int nparts = dia ? dia_blocks(A) : A->nslices;
+    const bool rows = !dia && A->use_rows && k >= kRowsMinCols;
+    const int nparts = dia ? dia_blocks(A) : (rows ? rows_parts(A) : A->nslices);
     double *dpart = partial_buffer((size_t)nparts * 2 * k);
     {
         ProfScope prof("spmm32", bytes, true);
@@ -1293,6 +1443,9 @@ void spmm_dot(const Mat *A, const float *P, long ldp, float *W, long ldw, int k,
         if (dia)
             launch_dia_spmm<2>(A, A->d_dia32, P, ldp, W, ldw, k, d_pmap, d_wmap, dpart, d_count,
                                prof.start_event(), prof.stop_event());
+        else if (rows)
+            launch_rows_spmm<2>(A, A->d_val32, P, ldp, W, ldw, k, d_pmap, d_wmap, dpart, d_count,
+                                prof.start_event(), prof.stop_event());
         else
             launch_sell_spmm<2>(A, A->d_val32, P, ldp, W, ldw, k, d_pmap, d_wmap, dpart, d_count,
                                 prof.start_event(), prof.stop_event());

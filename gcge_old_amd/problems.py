@@ -204,10 +204,26 @@ def lognormal_kappa(E: int, seed: int) -> np.ndarray:
     return np.exp(rng.standard_normal(E ** 3))
 
 
-def symmetric_permutation(A: CSR, perm: np.ndarray) -> CSR:
-    """P A P^T with row i of the result = row perm[i] of A."""
-    import scipy.sparse as sp
-
-    m = A.to_scipy()
-    P = sp.csr_matrix((np.ones(A.nrows), (np.arange(A.nrows), perm)), shape=m.shape)
-    return CSR.from_scipy(P @ m @ P.T)
+def symmetric_permutation(A: CSR, perm: np.ndarray, sort_rows: bool = True) -> CSR:
+    """P A P^T with row i of the result = row perm[i] of A (and column j =
+    column perm[j]); each row's entries are gathered in one pass and the
+    columns renumbered, then sorted within the row unless sort_rows is
+    False (CSR does not require sorted rows; the sort is the costly step at
+    10^8 entries)."""
+    n = A.nrows
+    perm = np.asarray(perm, dtype=np.int64)
+    inv = np.empty(n, dtype=np.int64)
+    inv[perm] = np.arange(n)
+    rp = A.rowptr.astype(np.int64)
+    counts = np.diff(rp)[perm]
+    rowptr = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(counts, out=rowptr[1:])
+    src = np.repeat(rp[perm] - rowptr[:-1], counts) + np.arange(rowptr[-1])
+    cols = inv[A.colidx[src]]
+    vals = A.vals[src]
+    del src
+    if sort_rows:
+        rows = np.repeat(np.arange(n, dtype=np.int64), counts)
+        order = np.lexsort((cols, rows))
+        cols, vals = cols[order], vals[order]
+    return CSR(n, A.ncols, rowptr.astype(np.int32), cols.astype(np.int32), vals)

@@ -15,6 +15,9 @@
  *              (one wavefront per slice, entries of a slice column-major),
  *              int32 column indices, float64 values (and an optional float32
  *              copy of the values for the mixed-precision inner solves);
+ *              matrices whose entries lie on few diagonals also get one
+ *              float64 array per diagonal, which their products then use
+ *              (GCGE_HIP_MatSetStorage);
  *   multivecs  one column-major block of n x ncols float64, leading dimension
  *              n rounded up to 64; column j is a vector view.
  * All device work runs on one HIP stream per process.  Operations that
@@ -65,13 +68,17 @@ void GCGE_HIP_MatSetPrecision(void *mat, int fp32);
  * covered by at most 9 runs of three consecutive offsets, rows with
  * increasing columns, diagonals at least half full: stencil and FEM matrices
  * in their natural ordering) also gets diagonal storage, and its products
- * use it.  diagonal == 0 switches to the sliced-ELLPACK form, which every
- * matrix has; diagonal != 0 switches back when diagonal storage exists.
- * Returns 1 when the products now use diagonal storage.  Results of the two
- * forms agree to rounding (same products, same order, fused multiply-adds). */
-int GCGE_HIP_MatSetStorage(void *mat, int diagonal);
+ * use it.  A matrix most of whose entries sit in sliced-ELLPACK entry rows
+ * spanning 256 columns or more (scattered columns, e.g. a random symmetric
+ * permutation) instead
+ * runs its products of 16 columns or more with whole-row gathers from a
+ * row-major copy of the column block.  storage: 0 = sliced ELLPACK with
+ * element gathers (every matrix has it), 1 = diagonal (when it exists,
+ * else 0), 2 = sliced ELLPACK with row gathers.  Returns the storage now
+ * used (0, 1 or 2).  Results of the forms agree to rounding. */
+int GCGE_HIP_MatSetStorage(void *mat, int storage);
 /* number of stored diagonals and of runs (0, 0 without diagonal storage);
- * returns 1 when the products use diagonal storage */
+ * returns the storage the products use (0, 1 or 2 as above) */
 int GCGE_HIP_MatStorageInfo(void *mat, int *ndiag, int *ngroups);
 
 /* ---- multivecs (replace the default MultiVecCreateByMat of per-vector
@@ -138,6 +145,11 @@ void GCGE_HIP_PatternGradient(void *mat, void **X, int xs, void **Y, int ys, int
  * dL/dkappa. */
 void GCGE_HIP_Q1ElementGradient(int E, void **X, int xs, void **Y, int ys, int k, const double *w,
                                 double *grad);
+/* The same for blocks whose rows are a permutation of the nodes (a solve of
+ * P K P^T, P M P^T): node_row[i] (host, (E-1)^3 ints) is the row that holds
+ * interior node i; entries out of range are rejected with an error. */
+void GCGE_HIP_Q1ElementGradientRows(int E, void **X, int xs, void **Y, int ys, int k,
+                                    const double *w, const int *node_row, double *grad);
 
 /* ---- measurement --------------------------------------------------------- */
 /* Per-kernel device time, measured with HIP events on the backend stream

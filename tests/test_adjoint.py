@@ -207,3 +207,43 @@ def test_torch_autograd_wrapper():
     w = torch.linspace(1.0, 3.0, nev, dtype=torch.float64)
     (w * lam).sum().backward()
     assert abs(float(kappa.detach() @ kappa.grad) - float(w @ lam.detach())) <= 1e-8 * float(w @ lam.detach())
+
+
+def test_symmetric_permutation_matches_scipy():
+    """P A P^T from the row gather equals the sparse triple product, with and
+    without sorting the renumbered columns of each row."""
+    import scipy.sparse as sp
+
+    K, _ = problems.q1_fem_3d(6, _kappa(6, 4))
+    perm = np.random.default_rng(8).permutation(K.nrows)
+    m = K.to_scipy()
+    P = sp.csr_matrix((np.ones(K.nrows), (np.arange(K.nrows), perm)), shape=m.shape)
+    want = (P @ m @ P.T).toarray()
+    for sort_rows in (True, False):
+        Kp = problems.symmetric_permutation(K, perm, sort_rows=sort_rows)
+        np.testing.assert_array_equal(Kp.to_scipy().toarray(), want)
+        if sort_rows:
+            for r in range(0, Kp.nrows, 37):
+                assert np.all(np.diff(Kp.colidx[Kp.rowptr[r]:Kp.rowptr[r + 1]]) > 0)
+
+
+@pytest.mark.gpu
+def test_device_kappa_gradient_of_permuted_blocks():
+    """Blocks of the permuted pencil (row i holds node perm[i]) with the node ->
+    row map give the same element gradient as the unpermuted blocks."""
+    from gcge_old_amd import adjoint
+
+    E = 7
+    rng = np.random.default_rng(9)
+    n = (E - 1) ** 3
+    X = rng.standard_normal((n, 5))
+    w = rng.uniform(size=5)
+    perm = rng.permutation(n)
+    inv = np.empty(n, dtype=np.int64)
+    inv[perm] = np.arange(n)
+    print("q1 element gradient, permuted rows", flush=True)
+    g = adjoint.q1_kappa_gradient(E, X, w)
+    gp = adjoint.q1_kappa_gradient(E, X[perm], w, node_row=inv)
+    np.testing.assert_allclose(gp, g, rtol=1e-14, atol=1e-15 * np.abs(g).max())
+    np.testing.assert_allclose(g, oadj.q1_element_gradient(E, X, w), rtol=1e-12,
+                               atol=1e-14 * np.abs(g).max())

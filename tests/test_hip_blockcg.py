@@ -20,14 +20,16 @@ pytestmark = pytest.mark.gpu
 
 
 def device_block_cg(A, B, rhs, x0, max_it, rate, if_shift=0, shift=0.0, fp32=False,
-                    diagonal=True):
+                    diagonal=True, storage=None):
     lib = _native.hip()
     Ad = DeviceMatrix(A)
     Bd = DeviceMatrix(B) if B is not None else None
-    if not diagonal:
+    if storage is None and not diagonal:
+        storage = "sliced"
+    if storage is not None:
         for d in (Ad, Bd):
             if d is not None:
-                d.set_storage(False)
+                d.set_storage(storage)
     Ad.set_precision(fp32)
     if Bd is not None:
         Bd.set_precision(fp32)
@@ -128,6 +130,27 @@ def test_device_bookkeeping_matches_host_bookkeeping(fp32, monkeypatch):
         out[mode] = device_block_cg(K, None, b, x0, 20, 1e-3, fp32=fp32)
     for a, h in zip(out["device"], out["host"]):
         np.testing.assert_array_equal(a, h)
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_device_block_cg_row_gathers_on_permuted_matrix(fp32):
+    """A randomly permuted Q1 pencil (row gathers, 20 columns: the p.w
+    epilogue of the row-gather product) against the restatement on the same
+    matrices: float64 to rounding, float32 within float32 rounding."""
+    K, M = problems.q1_fem_3d(9, problems.lognormal_kappa(9, 5))
+    perm = np.random.default_rng(36).permutation(K.nrows)
+    Kp = problems.symmetric_permutation(K, perm)
+    rng = np.random.default_rng(37)
+    b = rng.standard_normal((K.nrows, 20)) * np.logspace(0, -3, 20)
+    x0 = 1e-3 * rng.standard_normal((K.nrows, 20))
+    print(f"block CG row gathers fp32={fp32}", flush=True)
+    x, _ = device_block_cg(Kp, None, b, x0, 12, 1e-3, fp32=fp32)
+    xo, _ = oracle.block_cg(Kp, None, b, x0, 12, 1e-3)
+    if fp32:
+        rel = np.linalg.norm(x - xo, axis=0) / np.linalg.norm(xo, axis=0)
+        assert np.all(rel < 1e-4), rel
+    else:
+        _close(x, xo, 1e-9)
 
 
 def test_device_block_cg_fp32_large_beta():

@@ -66,17 +66,19 @@ MATS = {
 }
 
 
-@pytest.mark.parametrize("storage", ["default", "sliced"])
+@pytest.mark.parametrize("storage", ["default", "sliced", "rows"])
 @pytest.mark.parametrize("name", sorted(MATS))
-@pytest.mark.parametrize("k", [1, 2, 3, 5, 8, 10, 13, 17, 33])
+@pytest.mark.parametrize("k", [1, 2, 3, 5, 8, 10, 13, 17, 33, 70])
 def test_spmm_matches_oracle(name, k, storage):
-    """Both storages: diagonal (where the matrix has it) and sliced ELLPACK."""
+    """Every storage: the matrix's own choice (diagonal where it has it, row
+    gathers for scattered columns), sliced ELLPACK with element gathers, and
+    row gathers (from 16 columns; 70 columns take two column passes)."""
     A = MATS[name]()
     rng = _rng(k)
     X = rng.standard_normal((A.ncols, k + 3))
     dA = DeviceMatrix(A)
-    if storage == "sliced":
-        assert not dA.set_storage(False)
+    if storage != "default":
+        assert not dA.set_storage(storage)
     dX = DeviceMultiVec.from_host(X)
     dY = DeviceMultiVec(A.nrows, k + 2)
     _native.hip().GCGE_HIP_SpMM(dA.handle, dX.handle, 2, dY.handle, 1, k)
@@ -98,6 +100,7 @@ DIAGONAL_LAYOUT = {
     "tiny": (3, 2),            # -2 0 | 2
     "ragged": (0, 0),          # random pattern: too many diagonals
 }
+SCATTERED = {"ragged"}  # matrices whose products gather whole rows by default
 
 
 @pytest.mark.parametrize("name", sorted(DIAGONAL_LAYOUT))
@@ -106,13 +109,15 @@ def test_diagonal_storage_layout(name):
     want = DIAGONAL_LAYOUT[name]
     assert (info["ndiag"], info["runs"]) == want
     assert info["diagonal"] == (want[0] > 0)
+    assert info["storage"] == ("diagonal" if want[0] else ("rows" if name in SCATTERED else "sliced"))
 
 
 def test_no_diagonal_storage_for_permuted_or_unsorted_rows():
-    K = problems.q1_fem_3d(6)[0]
+    K = problems.q1_fem_3d(12)[0]  # n = 1331: permuted rows span far more than 256 columns
     perm = _rng(9).permutation(K.nrows)
     Kp = problems.CSR.from_scipy(K.to_scipy()[perm][:, perm].tocsr())
-    assert not DeviceMatrix(Kp).storage_info()["diagonal"]
+    info = DeviceMatrix(Kp).storage_info()
+    assert not info["diagonal"] and info["storage"] == "rows"
     # same entries, one row's columns listed in decreasing order
     rp, ci, v = K.rowptr.copy(), K.colidx.copy(), K.vals.copy()
     ci[rp[7]:rp[8]] = ci[rp[7]:rp[8]][::-1].copy()
