@@ -972,6 +972,63 @@ __global__ __launch_bounds__(kBlock) void k_lincomb(int n, const double *__restr
     }
 }
 
+// Y = beta Y + V C on the matrix cores, computed as Y^T = C^T V^T with
+// v_mfma_f64_16x16x4f64: a wavefront owns 16 RT rows and 16 TJ output columns
+// (grid y: column groups); per step of 4 input columns the B operand is 16
+// consecutive rows of a V column per lane group (lane l: V(r0 + l % 16,
+// m0 + l / 16)) and the A operand 16 coefficients of a row of Ct; lane l,
+// component v of a tile then holds Y(r0 + l % 16, j0 + l / 16 + 4 v), so 16
+// lanes write 16 consecutive rows of one column.
+template <int TJ, int RT>
+__global__ __launch_bounds__(kBlock) void k_lincomb_mfma(int n, const double *__restrict__ V, long ldv,
+                                                        int m, const double *__restrict__ Ct, int ldct,
+                                                        double beta, double *__restrict__ Y, long ldy,
+                                                        int k)
+{
+    const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+    const int r0 = (blockIdx.x * (kBlock / kWave) + wid) * 16 * RT;
+    if (r0 >= n) return;
+    const int j0 = blockIdx.y * 16 * TJ;
+    const int kk = lane / 16, li = lane % 16;
+    gcge_double4 acc[TJ][RT];
+#pragma unroll
+    for (int t = 0; t < TJ; ++t)
+#pragma unroll
+        for (int q = 0; q < RT; ++q) acc[t][q] = gcge_double4{0.0, 0.0, 0.0, 0.0};
+    for (int m0 = 0; m0 < m; m0 += 4) {
+        const int mm = m0 + kk;
+        const bool mok = mm < m;
+        double a[TJ], b[RT];
+#pragma unroll
+        for (int t = 0; t < TJ; ++t) {
+            const int j = j0 + t * 16 + li;
+            a[t] = (mok && j < k) ? Ct[(long)mm * ldct + j] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < RT; ++q) {
+            const int r = r0 + q * 16 + li;
+            b[q] = (mok && r < n) ? V[(long)mm * ldv + r] : 0.0;
+        }
+#pragma unroll
+        for (int t = 0; t < TJ; ++t)
+#pragma unroll
+            for (int q = 0; q < RT; ++q)
+                acc[t][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], b[q], acc[t][q], 0, 0, 0);
+    }
+#pragma unroll
+    for (int t = 0; t < TJ; ++t)
+#pragma unroll
+        for (int q = 0; q < RT; ++q)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int j = j0 + t * 16 + kk + 4 * v, r = r0 + q * 16 + li;
+                if (j < k && r < n) {
+                    double *y = Y + (long)j * ldy + r;
+                    *y = (beta == 0.0) ? acc[t][q][v] : beta * (*y) + acc[t][q][v];
+                }
+            }
+}
+
 // ---- streaming kernels ------------------------------------------------------------------
 // y = a x + b y with the reference special cases (a == 0: x not read;
 // b == 0: y not read)
@@ -1666,6 +1723,27 @@ void lincomb_dev(int n, const double *V, long ldv, int m, const double *d_Ct, in
     ProfScope prof("lincomb", 8.0 * (double)n * (m + k) + (beta == 0.0 ? 0.0 : 8.0 * (double)n * k));
     const dim3 grid(ceil_div(n, kBlock)), block(kBlock);
     hipStream_t st = state().stream;
+    static const int mode = [] {
+        const char *e = getenv("GCGE_HIP_LINCOMB");  // "fma": the vector kernel (measurement)
+        return (e && strcmp(e, "fma") == 0) ? 1 : 0;
+    }();
+    // matrix cores for 9..16 and more than 32 columns: 64 rows x 16 columns
+    // or 32 rows x 64 columns per wavefront; measured on MI355X (n = 250047):
+    // m = 70, k = 10: 27.7 us against 34.1 (vector kernel); m = 90, k = 50:
+    // 113.7 against 136.3; m = 80, k = 30: 57.5 against 47.9, so the vector
+    // kernel keeps 17..32
+    if (mode == 0 && k > 8 && (k <= 16 || k > 32)) {
+        const int wpb = kBlock / kWave;
+        if (k <= 16) {
+            hipLaunchKernelGGL((k_lincomb_mfma<1, 4>), dim3(ceil_div(n, 64 * wpb)), block, 0, st, n, V, ldv, m,
+                               d_Ct, ldct, beta, Y, ldy, k);
+        } else {
+            hipLaunchKernelGGL((k_lincomb_mfma<4, 2>), dim3(ceil_div(n, 32 * wpb), ceil_div(k, 64)), block, 0, st,
+                               n, V, ldv, m, d_Ct, ldct, beta, Y, ldy, k);
+        }
+        HIP_CHECK(hipGetLastError());
+        return;
+    }
     switch (lincomb_group(k)) {
     case 8: hipLaunchKernelGGL(k_lincomb<8>, grid, block, 0, st, n, V, ldv, m, d_Ct, ldct, beta, Y, ldy, k); break;
     case 16: hipLaunchKernelGGL(k_lincomb<16>, grid, block, 0, st, n, V, ldv, m, d_Ct, ldct, beta, Y, ldy, k); break;
