@@ -75,6 +75,9 @@ def parse():
                         "all elements")
     p.add_argument("--fd", type=int, default=8, help="elements checked by finite differences "
                                                      "(q1-64-adjoint)")
+    p.add_argument("--reorder", choices=("none", "rcm"), default="none",
+                   help="q1-192-permuted: reorder the permuted pencil by reverse Cuthill-McKee "
+                        "(computed from the matrix graph, a preprocessing step) before the solve")
     p.add_argument("--mesh", type=int, default=None, help="elements per axis (overrides the workload)")
     p.add_argument("--nev", type=int, default=None)
     p.add_argument("--kappa-seed", type=int, default=1)
@@ -392,10 +395,25 @@ def permuted_workload(args):
     Mp = problems.symmetric_permutation(M, perm, sort_rows=False)
     del K, M
     t_prep = time.time() - t0
+    # row i of the pencil holds node node_of_row[i]
+    node_of_row = perm
+    t_rcm = None
+    if args.reorder == "rcm":
+        from scipy.sparse.csgraph import reverse_cuthill_mckee
+
+        t1 = time.time()
+        g = Mp.to_scipy()  # M has every stored position of K
+        g.sort_indices()
+        r = reverse_cuthill_mckee(g, symmetric_mode=True)
+        del g
+        Kp = problems.symmetric_permutation(Kp, r)
+        Mp = problems.symmetric_permutation(Mp, r)
+        node_of_row = perm[r]
+        t_rcm = time.time() - t1
     dK, dM = gcge.DeviceMatrix(Kp), gcge.DeviceMatrix(Mp)
     print(f"# workload: Q1 {E}^3 lognormal kappa, randomly permuted, n={n} nnz(K)={Kp.nnz} "
-          f"nnz(M)={Mp.nnz} nev={nev} (assembly + permutation {t_prep:.1f}s, storage "
-          f"{dK.storage_info()})", file=sys.stderr, flush=True)
+          f"nnz(M)={Mp.nnz} nev={nev} (assembly + permutation {t_prep:.1f}s, reordering "
+          f"{args.reorder} {t_rcm}, storage {dK.storage_info()})", file=sys.stderr, flush=True)
     def flags_for(max_it):
         return ["-gcge_conv_type", "RB", "-gcge_ev_tol", str(args.tol), "-gcge_ev_max_it",
                 str(max_it), "-gcge_block_size", str(2 * nev), "-gcge_full_rr", "1",
@@ -434,7 +452,7 @@ def permuted_workload(args):
     Xd = gcge.DeviceMultiVec.from_host(X)
     del X
     inv = np.empty(n, dtype=np.int64)
-    inv[perm] = np.arange(n)
+    inv[node_of_row] = np.arange(n)
     lib.GCGE_HIP_Synchronize()
     t = time.perf_counter()
     g = adjoint.q1_kappa_gradient(E, Xd, w, node_row=inv)
@@ -452,7 +470,9 @@ def permuted_workload(args):
             "ms_per_step": round(fp64 * 1e3, 2), "higher_is_better": False, "scaling": "weak",
             "vs_baseline": None, "dtype": "f64", "data": "synthetic",
             "config": {"workload": f"Q1 FEM {E}^3 lognormal kappa (seed {args.kappa_seed}), rows and "
-                                   "columns randomly permuted, nev=200, block 400 requested",
+                                   "columns randomly permuted, nev=200, block 400 requested"
+                                   + (", then reordered by reverse Cuthill-McKee" if t_rcm is not None else ""),
+                       "reorder": args.reorder, "reorder_s": t_rcm,
                        "n": n, "nnz_K": Kp.nnz, "nnz_M": Mp.nnz, "nev": nev,
                        "rayleigh_ritz": "full (-gcge_full_rr 1)", "w_orth": "svqb"},
             "fp64": legs["fp64"], "mixed": legs["mixed"],

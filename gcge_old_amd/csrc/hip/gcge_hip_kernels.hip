@@ -815,7 +815,7 @@ __global__ __launch_bounds__(kBlock) void k_gram_cols(int n, int rows_per_block,
 // 16 consecutive threads reading one column's 16 consecutive rows; every
 // thread issues all its loads of a stage at once, and the next stage's loads
 // are in flight while the matrix cores work on the current one.  The
-// (16 x 16) output tiles are shared round-robin by the four waves; each MFMA
+// (16 x 16) output tiles are shared round-robin by the NW waves; each MFMA
 // takes one element of V and one of W per lane from LDS (A: lane l holds
 // Vt[l % 16][l / 16] of the 16 x 4 block, B: W[l / 16][l % 16]), and lane l,
 // component v of the accumulator holds D[l / 16 + 4 v][l % 16] (layout checked
@@ -824,8 +824,8 @@ typedef double gcge_double4 __attribute__((ext_vector_type(4)));
 constexpr int kGmRows = 16;
 constexpr int kGmCols = 128;
 
-template <int PW, int EPT>
-__global__ __launch_bounds__(kBlock) void k_gram_mfma(int n, int rows_per_block,
+template <int PW, int EPT, int NW>
+__global__ __launch_bounds__(64 * NW) void k_gram_mfma(int n, int rows_per_block,
                                                      const double *__restrict__ V, long ldv,
                                                      int m, const double *__restrict__ W,
                                                      long ldw, int k,
@@ -840,8 +840,8 @@ __global__ __launch_bounds__(kBlock) void k_gram_mfma(int n, int rows_per_block,
     const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
     const int r0 = blockIdx.x * rows_per_block, r1 = min(n, r0 + rows_per_block);
     const int ncols = MP + KP;
-    // this thread's stage elements: row q, columns c0 + 16 t (16 = kBlock / kGmRows)
-    constexpr int kColStep = kBlock / kGmRows;
+    // this thread's stage elements: row q, columns c0 + kColStep t
+    constexpr int kColStep = 64 * NW / kGmRows;
     const int q = threadIdx.x % kGmRows, c0 = threadIdx.x / kGmRows;
     double buf[EPT];
     auto load_stage = [&](int rs) {
@@ -877,7 +877,7 @@ __global__ __launch_bounds__(kBlock) void k_gram_mfma(int n, int rows_per_block,
             const int qa = kk * 4 + lane / 16;
 #pragma unroll
             for (int p = 0; p < PW; ++p) {
-                const int pair = wid + 4 * p;
+                const int pair = wid + NW * p;
                 if (pair < npairs) {
                     const double a = lds_gm[qa * ld + (pair / tj) * 16 + lane % 16];
                     const double b = lds_gm[qa * ld + MP + (pair % tj) * 16 + lane % 16];
@@ -890,7 +890,7 @@ __global__ __launch_bounds__(kBlock) void k_gram_mfma(int n, int rows_per_block,
     double *out = partial + (long)blockIdx.x * m * k;
 #pragma unroll
     for (int p = 0; p < PW; ++p) {
-        const int pair = wid + 4 * p;
+        const int pair = wid + NW * p;
         if (pair < npairs) {
             const int j = j_base + (pair % tj) * 16 + lane % 16;
 #pragma unroll
@@ -899,6 +899,40 @@ __global__ __launch_bounds__(kBlock) void k_gram_mfma(int n, int rows_per_block,
                 if (i < i_base + mb && j < j_base + kb) out[(long)j * m + i] = acc[p][v];
             }
         }
+    }
+}
+
+template <int P, int NW>
+void launch_gram_mfma_e(int ept, dim3 grid, size_t lds, int n, int rows, const double *V, long ldv,
+                        int m, const double *W, long ldw, int k, double *partial)
+{
+    hipStream_t st = state().stream;
+    const dim3 block(64 * NW);
+    if (ept <= 2)
+        hipLaunchKernelGGL((k_gram_mfma<P, 2, NW>), grid, block, lds, st, n, rows, V, ldv, m, W, ldw, k, partial);
+    else if (ept <= 4)
+        hipLaunchKernelGGL((k_gram_mfma<P, 4, NW>), grid, block, lds, st, n, rows, V, ldv, m, W, ldw, k, partial);
+    else if (ept <= 8)
+        hipLaunchKernelGGL((k_gram_mfma<P, 8, NW>), grid, block, lds, st, n, rows, V, ldv, m, W, ldw, k, partial);
+    else
+        hipLaunchKernelGGL((k_gram_mfma<P, 16, NW>), grid, block, lds, st, n, rows, V, ldv, m, W, ldw, k, partial);
+}
+
+// tile pairs per wave: 1, 2 or 8 (the 4-pair instance keeps its accumulators
+// in VGPRs and drops to one or two waves per SIMD; the 8-pair one does not);
+// NW = 4 or 8 waves per workgroup
+void launch_gram_mfma(int nw, int pw, int ept, dim3 grid, size_t lds, int n, int rows,
+                      const double *V, long ldv, int m, const double *W, long ldw, int k,
+                      double *partial)
+{
+    if (nw == 8) {
+        if (pw <= 1) launch_gram_mfma_e<1, 8>(ept, grid, lds, n, rows, V, ldv, m, W, ldw, k, partial);
+        else if (pw <= 2) launch_gram_mfma_e<2, 8>(ept, grid, lds, n, rows, V, ldv, m, W, ldw, k, partial);
+        else launch_gram_mfma_e<8, 8>(ept, grid, lds, n, rows, V, ldv, m, W, ldw, k, partial);
+    } else {
+        if (pw <= 1) launch_gram_mfma_e<1, 4>(ept, grid, lds, n, rows, V, ldv, m, W, ldw, k, partial);
+        else if (pw <= 2) launch_gram_mfma_e<2, 4>(ept, grid, lds, n, rows, V, ldv, m, W, ldw, k, partial);
+        else launch_gram_mfma_e<8, 4>(ept, grid, lds, n, rows, V, ldv, m, W, ldw, k, partial);
     }
 }
 
@@ -1589,9 +1623,12 @@ void gram(int n, const double *V, long ldv, int m, const double *W, long ldw, in
         // matrix cores: 128 x 128 output blocks (grid y, z), tile pairs shared by four waves
         const int gy = ceil_div(m, kGmCols), gz = ceil_div(k, kGmCols);
         const int ti = ceil_div(std::min(m, kGmCols), 16), tj = ceil_div(std::min(k, kGmCols), 16);
-        const int pw = ceil_div(ti * tj, kBlock / kWave);
+        // eight waves per workgroup beyond 32 output tiles (four would hold 9..16
+        // accumulator tiles each and run at one wave per SIMD)
+        const int nw = ti * tj > 32 ? 8 : 4;
+        const int pw = ceil_div(ti * tj, nw);
         const int ncols = 16 * (ti + tj);
-        const int ept = ceil_div(kGmRows * ncols, kBlock);
+        const int ept = ceil_div(kGmRows * ncols, 64 * nw);
         const size_t lds = (size_t)kGmRows * (ncols + 1) * sizeof(double);
         // about four workgroups per CU in all, at least two stages each, and
         // partials of at most a tenth of the input bytes (the reduction pass
@@ -1607,27 +1644,7 @@ void gram(int n, const double *V, long ldv, int m, const double *W, long ldw, in
         {
             ProfScope prof("gram", 8.0 * (double)n * (m + k));
             dim3 grid(nbx, gy, gz);
-#define GCGE_GRAM_MFMA(P, E)                                                                       \
-    hipLaunchKernelGGL((k_gram_mfma<P, E>), grid, dim3(kBlock), lds, s.stream, n, rows, V, ldv, m, W, ldw, \
-                       k, partial)
-#define GCGE_GRAM_EPT(P)                     \
-    if (ept <= 2) GCGE_GRAM_MFMA(P, 2);      \
-    else if (ept <= 4) GCGE_GRAM_MFMA(P, 4);  \
-    else if (ept <= 8) GCGE_GRAM_MFMA(P, 8); \
-    else GCGE_GRAM_MFMA(P, 16);
-            if (pw <= 1) {
-                GCGE_GRAM_EPT(1)
-            } else if (pw <= 2) {
-                GCGE_GRAM_EPT(2)
-            } else if (pw <= 8) {
-                // (the 4-pair instance keeps its accumulators in VGPRs and
-                // drops to one or two waves per SIMD; the 8-pair one does not)
-                GCGE_GRAM_EPT(8)
-            } else {
-                GCGE_GRAM_EPT(16)
-            }
-#undef GCGE_GRAM_EPT
-#undef GCGE_GRAM_MFMA
+            launch_gram_mfma(nw, pw, ept, grid, lds, n, rows, V, ldv, m, W, ldw, k, partial);
             HIP_CHECK(hipGetLastError());
         }
         {

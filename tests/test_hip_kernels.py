@@ -149,7 +149,8 @@ def test_spmm_storages_agree(name):
                                    (1000, 70, 10), (4097, 130, 65), (100000, 12, 3),
                                    (100000, 1, 1), (250047, 2, 3), (5000, 8, 8), (777, 5, 5),
                                    (3, 4, 2), (1025, 1, 7), (300000, 8, 1),
-                                   (250047, 80, 10), (3000, 200, 16), (777, 33, 9)])
+                                   (250047, 80, 10), (3000, 200, 16), (777, 33, 9),
+                                   (2000, 100, 100), (1500, 256, 200), (999, 96, 96)])
 def test_gram_matches_oracle(n, m, k):
     rng = _rng(n + m + k)
     V = rng.standard_normal((n, m + 1))
