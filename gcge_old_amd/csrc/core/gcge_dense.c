@@ -173,6 +173,27 @@ static double hypot_safe(double a, double b)
     return bb * sqrt(1.0 + r * r);
 }
 
+/* sqrt(a^2 + b^2) with one square root when the squares neither overflow
+ * nor underflow (the division of the scaled form sits on the QL
+ * recurrence's dependency chain), the scaled form otherwise */
+static inline double hypot_fast(double a, double b)
+{
+    const double s = a * a + b * b;
+    if (s > 1e-290 && s < 1e290) return sqrt(s);
+    return hypot_safe(a, b);
+}
+
+/* 2-norm of x[0..m-1]: sum of squares, scaled form when out of range */
+static double norm2(const double *x, int m)
+{
+    double s = 0.0;
+    for (int i = 0; i < m; ++i) s += x[i] * x[i];
+    if (s > 1e-290 && s < 1e290) return sqrt(s);
+    double r = 0.0;
+    for (int i = 0; i < m; ++i) r = hypot_safe(r, x[i]);
+    return r;
+}
+
 /* A (n x n, column-major, full symmetric) -> tridiagonal (d, e) with
  * A = Q T Q^T; e[i] couples i and i+1.  The Householder vectors are left in
  * the columns of A below the subdiagonal (unit leading entry implied),
@@ -183,8 +204,8 @@ static void sym_tridiag(int n, double *A, double *d, double *e, double *tau, dou
     for (int j = 0; j < n - 2; ++j) {
         const int m = n - j - 1; /* length of the column below the diagonal */
         double *x = &AC(j + 1, j);
-        double alpha = x[0], xnorm = 0.0;
-        for (int i = 1; i < m; ++i) xnorm = hypot_safe(xnorm, x[i]);
+        double alpha = x[0];
+        const double xnorm = m > 1 ? norm2(x + 1, m - 1) : 0.0;
         d[j] = AC(j, j);
         if (xnorm == 0.0) {
             tau[j] = 0.0;
@@ -299,15 +320,16 @@ static int tridiag_ql(int n, double *d, double *e, double *Z, int ldz, double *c
                 for (i = m - 1; i >= l; --i) {
                     double f = s * e[i];
                     double b = c * e[i];
-                    r = hypot_safe(f, g);
+                    r = hypot_fast(f, g);
                     e[i + 1] = r;
                     if (r == 0.0) {
                         d[i + 1] -= p;
                         e[m] = 0.0;
                         break;
                     }
-                    s = f / r;
-                    c = g / r;
+                    const double ri = 1.0 / r;
+                    s = f * ri;
+                    c = g * ri;
                     g = d[i + 1] - p;
                     r = (d[i] - g) * s + 2.0 * c * b;
                     p = s * r;
