@@ -77,6 +77,39 @@ class CSR:
         aa = data[3 + nrow + 1 + nnz:3 + nrow + 1 + 2 * nnz].copy()
         return CSR(nrow, ncol, ia, ja, aa)
 
+    PETSC_MAT_CLASSID = 1211216
+
+    @staticmethod
+    def read_petsc(path: str) -> "CSR":
+        """A PETSc binary matrix file, the input of the reference's SLEPc
+        driver (app/slepc/gcge_app_slepc.c:25-36, MatLoad): big-endian
+        int32 class id 1211216, rows, columns, nonzeros, the row lengths,
+        the column indices, then float64 values."""
+        with open(path, "rb") as f:
+            data = f.read()
+        head = np.frombuffer(data, dtype=">i4", count=4)
+        if head[0] != CSR.PETSC_MAT_CLASSID:
+            raise ValueError(f"{path}: not a PETSc binary matrix (class id {head[0]})")
+        m, n, nz = (int(v) for v in head[1:])
+        if len(data) != 16 + 4 * m + 12 * nz:
+            raise ValueError(f"{path}: size {len(data)} does not match {m} rows and {nz} nonzeros")
+        lens = np.frombuffer(data, dtype=">i4", count=m, offset=16).astype(np.int64)
+        cols = np.frombuffer(data, dtype=">i4", count=nz, offset=16 + 4 * m).astype(np.int32)
+        vals = np.frombuffer(data, dtype=">f8", count=nz, offset=16 + 4 * m + 4 * nz).astype(np.float64)
+        rowptr = np.zeros(m + 1, dtype=np.int64)
+        np.cumsum(lens, out=rowptr[1:])
+        if rowptr[-1] != nz:
+            raise ValueError(f"{path}: row lengths sum to {rowptr[-1]}, not {nz}")
+        return CSR(m, n, rowptr.astype(np.int32), cols, vals)
+
+    def write_petsc(self, path: str) -> None:
+        """The same format (readable by PETSc's MatLoad)."""
+        with open(path, "wb") as f:
+            f.write(np.array([CSR.PETSC_MAT_CLASSID, self.nrows, self.ncols, self.nnz], ">i4").tobytes())
+            f.write(np.diff(self.rowptr).astype(">i4").tobytes())
+            f.write(np.asarray(self.colidx).astype(">i4").tobytes())
+            f.write(np.asarray(self.vals).astype(">f8").tobytes())
+
     def write_text(self, path: str) -> None:
         with open(path, "w") as f:
             f.write(f"{self.nrows}\n{self.ncols}\n{self.nnz}\n")

@@ -4,6 +4,8 @@ that holds the reference tree; the GPU box only reads the results).
 1. data/*.npz -- the CSR matrices the reference ships as test inputs
    (test/data/testA, testB, A_3.txt, M_3.txt, A_5.txt, M_5.txt), converted
    from the reference text format to npz.
+   data/A_5.petsc.bin, M_5.petsc.bin -- the same matrices as the
+   reference ships them in PETSc's binary format (copied unchanged).
 2. ref_runs.npz -- outputs of the reference solver built from its own
    sources (oracle/_ref/gcge_ref) on small problems, each started from a
    fixed initial block: final eigenvalues, residuals, iteration counts and
@@ -82,8 +84,17 @@ def build(spec):
     raise ValueError(spec)
 
 
+def copy_petsc_fixtures():
+    import shutil
+
+    for f in ("A_5.petsc.bin", "M_5.petsc.bin"):
+        shutil.copyfile(os.path.join(REF, "test", "data", f), os.path.join(HERE, "data", f))
+        print("fixture", f)
+
+
 def main():
     os.makedirs(os.path.join(HERE, "data"), exist_ok=True)
+    copy_petsc_fixtures()
     for f in DATA:
         m = problems.CSR.read_text(os.path.join(REF, "test", "data", f))
         name = f.replace(".txt", "")

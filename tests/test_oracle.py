@@ -106,6 +106,34 @@ def test_csr_text_round_trip(tmp_path, golden_dir):
     assert oracle.lib().ORACLE_CSR_ReadFile(str(tmp_path / "missing").encode()) is None
 
 
+@pytest.mark.parametrize("name", ["A_5", "M_5"])
+def test_petsc_binary_fixture_matches_text_fixture(name, golden_dir, tmp_path):
+    """The reference ships A_5 / M_5 both as CSR text and as PETSc binary
+    (the SLEPc driver's MatLoad input): the binary reader gives the same
+    matrix bit for bit, and writing it back reproduces the file."""
+    path = os.path.join(golden_dir, "data", name + ".petsc.bin")
+    A = problems.CSR.read_petsc(path)
+    T = _fixture(golden_dir, name)
+    np.testing.assert_array_equal(A.rowptr, T.rowptr)
+    np.testing.assert_array_equal(A.colidx, T.colidx)
+    np.testing.assert_array_equal(A.vals, T.vals)
+    out = str(tmp_path / "copy.bin")
+    A.write_petsc(out)
+    with open(out, "rb") as f1, open(path, "rb") as f2:
+        assert f1.read() == f2.read()
+
+
+def test_petsc_binary_rejects_other_files(tmp_path):
+    bad = tmp_path / "bad.bin"
+    bad.write_bytes(np.array([1211214, 2, 2, 0], ">i4").tobytes())  # a vector class id
+    with pytest.raises(ValueError):
+        problems.CSR.read_petsc(str(bad))
+    short = tmp_path / "short.bin"
+    short.write_bytes(np.array([1211216, 2, 2, 3, 1, 1], ">i4").tobytes())
+    with pytest.raises(ValueError):
+        problems.CSR.read_petsc(str(short))
+
+
 def test_problem_generators_closed_form():
     """The generators themselves: dense eigenvalues of small instances equal
     the closed forms the end-to-end checks rely on."""
