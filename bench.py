@@ -78,6 +78,9 @@ def parse():
     p.add_argument("--reorder", choices=("none", "rcm"), default="none",
                    help="q1-192-permuted: reorder the permuted pencil by reverse Cuthill-McKee "
                         "(computed from the matrix graph, a preprocessing step) before the solve")
+    p.add_argument("--assembly", choices=("device", "host"), default="device",
+                   help="q1-192-permuted: assemble K and M on the device from kappa and the "
+                        "node -> row map, or on the host (numpy) and copy the CSR arrays")
     p.add_argument("--mesh", type=int, default=None, help="elements per axis (overrides the workload)")
     p.add_argument("--nev", type=int, default=None)
     p.add_argument("--kappa-seed", type=int, default=1)
@@ -388,32 +391,54 @@ def permuted_workload(args):
     iters = args.max_it or 3
     t0 = time.time()
     kappa = problems.lognormal_kappa(E, args.kappa_seed)
-    K, M = problems.q1_fem_3d(E, kappa)
-    n = K.nrows
+    n = (E - 1) ** 3
     perm = np.random.Generator(np.random.PCG64(args.kappa_seed + 1000)).permutation(n)
-    Kp = problems.symmetric_permutation(K, perm, sort_rows=False)
-    Mp = problems.symmetric_permutation(M, perm, sort_rows=False)
-    del K, M
-    t_prep = time.time() - t0
     # row i of the pencil holds node node_of_row[i]
     node_of_row = perm
+
+    def inverse(p):
+        q = np.empty(n, dtype=np.int32)
+        q[p] = np.arange(n, dtype=np.int32)
+        return q
+
+    if args.assembly == "device":
+        gcge.init()
+        dK = gcge.DeviceMatrix.q1(E, kappa, "stiffness", inverse(node_of_row))
+        dM = gcge.DeviceMatrix.q1(E, kappa, "mass", inverse(node_of_row))
+    else:
+        K, M = problems.q1_fem_3d(E, kappa)
+        Kp = problems.symmetric_permutation(K, perm, sort_rows=False)
+        Mp = problems.symmetric_permutation(M, perm, sort_rows=False)
+        del K, M
+        dK, dM = gcge.DeviceMatrix(Kp), gcge.DeviceMatrix(Mp)
+    lib.GCGE_HIP_Synchronize()
+    t_prep = time.time() - t0
     t_rcm = None
     if args.reorder == "rcm":
         from scipy.sparse.csgraph import reverse_cuthill_mckee
 
         t1 = time.time()
+        Mp = dM.to_csr() if args.assembly == "device" else Mp
         g = Mp.to_scipy()  # M has every stored position of K
         g.sort_indices()
         r = reverse_cuthill_mckee(g, symmetric_mode=True)
         del g
-        Kp = problems.symmetric_permutation(Kp, r)
-        Mp = problems.symmetric_permutation(Mp, r)
         node_of_row = perm[r]
+        if args.assembly == "device":
+            del dK, dM, Mp
+            dK = gcge.DeviceMatrix.q1(E, kappa, "stiffness", inverse(node_of_row))
+            dM = gcge.DeviceMatrix.q1(E, kappa, "mass", inverse(node_of_row))
+        else:
+            Kp = problems.symmetric_permutation(Kp, r)
+            Mp = problems.symmetric_permutation(Mp, r)
+            dK, dM = gcge.DeviceMatrix(Kp), gcge.DeviceMatrix(Mp)
         t_rcm = time.time() - t1
-    dK, dM = gcge.DeviceMatrix(Kp), gcge.DeviceMatrix(Mp)
-    print(f"# workload: Q1 {E}^3 lognormal kappa, randomly permuted, n={n} nnz(K)={Kp.nnz} "
-          f"nnz(M)={Mp.nnz} nev={nev} (assembly + permutation {t_prep:.1f}s, reordering "
-          f"{args.reorder} {t_rcm}, storage {dK.storage_info()})", file=sys.stderr, flush=True)
+    if args.assembly == "host":
+        del Kp, Mp
+    print(f"# workload: Q1 {E}^3 lognormal kappa, randomly permuted, n={n} nnz(K)={dK.nnz} "
+          f"nnz(M)={dM.nnz} nev={nev} (set-up: {args.assembly} assembly + permutation + device "
+          f"matrices {t_prep:.2f}s, reordering {args.reorder} {t_rcm}, storage "
+          f"{dK.storage_info()})", file=sys.stderr, flush=True)
     def flags_for(max_it):
         return ["-gcge_conv_type", "RB", "-gcge_ev_tol", str(args.tol), "-gcge_ev_max_it",
                 str(max_it), "-gcge_block_size", str(2 * nev), "-gcge_full_rr", "1",
@@ -473,7 +498,8 @@ def permuted_workload(args):
                                    "columns randomly permuted, nev=200, block 400 requested"
                                    + (", then reordered by reverse Cuthill-McKee" if t_rcm is not None else ""),
                        "reorder": args.reorder, "reorder_s": t_rcm,
-                       "n": n, "nnz_K": Kp.nnz, "nnz_M": Mp.nnz, "nev": nev,
+                       "assembly": args.assembly, "setup_s": round(t_prep, 3),
+                       "n": n, "nnz_K": dK.nnz, "nnz_M": dM.nnz, "nev": nev,
                        "rayleigh_ritz": "full (-gcge_full_rr 1)", "w_orth": "svqb"},
             "fp64": legs["fp64"], "mixed": legs["mixed"],
             "mixed_speedup_per_iteration": round(fp64 / mixed, 3),

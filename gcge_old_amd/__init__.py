@@ -54,6 +54,53 @@ class DeviceMatrix:
             raise RuntimeError("GCGE_HIP_MatCreateCSR failed")
         self.fp32 = False
 
+    @classmethod
+    def q1(cls, E: int, kappa=None, which: str = "stiffness", node_row=None) -> "DeviceMatrix":
+        """Q1 stiffness (``which="stiffness"``, coefficients ``kappa``: E^3
+        element values, None for 1) or mass (``"mass"``) matrix of the E^3
+        mesh assembled on the device, the same matrix bit for bit as
+        ``problems.q1_fem_3d`` (rows and columns renumbered by ``node_row``:
+        row ``node_row[i]`` holds node i, rows sorted by column)."""
+        init()
+        if which not in ("stiffness", "mass"):
+            raise ValueError("which must be 'stiffness' or 'mass'")
+        w = 0 if which == "stiffness" else 1
+        kp = None
+        if kappa is not None and w == 0:
+            kp = np.ascontiguousarray(kappa, dtype=np.float64).ravel()
+            if kp.size != E ** 3:
+                raise ValueError(f"kappa needs {E ** 3} element values, got {kp.size}")
+        nr = None
+        if node_row is not None:
+            nr = np.ascontiguousarray(node_row, dtype=np.int32).ravel()
+            n = (E - 1) ** 3
+            if nr.size != n:
+                raise ValueError(f"node_row needs {n} entries, got {nr.size}")
+            if n and (nr.min() < 0 or nr.max() >= n or
+                      not np.all(np.bincount(nr, minlength=n) == 1)):
+                raise ValueError("node_row must be a permutation of 0..n-1")
+        self = cls.__new__(cls)
+        self.csr = None
+        self.handle = _native.hip().GCGE_HIP_MatCreateQ1(
+            E, _native.as_dptr(kp) if kp is not None else None, w,
+            _native.as_iptr(nr) if nr is not None else None)
+        if not self.handle:
+            raise RuntimeError("GCGE_HIP_MatCreateQ1 failed")
+        n = (E - 1) ** 3
+        self.nrows = self.ncols = n
+        self.nnz = int(_native.hip().GCGE_HIP_MatGetCSR(self.handle, None, None, None))
+        self.fp32 = False
+        return self
+
+    def to_csr(self) -> problems.CSR:
+        """The matrix copied back from the device in CSR form."""
+        rp = np.zeros(self.nrows + 1, dtype=np.int32)
+        ci = np.zeros(self.nnz, dtype=np.int32)
+        v = np.zeros(self.nnz, dtype=np.float64)
+        _native.hip().GCGE_HIP_MatGetCSR(self.handle, _native.as_iptr(rp), _native.as_iptr(ci),
+                                        _native.as_dptr(v))
+        return problems.CSR(self.nrows, self.ncols, rp, ci, v)
+
     def set_precision(self, fp32: bool) -> None:
         """fp32=True: keep a float32 copy of the values and run the inner
         block CG products in float32 (mixed-precision form)."""

@@ -67,6 +67,10 @@ def _declare(name: str, lib: C.CDLL) -> None:
         lib.GCGE_HIP_SetSeed.argtypes = [C.c_uint64]
         lib.GCGE_HIP_MatCreateCSR.argtypes = [C.c_int, C.c_int, C.c_int, iptr, iptr, dptr]
         lib.GCGE_HIP_MatCreateCSR.restype = C.c_void_p
+        lib.GCGE_HIP_MatCreateQ1.argtypes = [C.c_int, dptr, C.c_int, iptr]
+        lib.GCGE_HIP_MatCreateQ1.restype = C.c_void_p
+        lib.GCGE_HIP_MatGetCSR.argtypes = [C.c_void_p, iptr, iptr, dptr]
+        lib.GCGE_HIP_MatGetCSR.restype = C.c_int
         lib.GCGE_HIP_MatDestroy.argtypes = [C.c_void_p]
         lib.GCGE_HIP_MatSetPrecision.argtypes = [C.c_void_p, C.c_int]
         lib.GCGE_HIP_MatSetStorage.argtypes = [C.c_void_p, C.c_int]

@@ -231,6 +231,27 @@ static MatHost *mat_host(const Mat *m)
 // increasing order (the SpMM then adds a row's products in CSR order), and
 // the diagonals must be at least half full.  Otherwise the matrix keeps only
 // its sliced-ELLPACK form.
+// Runs of the sorted offsets; false when they need more than kDiaMaxGroups
+// runs or the diagonals are less than half full.
+static bool dia_layout(const std::vector<int> &offs, long nnz, int n, DiaDesc *dd)
+{
+    const int nd = (int)offs.size();
+    if (nd == 0 || (double)nnz < 0.5 * nd * (double)n) return false;
+    int g = 0;
+    for (int i = 0; i < nd;) {
+        if (g == kDiaMaxGroups) return false;
+        const int c = offs[i] + 1;
+        dd->center[g] = c;
+        for (int p = 0; p < 3; ++p) {
+            dd->slot[g][p] = -1;
+            if (i < nd && offs[i] == c - 1 + p) dd->slot[g][p] = i++;
+        }
+        ++g;
+    }
+    dd->ngroups = g;
+    return true;
+}
+
 static void build_dia(Mat *m, const int *rowptr, const int *colidx, const double *vals)
 {
     const int n = m->nrows;
@@ -260,20 +281,8 @@ static void build_dia(Mat *m, const int *rowptr, const int *colidx, const double
     }
     std::sort(offs.begin(), offs.end());
     const int nd = (int)offs.size();
-    if ((double)m->nnz < 0.5 * nd * (double)n) return;
     DiaDesc dd{};
-    int g = 0;
-    for (int i = 0; i < nd;) {
-        if (g == kDiaMaxGroups) return;
-        const int c = offs[i] + 1;
-        dd.center[g] = c;
-        for (int p = 0; p < 3; ++p) {
-            dd.slot[g][p] = -1;
-            if (i < nd && offs[i] == c - 1 + p) dd.slot[g][p] = i++;
-        }
-        ++g;
-    }
-    dd.ngroups = g;
+    if (!dia_layout(offs, m->nnz, n, &dd)) return;
     const size_t ld = (size_t)m->nslices * 64;
     std::vector<double> dia((size_t)nd * ld, 0.0);
     for (int r = 0; r < n; ++r)
@@ -378,6 +387,158 @@ Mat *create_mat(int nrows, int ncols, int nnz, const int *rowptr, const int *col
             if (std::fabs(vals[e]) > 1e-20) ++c;
         mh->nz_count[i] = c;
     }
+    g_mat_host.push_back({m, mh});
+    return m;
+}
+
+// Q1 stiffness (which = 0) or mass (which = 1) matrix of the E^3 mesh built
+// on the device from the element coefficients (gcge_hip_assembly.hip): the
+// Mat that create_mat makes from the host assembly with every row in
+// increasing column order, entry for entry (same values, same storage
+// decisions), without a host CSR.  node_row (host, optional) renumbers rows
+// and columns: row node_row[i] holds node i.
+Mat *create_mat_q1(int E, const double *kappa, int which, const int *node_row)
+{
+    require_init();
+    if (E < 2 || (which != 0 && which != 1))
+        fail_msg("GCGE_HIP_MatCreateQ1: E must be at least 2 and which 0 (stiffness) or 1 (mass)");
+    const long N = E - 1, nl = N * N * N;
+    if (27 * nl + 64 * 27 > (long)INT32_MAX) fail_msg("GCGE_HIP_MatCreateQ1: mesh too large for int32 indices");
+    const int n = (int)nl;
+    bool natural = true;
+    if (node_row) {
+        std::vector<char> seen(n, 0);
+        for (int i = 0; i < n; ++i) {
+            const int r = node_row[i];
+            if (r < 0 || r >= n || seen[r]) fail_msg("GCGE_HIP_MatCreateQ1: node_row is not a permutation of the nodes");
+            seen[r] = 1;
+            if (r != i) natural = false;
+        }
+    }
+    State &st = state();
+    const double h = 1.0 / E;
+    double *d_kappa = nullptr;
+    int *d_node_row = nullptr;
+    if (which == 0 && kappa) {
+        const size_t ne = (size_t)E * E * E;
+        HIP_CHECK(hipMalloc((void **)&d_kappa, ne * sizeof(double)));
+        HIP_CHECK(hipMemcpyAsync(d_kappa, kappa, ne * sizeof(double), hipMemcpyHostToDevice, st.stream));
+    }
+    if (!natural) {
+        HIP_CHECK(hipMalloc((void **)&d_node_row, (size_t)n * sizeof(int)));
+        HIP_CHECK(hipMemcpyAsync(d_node_row, node_row, (size_t)n * sizeof(int), hipMemcpyHostToDevice, st.stream));
+    }
+    const Q1Args q{E, which, which == 0 ? h / 36.0 : std::pow(h, 3.0) / 216.0, d_kappa, d_node_row};
+    // pass 1: which stencil slots each row keeps
+    int *d_mask = nullptr;
+    HIP_CHECK(hipMalloc((void **)&d_mask, 2 * (size_t)n * sizeof(int)));
+    q1_row_masks(q, d_mask, d_mask + n);
+    std::vector<int> mask(2 * (size_t)n);
+    HIP_CHECK(hipMemcpyAsync(mask.data(), d_mask, mask.size() * sizeof(int), hipMemcpyDeviceToHost, st.stream));
+    HIP_CHECK(hipStreamSynchronize(st.stream));
+    std::vector<int> rowptr((size_t)n + 1, 0);
+    for (int r = 0; r < n; ++r) rowptr[r + 1] = rowptr[r] + __builtin_popcount((unsigned)mask[r]);
+    const int nnz = rowptr[n];
+
+    Mat *m = (Mat *)calloc(1, sizeof(Mat));
+    m->magic = kMatMagic;
+    m->nrows = n;
+    m->ncols = n;
+    m->nnz = nnz;
+    const int ns = (n + 63) / 64;
+    m->nslices = ns;
+    std::vector<int> sptr(ns + 1, 0), sw(ns, 0);
+    for (int sl = 0; sl < ns; ++sl) {
+        int w = 0;
+        for (int r = sl * 64; r < std::min(n, sl * 64 + 64); ++r) w = std::max(w, rowptr[r + 1] - rowptr[r]);
+        sw[sl] = w;
+        sptr[sl + 1] = sptr[sl] + w * 64;
+    }
+    const size_t len = (size_t)sptr[ns];
+    m->sell_len = len;
+    HIP_CHECK(hipMalloc((void **)&m->d_slice_ptr, (ns + 1) * sizeof(int)));
+    HIP_CHECK(hipMalloc((void **)&m->d_slice_w, std::max(ns, 1) * sizeof(int)));
+    HIP_CHECK(hipMalloc((void **)&m->d_col, std::max<size_t>(len, 1) * sizeof(int)));
+    HIP_CHECK(hipMalloc((void **)&m->d_val, std::max<size_t>(len, 1) * sizeof(double)));
+    HIP_CHECK(hipMalloc((void **)&m->d_csrpos, std::max<size_t>(len, 1) * sizeof(int)));
+    int *d_rowptr = nullptr;
+    HIP_CHECK(hipMalloc((void **)&d_rowptr, ((size_t)n + 1) * sizeof(int)));
+    HIP_CHECK(hipMemcpyAsync(d_rowptr, rowptr.data(), ((size_t)n + 1) * sizeof(int), hipMemcpyHostToDevice, st.stream));
+    HIP_CHECK(hipMemcpyAsync(m->d_slice_ptr, sptr.data(), (ns + 1) * sizeof(int), hipMemcpyHostToDevice, st.stream));
+    if (ns) HIP_CHECK(hipMemcpyAsync(m->d_slice_w, sw.data(), ns * sizeof(int), hipMemcpyHostToDevice, st.stream));
+
+    // diagonal storage: the offsets the rows hold (rows are in increasing
+    // column order by construction)
+    std::vector<int> offs;
+    bool dia_ok = nnz > 0;
+    if (dia_ok && natural) {
+        int present = 0;
+        for (int r = 0; r < n; ++r) present |= mask[r];
+        for (int o = 0; o < 27; ++o)
+            if (present >> o & 1) {
+                const int d = (o % 3 - 1) + (int)N * ((o / 3) % 3 - 1) + (int)(N * N) * (o / 9 - 1);
+                if (std::find(offs.begin(), offs.end(), d) == offs.end()) offs.push_back(d);
+            }
+    } else if (dia_ok) {
+        size_t hint = 0;
+        for (int i = 0; i < n && dia_ok; ++i) {
+            const int row = node_row[i], mk = mask[row];
+            for (int o = 0; o < 27 && dia_ok; ++o) {
+                if (!(mk >> o & 1)) continue;
+                const long nb = i + (o % 3 - 1) + N * ((o / 3) % 3 - 1) + N * N * (o / 9 - 1);
+                const int d = node_row[nb] - row;
+                if (hint < offs.size() && offs[hint] == d) {
+                    ++hint;
+                    continue;
+                }
+                auto it = std::find(offs.begin(), offs.end(), d);
+                if (it == offs.end()) {
+                    if ((int)offs.size() == 3 * kDiaMaxGroups) dia_ok = false;
+                    offs.push_back(d);
+                    hint = offs.size();
+                } else {
+                    hint = (size_t)(it - offs.begin()) + 1;
+                }
+            }
+        }
+    }
+    std::sort(offs.begin(), offs.end());
+    DiaDesc dd{};
+    DiaOffsets doffs{};
+    const long dia_ld = (long)ns * 64;
+    if (dia_ok && dia_layout(offs, nnz, n, &dd)) {
+        doffs.nd = (int)offs.size();
+        for (int i = 0; i < doffs.nd; ++i) doffs.off[i] = offs[i];
+        HIP_CHECK(hipMalloc((void **)&m->d_dia, (size_t)doffs.nd * dia_ld * sizeof(double)));
+        HIP_CHECK(hipMemsetAsync(m->d_dia, 0, (size_t)doffs.nd * dia_ld * sizeof(double), st.stream));
+        m->dia = dd;
+        m->ndiag = doffs.nd;
+        m->use_dia = 1;
+    }
+    // pass 2: the entries
+    sell_pad(n, n, ns, m->d_slice_ptr, m->d_slice_w, m->d_col, m->d_val, m->d_csrpos);
+    q1_fill(q, d_rowptr, m->d_slice_ptr, m->d_col, m->d_val, m->d_csrpos, doffs, m->d_dia, dia_ld);
+    // column spread of the entry rows (scattered-column products)
+    int *d_spread = nullptr;
+    std::vector<int> spread(2 * (size_t)std::max(ns, 1), 0);
+    HIP_CHECK(hipMalloc((void **)&d_spread, spread.size() * sizeof(int)));
+    sell_spread(ns, m->d_slice_ptr, m->d_slice_w, m->d_col, m->d_csrpos, d_spread);
+    HIP_CHECK(hipMemcpyAsync(spread.data(), d_spread, spread.size() * sizeof(int), hipMemcpyDeviceToHost, st.stream));
+    HIP_CHECK(hipStreamSynchronize(st.stream));
+    long tight = 0, total = 0;
+    for (int sl = 0; sl < ns; ++sl) {
+        tight += spread[2 * (size_t)sl];
+        total += spread[2 * (size_t)sl + 1];
+    }
+    m->irregular = (total > 0 && 2 * tight < total) ? 1 : 0;
+    m->use_rows = m->irregular;
+    HIP_CHECK(hipFree(d_spread));
+    HIP_CHECK(hipFree(d_rowptr));
+    HIP_CHECK(hipFree(d_mask));
+    if (d_node_row) HIP_CHECK(hipFree(d_node_row));
+    if (d_kappa) HIP_CHECK(hipFree(d_kappa));
+    MatHost *mh = new MatHost;
+    mh->nz_count.assign(mask.begin() + n, mask.end());
     g_mat_host.push_back({m, mh});
     return m;
 }
@@ -1376,6 +1537,40 @@ void *GCGE_HIP_MatCreateCSR(int nrows, int ncols, int nnz, const int *rowptr, co
                             const double *vals)
 {
     return create_mat(nrows, ncols, nnz, rowptr, colidx, vals);
+}
+
+void *GCGE_HIP_MatCreateQ1(int E, const double *kappa, int which, const int *node_row)
+{
+    return create_mat_q1(E, kappa, which, node_row);
+}
+
+int GCGE_HIP_MatGetCSR(void *mat, int *rowptr, int *colidx, double *vals)
+{
+    Mat *m = as_mat(mat);
+    if (!rowptr && !colidx && !vals) return m->nnz;
+    State &st = state();
+    int *d_col = nullptr, *d_len = nullptr;
+    double *d_val = nullptr;
+    HIP_CHECK(hipMalloc((void **)&d_col, std::max(m->nnz, 1) * sizeof(int)));
+    HIP_CHECK(hipMalloc((void **)&d_val, std::max(m->nnz, 1) * sizeof(double)));
+    HIP_CHECK(hipMalloc((void **)&d_len, std::max(m->nrows, 1) * sizeof(int)));
+    sell_to_csr(m, d_col, d_val, d_len);
+    std::vector<int> rowlen(m->nrows);
+    if (m->nrows)
+        HIP_CHECK(hipMemcpyAsync(rowlen.data(), d_len, m->nrows * sizeof(int), hipMemcpyDeviceToHost, st.stream));
+    if (colidx && m->nnz)
+        HIP_CHECK(hipMemcpyAsync(colidx, d_col, (size_t)m->nnz * sizeof(int), hipMemcpyDeviceToHost, st.stream));
+    if (vals && m->nnz)
+        HIP_CHECK(hipMemcpyAsync(vals, d_val, (size_t)m->nnz * sizeof(double), hipMemcpyDeviceToHost, st.stream));
+    HIP_CHECK(hipStreamSynchronize(st.stream));
+    if (rowptr) {
+        rowptr[0] = 0;
+        for (int r = 0; r < m->nrows; ++r) rowptr[r + 1] = rowptr[r] + rowlen[r];
+    }
+    HIP_CHECK(hipFree(d_len));
+    HIP_CHECK(hipFree(d_val));
+    HIP_CHECK(hipFree(d_col));
+    return m->nnz;
 }
 
 void GCGE_HIP_MatDestroy(void *mat)

@@ -243,4 +243,36 @@ void pattern_gradient(const Mat *A, const double *X, long ldx, const double *Y, 
 void q1_element_gradient(int E, const double *X, long ldx, const double *Y, long ldy, int k,
                          const double *d_w, double *d_grad, const int *d_node_row = nullptr);
 
+// ---- Q1 assembly and sliced-ELLPACK helpers (gcge_hip_assembly.hip) -----------------------
+// which = 0: stiffness, entries (sum of 36 Khat kappa_e) * scale, exact zeros
+// dropped; which = 1: mass, (count * 216 Mhat) * scale.  kappa: E^3 device
+// doubles (element ex + E ey + E^2 ez) or null (kappa = 1); node_row: device
+// map node -> row (null: natural order).
+struct Q1Args {
+    int E, which;
+    double scale;
+    const double *kappa;
+    const int *node_row;
+};
+// offsets col - row of diagonal storage, sorted (nd <= 27)
+struct DiaOffsets {
+    int nd;
+    int off[27];
+};
+// per row: bit o of d_mask = slot o kept, d_big = kept entries with |a| > 1e-20
+void q1_row_masks(const Q1Args &q, int *d_mask, int *d_big);
+// kept entries of each row at their sliced-ELLPACK positions (columns
+// increasing, d_pos = CSR position), and in diagonal storage when d_dia
+void q1_fill(const Q1Args &q, const int *d_rowptr, const int *d_sptr, int *d_col, double *d_val,
+             int *d_pos, const DiaOffsets &dia, double *d_dia, long dia_ld);
+// padding of every slice: zero in the row's own column, CSR position -1
+void sell_pad(int nrows, int ncols, int nslices, const int *d_sptr, const int *d_sw, int *d_col,
+              double *d_val, int *d_pos);
+// d_out[2 s], d_out[2 s + 1]: entries of slice s in entry rows spanning
+// < 256 columns, and all its stored entries
+void sell_spread(int nslices, const int *d_sptr, const int *d_sw, const int *d_col,
+                 const int *d_pos, int *d_out);
+// stored entries back at their CSR positions, entries per row
+void sell_to_csr(const Mat *A, int *d_col, double *d_val, int *d_rowlen);
+
 }  // namespace gcge_hip

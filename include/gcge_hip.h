@@ -81,6 +81,25 @@ int GCGE_HIP_MatSetStorage(void *mat, int storage);
  * returns the storage the products use (0, 1 or 2 as above) */
 int GCGE_HIP_MatStorageInfo(void *mat, int *ndiag, int *ngroups);
 
+/* Q1 finite-element matrices assembled on the device (no counterpart in the
+ * reference, whose drivers read assembled matrices from files): the uniform
+ * E^3 mesh of the unit cube, trilinear elements, Dirichlet nodes eliminated,
+ * n = (E-1)^3 interior nodes numbered (ix-1) + N(iy-1) + N^2(iz-1), N = E-1.
+ * which = 0: stiffness K = sum_e kappa_e h Khat (kappa: E^3 host doubles in
+ * element order ex + E ey + E^2 ez, NULL for kappa = 1; entries that come out
+ * exactly zero, such as every face-neighbour entry, are not stored);
+ * which = 1: consistent mass M = sum_e h^3 Mhat (kappa ignored).  node_row
+ * (host, n ints, optional) renumbers rows and columns: row node_row[i] holds
+ * node i; it must be a permutation (else an error).  Rows list their columns
+ * in increasing order.  The result equals, entry for entry and bit for bit,
+ * GCGE_HIP_MatCreateCSR of the host assembly that adds each entry's element
+ * terms in the order of the element offsets (x slowest) and scales once. */
+void *GCGE_HIP_MatCreateQ1(int E, const double *kappa, int which, const int *node_row);
+/* Copy a matrix back in CSR form (rowptr[nrows+1], colidx[nnz], vals[nnz];
+ * any may be NULL), entries in the order they were given or assembled.
+ * Returns nnz. */
+int GCGE_HIP_MatGetCSR(void *mat, int *rowptr, int *colidx, double *vals);
+
 /* ---- multivecs (replace the default MultiVecCreateByMat of per-vector
  *      allocations, reference src/gcge_ops.c:72-80) ----------------------- */
 /* Returns a void** handle usable wherever the table expects a multivec. */

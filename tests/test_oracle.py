@@ -156,6 +156,55 @@ def test_problem_generators_closed_form():
     assert (3 * 4 - 2) ** 3 == M.nnz
 
 
+def test_q1_assembly_against_quadrature_element_loop():
+    """q1_fem_3d with random kappa against an element-by-element dense
+    assembly whose element matrices come from 2-point Gauss quadrature of the
+    trilinear shape functions (exact for these integrands), Dirichlet nodes
+    removed afterwards: pins the element numerators, the kappa indexing
+    ex + E ey + E^2 ez and the node numbering, which the kappa = 1 spectrum
+    cannot."""
+    import itertools
+
+    E = 4
+    h = 1.0 / E
+    kappa = problems.lognormal_kappa(E, 7)
+    g = np.array([0.5 - 0.5 / np.sqrt(3.0), 0.5 + 0.5 / np.sqrt(3.0)])
+    corners = list(itertools.product((0, 1), repeat=3))  # (ax, ay, az)
+
+    def phi(a, xi):
+        return np.prod([xi[d] if a[d] else 1.0 - xi[d] for d in range(3)])
+
+    def dphi(a, xi):
+        out = np.empty(3)
+        for d in range(3):
+            f = 1.0 if a[d] else -1.0
+            out[d] = f * np.prod([xi[e] if a[e] else 1.0 - xi[e] for e in range(3) if e != d])
+        return out
+
+    Ke = np.zeros((8, 8))
+    Me = np.zeros((8, 8))
+    for xi in itertools.product(g, repeat=3):
+        for p, a in enumerate(corners):
+            for q, b in enumerate(corners):
+                Ke[p, q] += 0.125 * dphi(a, xi) @ dphi(b, xi)  # weights 1/2 per axis
+                Me[p, q] += 0.125 * phi(a, xi) * phi(b, xi)
+    nodes = E + 1
+    Kd = np.zeros((nodes ** 3, nodes ** 3))
+    Md = np.zeros_like(Kd)
+    for ez, ey, ex in itertools.product(range(E), repeat=3):
+        ids = [(ex + a[0]) + nodes * (ey + a[1]) + nodes * nodes * (ez + a[2]) for a in corners]
+        ke = kappa[ex + E * ey + E * E * ez]
+        Kd[np.ix_(ids, ids)] += ke * h * Ke  # stiffness scales by h in 3D
+        Md[np.ix_(ids, ids)] += h ** 3 * Me
+    r = np.arange(1, E)
+    inner = [x + nodes * y + nodes * nodes * z for z in r for y in r for x in r]
+    K, M = problems.q1_fem_3d(E, kappa)
+    np.testing.assert_allclose(K.to_scipy().toarray(), Kd[np.ix_(inner, inner)], rtol=0,
+                               atol=1e-14 * np.abs(Kd).max())
+    np.testing.assert_allclose(M.to_scipy().toarray(), Md[np.ix_(inner, inner)], rtol=0,
+                               atol=1e-14 * np.abs(Md).max())
+
+
 RUN_CASES = {
     "A5M5_nev20": (("A_5", "M_5"), 20, ["-gcge_block_size", "10", "-gcge_ev_max_it", "40"], 7, True),
     "A5M5_nev40": (("A_5", "M_5"), 40, ["-gcge_ev_max_it", "60"], 7, True),
