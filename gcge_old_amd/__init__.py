@@ -113,11 +113,16 @@ class DeviceMatrix:
         """Products through "diagonal" storage (when the matrix has it),
         "sliced" ELLPACK with element gathers, or sliced ELLPACK with "rows"
         gathers from a row-major copy of the column block (from 16 columns);
-        True/False mean "diagonal"/"sliced".  Returns whether diagonal storage
-        is now used."""
+        True/False mean "diagonal"/"sliced".  "diagonal-consecutive" and
+        "diagonal-planes" fix the order of the diagonal row blocks (same
+        results bit for bit; "diagonal" chooses).  Returns whether diagonal
+        storage is now used."""
         if storage is True or storage is False:
             storage = "diagonal" if storage else "sliced"
-        used = _native.hip().GCGE_HIP_MatSetStorage(self.handle, self.STORAGES.index(storage))
+        code = {"diagonal-consecutive": 3, "diagonal-planes": 4}.get(storage)
+        if code is None:
+            code = self.STORAGES.index(storage)
+        used = _native.hip().GCGE_HIP_MatSetStorage(self.handle, code)
         return used == 1
 
     def storage_info(self) -> dict:
@@ -125,7 +130,8 @@ class DeviceMatrix:
         used = _native.hip().GCGE_HIP_MatStorageInfo(self.handle, _native.as_iptr(nd),
                                                      _native.as_iptr(ng))
         return {"diagonal": used == 1, "storage": self.STORAGES[used], "ndiag": int(nd[0]),
-                "runs": int(ng[0])}
+                "runs": int(ng[0]),
+                "plane_rows": int(_native.hip().GCGE_HIP_MatDiagonalPlane(self.handle))}
 
     def __del__(self):
         if getattr(self, "handle", None):

@@ -77,6 +77,8 @@ def _declare(name: str, lib: C.CDLL) -> None:
         lib.GCGE_HIP_MatSetStorage.restype = C.c_int
         lib.GCGE_HIP_MatStorageInfo.argtypes = [C.c_void_p, iptr, iptr]
         lib.GCGE_HIP_MatStorageInfo.restype = C.c_int
+        lib.GCGE_HIP_MatDiagonalPlane.argtypes = [C.c_void_p]
+        lib.GCGE_HIP_MatDiagonalPlane.restype = C.c_int
         lib.GCGE_HIP_MultiVecCreate.argtypes = [C.c_int, C.c_int]
         lib.GCGE_HIP_MultiVecCreate.restype = C.c_void_p
         lib.GCGE_HIP_MultiVecDestroy.argtypes = [C.c_void_p]

@@ -252,6 +252,29 @@ static bool dia_layout(const std::vector<int> &offs, long nnz, int n, DiaDesc *d
     return true;
 }
 
+// Plane stride of a diagonal layout whose 9 run centres lie within one of
+// a N + b S for a, b in {-1, 0, 1}, S > 2 N + 2 dividing the row count (the
+// 27-point stencil of a lexicographically numbered N1 x N2 x N3 grid,
+// S = N1 N2; a run may be centred one off when its middle diagonal is
+// absent, as the face-neighbour diagonals of a Q1 stiffness matrix are),
+// else 0.  The stride only orders the work: any S dividing the row count
+// gives the same products.
+static int dia_plane_stride(const DiaDesc &dd, int n)
+{
+    if (dd.ngroups != 9) return 0;
+    for (int dS = -1; dS <= 1; ++dS)
+        for (int dN = -1; dN <= 1; ++dN) {
+            const int N = dd.center[5] + dN, S = dd.center[7] + dS;
+            if (N <= 0 || S <= 2 * N + 2 || n % S != 0) continue;
+            bool ok = true;
+            for (int b = -1; b <= 1 && ok; ++b)
+                for (int a = -1; a <= 1 && ok; ++a)
+                    ok = std::abs(dd.center[(b + 1) * 3 + (a + 1)] - (a * N + b * S)) <= 1;
+            if (ok) return S;
+        }
+    return 0;
+}
+
 static void build_dia(Mat *m, const int *rowptr, const int *colidx, const double *vals)
 {
     const int n = m->nrows;
@@ -295,6 +318,7 @@ static void build_dia(Mat *m, const int *rowptr, const int *colidx, const double
                              state().stream));
     HIP_CHECK(hipStreamSynchronize(state().stream));
     m->dia = dd;
+    m->dia_plane = dia_plane_stride(dd, n);
     m->ndiag = nd;
     m->use_dia = 1;
 }
@@ -512,6 +536,7 @@ Mat *create_mat_q1(int E, const double *kappa, int which, const int *node_row)
         HIP_CHECK(hipMalloc((void **)&m->d_dia, (size_t)doffs.nd * dia_ld * sizeof(double)));
         HIP_CHECK(hipMemsetAsync(m->d_dia, 0, (size_t)doffs.nd * dia_ld * sizeof(double), st.stream));
         m->dia = dd;
+        m->dia_plane = dia_plane_stride(dd, n);
         m->ndiag = doffs.nd;
         m->use_dia = 1;
     }
@@ -1618,9 +1643,16 @@ int GCGE_HIP_MatSetStorage(void *mat, int storage)
 {
     Mat *m = as_mat(mat);
     HIP_CHECK(hipStreamSynchronize(state().stream));
-    m->use_dia = (storage == 1 && m->d_dia) ? 1 : 0;
+    m->use_dia = ((storage == 1 || storage == 3 || storage == 4) && m->d_dia) ? 1 : 0;
     m->use_rows = storage == 2 ? 1 : 0;
+    m->dia_order = storage == 3 ? 1 : (storage == 4 ? 2 : 0);
     return m->use_dia ? 1 : (m->use_rows ? 2 : 0);
+}
+
+int GCGE_HIP_MatDiagonalPlane(void *mat)
+{
+    Mat *m = as_mat(mat);
+    return m->use_dia ? dia_plane_rows(m) : 0;
 }
 
 int GCGE_HIP_MatStorageInfo(void *mat, int *ndiag, int *ngroups)

@@ -107,6 +107,11 @@ struct Mat {
     // of three): d_dia[s * nslices * 64 + row] = A(row, row + offset s), zero
     // where the row has no such entry; used by the SpMM when use_dia is set
     DiaDesc dia;
+    int dia_plane;  // rows per plane of a 27-point pattern (centers a N + b S,
+                    // a, b in {-1, 0, 1}, S = dia_plane dividing nrows), else 0
+    int dia_order;  // row blocks of the diagonal products: 0 automatic (plane
+                    // order from a plane stride of 8192 rows), 1 consecutive,
+                    // 2 plane order whenever dia_plane > 0
     int ndiag;
     double *d_dia;
     float *d_dia32;
@@ -142,6 +147,9 @@ constexpr int kAlign = 64;
 inline long padded_ld(int n) { return ((long)n + kAlign - 1) / kAlign * kAlign; }
 MultiVec *as_mv(void **handle);
 Vec *as_vec(void *handle);
+
+// rows per plane the diagonal products use (0: consecutive row blocks)
+int dia_plane_rows(const Mat *A);
 
 // ---- kernel launchers (stream-ordered, asynchronous) ---------------------------------
 // Y(:, ymap[c]) = A X(:, xmap[c]) for c < k; maps may be null (identity).

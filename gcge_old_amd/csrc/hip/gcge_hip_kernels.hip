@@ -385,6 +385,16 @@ __device__ __forceinline__ void dia_chunk(const XT (&v)[kDiaMaxGroups][3], const
     }
 }
 
+// Plane order (S > 0): the rows form planes of S rows, each cut into pb row
+// blocks (the last one partial); workgroup (q, g) takes in-plane block q of
+// the kBlocksPerGroup consecutive planes of plane group g, q varying fastest.
+// The waves of a workgroup read the same rows of X for the runs one plane
+// apart close together in time, and neighbouring in-plane blocks run
+// alongside as in consecutive order.
+struct DiaPlanes {
+    int S, pb, planes, groups;
+};
+
 template <typename VT, typename XT, int CK, int SPLIT, int DOT>
 __global__ __launch_bounds__(kBlock) void k_dia_spmm(int n, int nblocks, DiaDesc dd,
                                                     const VT *__restrict__ dval, long ldd,
@@ -393,13 +403,23 @@ __global__ __launch_bounds__(kBlock) void k_dia_spmm(int n, int nblocks, DiaDesc
                                                     const int *__restrict__ xmap,
                                                     const int *__restrict__ ymap,
                                                     double *__restrict__ dpart,
-                                                    const int *__restrict__ count)
+                                                    const int *__restrict__ count, DiaPlanes pm)
 {
     constexpr int kBlocksPerGroup = kBlock / kWave / SPLIT;
     const int blk = xcd_remap(blockIdx.x, gridDim.x);
     const int wid = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    const int rb = blk * kBlocksPerGroup + wid / SPLIT;
-    if (rb >= nblocks) return;
+    int rb, row0, in_plane = 0;
+    if (pm.S == 0) {
+        rb = blk * kBlocksPerGroup + wid / SPLIT;
+        if (rb >= nblocks) return;
+        row0 = rb * kDiaRows;
+    } else {
+        const int q = blk % pm.pb, z = (blk / pm.pb) * kBlocksPerGroup + wid / SPLIT;
+        if (q >= pm.pb || z >= pm.planes) return;
+        rb = z * pm.pb + q;
+        in_plane = q * kDiaRows;
+        row0 = z * pm.S + in_plane;
+    }
     const int part = wid % SPLIT;
     const int kc = count ? min(k, *count) : k;  // columns listed on the device
     const int gb = blockIdx.y * gcols, ge = min(kc, gb + gcols);
@@ -407,8 +427,9 @@ __global__ __launch_bounds__(kBlock) void k_dia_spmm(int n, int nblocks, DiaDesc
     const int per = (ge - gb + SPLIT - 1) / SPLIT;
     const int cb = gb + part * per, ce = min(ge, cb + per);
     if (cb >= ce) return;
-    const int row = rb * kDiaRows + lane - 1;
-    const bool own = lane >= 1 && lane <= kDiaRows && row < n;
+    const int row = row0 + lane - 1;
+    const bool own = lane >= 1 && lane <= kDiaRows && row < n &&
+                     (pm.S == 0 || in_plane + lane - 1 < pm.S);
     XT v[kDiaMaxGroups][3];
 #pragma unroll
     for (int g = 0; g < kDiaMaxGroups; ++g)
@@ -422,7 +443,21 @@ __global__ __launch_bounds__(kBlock) void k_dia_spmm(int n, int nblocks, DiaDesc
                            min(CK, ce - c0), xmap, ymap, dpart);
 }
 
-inline int dia_blocks(const Mat *A) { return (A->nrows + kDiaRows - 1) / kDiaRows; }
+// Plane order (automatic choice) for layouts with a plane stride of at least
+// 8192 rows; at smaller strides consecutive row blocks already share X
+// through L2.
+inline int plane_rows(const Mat *A)
+{
+    if (A->dia_plane <= 0 || A->dia_order == 1) return 0;
+    return (A->dia_order == 2 || A->dia_plane >= 8192) ? A->dia_plane : 0;
+}
+inline int dia_plane_blocks(int S) { return (S + kDiaRows - 1) / kDiaRows; }
+inline int dia_blocks(const Mat *A)
+{
+    const int S = plane_rows(A);
+    if (S) return A->nrows / S * dia_plane_blocks(S);
+    return (A->nrows + kDiaRows - 1) / kDiaRows;
+}
 
 // Column groups (blockIdx.y) of gcols columns, SPLIT waves per row block, CK
 // columns per register chunk.  GCGE_HIP_DIA_GROUP / GCGE_HIP_DIA_SPLIT
@@ -454,13 +489,20 @@ void launch_dia_spmm(const Mat *A, const VT *vals, const XT *X, long ldx, XT *Y,
     const int ng = (k + gcols - 1) / gcols;
     const int nblocks = dia_blocks(A);
     const int bpg = kBlock / kWave / split;
+    DiaPlanes pm{plane_rows(A), 0, 0, 0};
     int nb = (nblocks + bpg - 1) / bpg;
+    if (pm.S) {
+        pm.pb = dia_plane_blocks(pm.S);
+        pm.planes = A->nrows / pm.S;
+        pm.groups = (pm.planes + bpg - 1) / bpg;
+        nb = pm.pb * pm.groups;
+    }
     if (ng > 1) nb = (nb + kXcds - 1) / kXcds * kXcds;
     const long ldd = (long)A->nslices * kWave;
 #define GCGE_DIA_LAUNCH(CK, SP)                                                                      \
     hipExtLaunchKernelGGL((k_dia_spmm<VT, XT, CK, SP, DOT>), dim3(nb, ng), dim3(kBlock), 0,          \
                           state().stream, ev0, ev1, 0, A->nrows, nblocks, A->dia, vals, ldd, X, ldx,  \
-                          Y, ldy, k, gcols, d_xmap, d_ymap, dpart, d_count)
+                          Y, ldy, k, gcols, d_xmap, d_ymap, dpart, d_count, pm)
     const int ck_auto = sizeof(XT) == 8 ? 2 : (per <= 2 ? 2 : 4);
     const int ckw = forced_ck == 2 || forced_ck == 4 || forced_ck == 8 ? forced_ck : ck_auto;
 #define GCGE_DIA_CK(SP)                                     \
@@ -1448,6 +1490,8 @@ inline int reduction_rows(int n, int k)
 }
 
 }  // namespace
+
+int dia_plane_rows(const Mat *A) { return plane_rows(A); }
 
 // ---- launchers ---------------------------------------------------------------------------
 

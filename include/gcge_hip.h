@@ -80,6 +80,16 @@ int GCGE_HIP_MatSetStorage(void *mat, int storage);
 /* number of stored diagonals and of runs (0, 0 without diagonal storage);
  * returns the storage the products use (0, 1 or 2 as above) */
 int GCGE_HIP_MatStorageInfo(void *mat, int *ndiag, int *ngroups);
+/* Diagonal storage walks its row blocks in one of two orders, with the same
+ * result bit for bit: consecutive blocks, or, for the 27-point pattern of a
+ * lexicographically numbered grid (run centres a N + b S, a, b in {-1, 0, 1},
+ * S rows per plane dividing the row count), the same in-plane block of
+ * consecutive planes together, so that the rows of X the plane runs share are
+ * read from HBM once.  GCGE_HIP_MatSetStorage with storage 1 picks the order
+ * (plane order from S = 8192 rows), 3 forces consecutive blocks, 4 plane
+ * order where the pattern exists; both return 1.  Returns the plane stride
+ * the products use, 0 for consecutive blocks or other storages. */
+int GCGE_HIP_MatDiagonalPlane(void *mat);
 
 /* Q1 finite-element matrices assembled on the device (no counterpart in the
  * reference, whose drivers read assembled matrices from files): the uniform

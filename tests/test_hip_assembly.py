@@ -141,3 +141,50 @@ def test_device_assembly_rejects_bad_arguments():
         DeviceMatrix.q1(4, np.ones(5), "stiffness")
     with pytest.raises(ValueError):
         DeviceMatrix.q1(4, None, "damping")
+
+
+# ---- diagonal products in plane order ------------------------------------------------------
+
+
+@pytest.mark.parametrize("E,which", [(93, "stiffness"), (10, "stiffness"), (12, "mass")])
+def test_plane_order_products_equal_consecutive_order(E, which):
+    """The two row-block orders of the diagonal products compute every row
+    with the same arithmetic: equal bit for bit, for every column count
+    (E = 93: plane stride 92^2 = 8464 rows, the order chosen by default)"""
+    n = (E - 1) ** 3
+    kap = problems.lognormal_kappa(E, 2) if which == "stiffness" else None
+    dA = DeviceMatrix.q1(E, kap, which)
+    S = (E - 1) ** 2
+    if S >= 8192:
+        assert dA.storage_info()["plane_rows"] == S
+    for k in (1, 3, 10, 37):
+        X = np.random.default_rng(k).standard_normal((n, k))
+        dA.set_storage("diagonal-planes")
+        assert dA.storage_info()["plane_rows"] == S
+        y_planes = _spmm(dA, X)
+        dA.set_storage("diagonal-consecutive")
+        assert dA.storage_info()["plane_rows"] == 0
+        y_cons = _spmm(dA, X)
+        assert _same_bits(y_planes, y_cons)
+    dA.set_storage("sliced")
+    assert dA.storage_info()["plane_rows"] == 0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_plane_order_solve(precision):
+    """solves through the block CG (products with the p.w epilogue, float32
+    in the mixed form) agree in both orders"""
+    import gcge_old_amd as g
+
+    E, nev = 12, 10
+    flags = ["-gcge_conv_type", "RB", "-gcge_ev_tol", "1e-10", "-gcge_ev_max_it", "100"]
+    out = {}
+    for order in ("diagonal-planes", "diagonal-consecutive"):
+        K = DeviceMatrix.q1(E, problems.lognormal_kappa(E, 4), "stiffness")
+        M = DeviceMatrix.q1(E, None, "mass")
+        K.set_storage(order)
+        M.set_storage(order)
+        out[order] = g.solve(K, M, nev, flags, seed=3, precision=precision)
+        assert out[order]["num_conv"] == nev
+    a, b = out["diagonal-planes"]["eval"], out["diagonal-consecutive"]["eval"]
+    np.testing.assert_allclose(a, b, rtol=1e-10)
