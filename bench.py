@@ -552,14 +552,22 @@ def main():
     if args.kernels:
         kernel_sweep(K, M)
 
+    warm_launches = 0
     for w in range(args.warmup):
+        lib.GCGE_HIP_ProfileReset()
+        lib.GCGE_HIP_ProfileEnable(0 if args.no_profile else 2)
         out = gcge.solve(dK, dM, args.nev, flags, seed=100 + w)
+        lib.GCGE_HIP_ProfileEnable(0)
+        warm_launches = max(warm_launches, kernel_stats(lib).get("spmm", {}).get("launches", 0))
         print(f"# warmup {w}: {out['time']:.4f}s iters={out['num_iter']} conv={out['num_conv']}",
               file=sys.stderr, flush=True)
 
     # timed region: events on the SpMM launches only (recorded by the
-    # dispatches themselves), so the solve time carries no profiling packets
+    # dispatches themselves), so the solve time carries no profiling packets;
+    # the events are created beforehand (two per launch, with a margin)
     lib.GCGE_HIP_ProfileReset()
+    if not args.no_profile:
+        lib.GCGE_HIP_ProfileReserve(int(2 * 1.5 * max(warm_launches, 2000) * args.steps))
     lib.GCGE_HIP_ProfileEnable(0 if args.no_profile else 2)
     lib.GCGE_HIP_Synchronize()
     times, iters, outs = [], [], []

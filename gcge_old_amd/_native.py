@@ -110,6 +110,7 @@ def _declare(name: str, lib: C.CDLL) -> None:
         lib.GCGE_HIP_KernelStats.restype = C.c_int
         lib.GCGE_HIP_ProfileEnable.argtypes = [C.c_int]
         lib.GCGE_HIP_ProfileReset.argtypes = []
+        lib.GCGE_HIP_ProfileReserve.argtypes = [C.c_int]
 
 
 def as_dptr(a):

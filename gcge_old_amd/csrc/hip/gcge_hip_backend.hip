@@ -1091,12 +1091,13 @@ static void bcg_iterate_dev(Mat *A, int n, int len, PT *Rd, long ldr, int r0, do
         else
             bcg_p_update(n, Pd, ldp, pm, Rd, ldr, rm, c.beta, it == 1, len, c.count);
         spmm_dot(A, Pd, ldp, Wd, ldw, len, pm, wm, d_ptw, c.count, &slots[it - 1]);
+        // r.r partials go straight to the bookkeeping kernel (no reduction launch)
         if constexpr (scaled)
             bcg_xr_update(n, Xd, ldx, xm, Rd, ldr, rm, Pd, ldp, pm, Wd, ldw, wm, c.alpha_num, d_ptw,
-                          c.s_new, len, d_rr, c.count);
+                          c.s_new, len, d_rr, c.count, &c.rr_part, &c.rr_nbx);
         else
             bcg_xr_update(n, Xd, ldx, xm, Rd, ldr, rm, Pd, ldp, pm, Wd, ldw, wm, c.alpha_num, d_ptw,
-                          len, d_rr, c.count);
+                          len, d_rr, c.count, &c.rr_part, &c.rr_nbx);
         bcg_control(c, 0, it, it + 1 <= steps ? it + 1 : 0);
     }
     // state back: rho1, rho2, err, last_err; unlock, count, step sizes
@@ -1802,6 +1803,16 @@ void GCGE_HIP_ProfileEnable(int on)
 {
     state().profiling = on != 0;
     state().profile_mode = on;
+}
+
+void GCGE_HIP_ProfileReserve(int events)
+{
+    require_init();
+    while ((int)g_event_pool.size() < events) {
+        hipEvent_t e;
+        HIP_CHECK(hipEventCreate(&e));
+        g_event_pool.push_back(e);
+    }
 }
 
 void GCGE_HIP_ProfileReset(void)

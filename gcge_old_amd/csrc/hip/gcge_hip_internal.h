@@ -208,11 +208,16 @@ void coldots_pw(int n, const float *P, long ldp, const int *d_pmap, const float 
 void bcg_xr_update(int n, double *X, long ldx, const int *d_xmap, double *R, long ldr,
                    const int *d_rmap, const double *P, long ldp, const int *d_pmap,
                    const double *W, long ldw, const int *d_wmap, const double *d_rho2,
-                   const double *d_ptw, int k, double *d_rr_out, const int *d_count = nullptr);
+                   const double *d_ptw, int k, double *d_rr_out, const int *d_count = nullptr,
+                   const double **rr_part = nullptr, int *rr_nbx = nullptr);
 void bcg_xr_update(int n, double *X, long ldx, const int *d_xmap, float *R, long ldr,
                    const int *d_rmap, const float *P, long ldp, const int *d_pmap, const float *W,
                    long ldw, const int *d_wmap, const double *d_rho2, const double *d_ptw,
-                   const double *d_scale, int k, double *d_rr_out, const int *d_count = nullptr);
+                   const double *d_scale, int k, double *d_rr_out, const int *d_count = nullptr,
+                   const double **rr_part = nullptr, int *rr_nbx = nullptr);
+// (r.r per column: rr_part given, the partial sums rr_part[c * rr_nbx + b]
+// are left for the bookkeeping kernel instead of being summed into d_rr_out;
+// both sum them the same way, lane-strided over one wavefront per column)
 // Block-CG bookkeeping on the device (one workgroup): the per-column scalars
 // and the list of unconverged columns of the reference loop
 // (src/gcge_cg.c:395-490), kept in device memory between steps so that the
@@ -223,6 +228,8 @@ struct BcgControl {
     double *beta, *alpha_num, *s_old, *s_new;               // by position, read by the steps
     const double *ptw;   // p.w of the step [len], p.p [len + id] (scaled directions)
     const double *rr;    // r.r of the step, or the initial r.r (init)
+    const double *rr_part;  // or, for a finished step, its partial sums
+    int rr_nbx;             // (rr_nbx per column, from bcg_xr_update)
     int *unlock;         // unconverged columns [len]
     int *count;          // number of entries of unlock
     int *maps;           // p, r, x, w column lists [4 len]

@@ -1197,6 +1197,26 @@ __global__ __launch_bounds__(kBlock) void k_coldots(int n, int rows_per_block,
     if (threadIdx.x == 0) partial[(long)c * gridDim.x + bx] = s;
 }
 
+// out[c] = sum of the nbx partials of column c, one wavefront per column:
+// lane l adds partials l, l + 64, ... in order, then the wavefront sum (the
+// block-CG bookkeeping kernel sums r.r partials the same way)
+__device__ __forceinline__ double wave_col_sum(const double *__restrict__ part, int nbx, int lane)
+{
+    double v = 0.0;
+    for (int b = lane; b < nbx; b += kWave) v += part[b];
+    return wave_sum(v);
+}
+__global__ __launch_bounds__(kBlock) void k_reduce_cols_wave(const double *__restrict__ partial,
+                                                            int nbx, int ncols,
+                                                            double *__restrict__ out)
+{
+    const int c = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    if (c >= ncols) return;
+    const int lane = threadIdx.x % kWave;
+    const double v = wave_col_sum(partial + (long)c * nbx, nbx, lane);
+    if (lane == 0) out[c] = v;
+}
+
 // out[c] = sum of the nbx partials of column c
 __global__ __launch_bounds__(kBlock) void k_reduce_rows(const double *__restrict__ partial,
                                                        int nbx, double *__restrict__ out)
@@ -1361,6 +1381,7 @@ __global__ __launch_bounds__(kBlock) void k_bcg_control(BcgControl d, int init, 
     __shared__ double s_rho1[kBcgControlMax], s_rho2[kBcgControlMax], s_err[kBcgControlMax],
         s_lerr[kBcgControlMax], s_scol[kBcgControlMax], s_pn[kBcgControlMax];
     __shared__ int s_unlock[kBcgControlMax];
+    __shared__ double s_rr[kBcgControlMax];
     __shared__ int s_count;
     const int t = threadIdx.x, len = d.len;
     for (int i = t; i < len; i += kBlock) {
@@ -1385,6 +1406,15 @@ __global__ __launch_bounds__(kBlock) void k_bcg_control(BcgControl d, int init, 
     if (t == 0) s_count = init ? len : *d.count;
     __syncthreads();
     const int nu = s_count;
+    if (finished_step > 0 && d.rr_part) {
+        // r.r of the step's columns from the partial sums, one wavefront per column
+        const int lane = t % kWave;
+        for (int id = t / kWave; id < nu; id += kBlock / kWave) {
+            const double v = wave_col_sum(d.rr_part + (long)id * d.rr_nbx, d.rr_nbx, lane);
+            if (lane == 0) s_rr[id] = v;
+        }
+        __syncthreads();
+    }
     if (finished_step > 0) {
         // rho1 <- rho2, rho2 <- r.r, |p| for the scaled directions, then
         // last_err from the new rho2 (each position touches its own entries)
@@ -1392,7 +1422,7 @@ __global__ __launch_bounds__(kBlock) void k_bcg_control(BcgControl d, int init, 
             const int idx = s_unlock[id];
             const int q = d.by_column ? idx : id;
             s_rho1[q] = s_rho2[q];
-            s_rho2[q] = d.rr[id];
+            s_rho2[q] = d.rr_part ? s_rr[id] : d.rr[id];
             if (d.scaled) s_pn[idx] = sqrt(d.ptw[len + id]);
             s_lerr[idx] = sqrt(fabs(s_rho2[q]));
         }
@@ -1933,7 +1963,8 @@ template <typename PT, typename RT>
 static void bcg_xr_t(int n, double *X, long ldx, const int *d_xmap, RT *R, long ldr,
                      const int *d_rmap, const PT *P, long ldp, const int *d_pmap, const PT *W,
                      long ldw, const int *d_wmap, const double *d_rho2, const double *d_ptw,
-                     const double *d_scale, int k, double *d_rr_out, const int *d_count)
+                     const double *d_scale, int k, double *d_rr_out, const int *d_count,
+                     const double **rr_part, int *rr_nbx)
 {
     if (k <= 0) return;
     const int rows = reduction_rows(n, k);
@@ -1943,27 +1974,34 @@ static void bcg_xr_t(int n, double *X, long ldx, const int *d_xmap, RT *R, long 
     hipLaunchKernelGGL((k_bcg_xr<PT, RT>), dim3(nbx, k), dim3(kBlock), 0, state().stream, n, rows, X,
                        ldx, d_xmap, R, ldr, d_rmap, P, ldp, d_pmap, W, ldw, d_wmap, d_rho2, d_ptw,
                        d_scale, partial, d_count);
-    hipLaunchKernelGGL(k_reduce_rows, dim3(k), dim3(kBlock), 0, state().stream, partial, nbx,
-                       d_rr_out);
+    if (rr_part) {
+        *rr_part = partial;
+        *rr_nbx = nbx;
+    } else {
+        hipLaunchKernelGGL(k_reduce_cols_wave, dim3(ceil_div(k, kBlock / kWave)), dim3(kBlock), 0,
+                           state().stream, partial, nbx, k, d_rr_out);
+    }
     HIP_CHECK(hipGetLastError());
 }
 
 void bcg_xr_update(int n, double *X, long ldx, const int *d_xmap, double *R, long ldr,
                    const int *d_rmap, const double *P, long ldp, const int *d_pmap,
                    const double *W, long ldw, const int *d_wmap, const double *d_rho2,
-                   const double *d_ptw, int k, double *d_rr_out, const int *d_count)
+                   const double *d_ptw, int k, double *d_rr_out, const int *d_count,
+                   const double **rr_part, int *rr_nbx)
 {
     bcg_xr_t(n, X, ldx, d_xmap, R, ldr, d_rmap, P, ldp, d_pmap, W, ldw, d_wmap, d_rho2, d_ptw,
-             (const double *)nullptr, k, d_rr_out, d_count);
+             (const double *)nullptr, k, d_rr_out, d_count, rr_part, rr_nbx);
 }
 
 void bcg_xr_update(int n, double *X, long ldx, const int *d_xmap, float *R, long ldr,
                    const int *d_rmap, const float *P, long ldp, const int *d_pmap, const float *W,
                    long ldw, const int *d_wmap, const double *d_rho2, const double *d_ptw,
-                   const double *d_scale, int k, double *d_rr_out, const int *d_count)
+                   const double *d_scale, int k, double *d_rr_out, const int *d_count,
+                   const double **rr_part, int *rr_nbx)
 {
     bcg_xr_t(n, X, ldx, d_xmap, R, ldr, d_rmap, P, ldp, d_pmap, W, ldw, d_wmap, d_rho2, d_ptw,
-             d_scale, k, d_rr_out, d_count);
+             d_scale, k, d_rr_out, d_count, rr_part, rr_nbx);
 }
 
 template <typename T>

@@ -193,6 +193,8 @@ int GCGE_HIP_KernelTime(const char *name, double *ms_total, int *count);
  * 8*n*(m+k), plus 8*n*k when the combination also reads Y) */
 int GCGE_HIP_KernelStats(const char *name, double *ms_total, int *count, double *bytes_total);
 void GCGE_HIP_ProfileReset(void);
+/* create events ahead so that a profiled region creates none */
+void GCGE_HIP_ProfileReserve(int events);
 
 #ifdef __cplusplus
 }
