@@ -4,6 +4,7 @@
 
 libgcge_amd.so   GCG core, host C (gcc)
 libgcge_hip.so   MI355X backend, HIP C++ for gfx950 (hipcc)
+q1_solve         example C program on the public headers (gcc)
 
 Every HIP translation unit is compiled with packed-FP32 instructions
 disabled (-Xclang -target-feature -Xclang -packed-fp32-ops).  Objects are
@@ -77,6 +78,14 @@ def build_hip() -> str:
     return out
 
 
+def build_examples() -> str:
+    """C programs on the public headers only, linked to both libraries."""
+    out = os.path.join(LIB, "q1_solve")
+    _run([CC, "-O2", "-Wall", "-I" + INCLUDE, os.path.join(CSRC, "examples", "q1_solve.c"),
+          "-o", out, "-L" + LIB, "-lgcge_amd", "-lgcge_hip", "-Wl,-rpath,$ORIGIN", "-lm"])
+    return out
+
+
 def build_oracle(with_reference: bool = True) -> None:
     """The checker: the CPU restatement, and the reference built from its own
     sources when the reference tree is present (this container only)."""
@@ -96,6 +105,9 @@ def build_all(verbose: bool = True) -> None:
             r = f.result()
             if verbose and r:
                 print("built", r)
+    r = build_examples()
+    if verbose:
+        print("built", r)
     build_oracle()
 
 

@@ -10,13 +10,15 @@ reference CSR operations) follow the same iteration up to rounding.  Checks:
 * every returned pair satisfies the residual test it was converged with;
 * the converged subspace agrees with the oracle's (largest principal angle).
 """
+import os
+
 import numpy as np
 import pytest
 import scipy.linalg as sla
 
 import gcge_old_amd as gcge
 import oracle
-from gcge_old_amd import problems
+from gcge_old_amd import _native, problems
 
 pytestmark = pytest.mark.gpu
 
@@ -176,3 +178,23 @@ def test_svqb_full_rr_device_matches_oracle():
     np.testing.assert_allclose(dev["eval"], cpu["eval"], rtol=1e-10)
     np.testing.assert_allclose(np.sort(dev["eval"]), problems.q1_eigs(E, nev), rtol=1e-10)
     assert np.all(_residuals(K, M, dev["eval"], dev["evec"]) <= 1.05e-8)
+
+
+def test_c_program_through_solver_init():
+    """A C program on the public headers only (gcge_old_amd/csrc/examples/
+    q1_solve.c): device-assembled Q1 pencil, GCGE_HIP_Solver_Init (the
+    counterpart of the reference's GCGE_CSR_Solver_Init), GCGE_SOLVER_Solve;
+    its eigenvalues against the closed form."""
+    import subprocess
+
+    exe = _native.lib_path("q1_solve")
+    assert os.path.exists(exe), "q1_solve is not built"
+    E, nev = 10, 10
+    p = subprocess.run([exe, str(E), str(nev), "-gcge_conv_type", "RB", "-gcge_ev_tol", "1e-10",
+                        "-gcge_ev_max_it", "100"], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr[-2000:]
+    ev = [float(line.split()[2]) for line in p.stdout.splitlines() if line.startswith("eval ")]
+    assert len(ev) == nev
+    np.testing.assert_allclose(ev, problems.q1_eigs(E, nev), rtol=1e-10)
+    last = [line for line in p.stdout.splitlines() if line.startswith("num_iter")][-1].split()
+    assert int(last[3]) >= nev

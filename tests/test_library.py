@@ -33,6 +33,16 @@ def test_every_declared_symbol_is_exported(header):
     assert not missing, f"{HEADERS[header]} lacks {missing}"
 
 
+def test_c_example_is_built_on_the_public_headers():
+    """build() compiles gcge_old_amd/csrc/examples/q1_solve.c against
+    include/ only (gcge_hip_solver.h is header-only) and links both
+    libraries; the GPU suite runs it."""
+    exe = _native.lib_path("q1_solve")
+    assert os.path.exists(exe) and os.access(exe, os.X_OK)
+    src = open(os.path.join(ROOT, "gcge_old_amd", "csrc", "examples", "q1_solve.c")).read()
+    assert re.findall(r'#include "([^"]+)"', src) == ["gcge_hip_solver.h"]
+
+
 def test_ops_table_layout_matches_reference_order():
     """The table members appear in the reference order (src/gcge_ops.h:29-150),
     so a driver built against the reference header sees the same offsets."""
