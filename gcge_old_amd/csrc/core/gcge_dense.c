@@ -161,7 +161,22 @@ void GCGE_Default_ArrayScale(GCGE_DOUBLE alpha, GCGE_DOUBLE *a, GCGE_INT length)
  * dsyev to rounding, eigenvectors up to sign and the choice of basis inside
  * multiple eigenvalues. */
 
-static double hypot_safe(double a, double b)
+/* The eigensolver is compiled twice, for baseline x86-64 and for AVX2
+ * (helpers inlined into each clone), and the clone is picked at load time
+ * where the host has AVX2.  Its vector loops are element-wise and its sums
+ * keep their order (no reassociation), and the clone targets AVX2 without
+ * fused multiply-adds, so both compute the same roundings: the results are
+ * bit-identical, the AVX2 one 4 doubles per instruction instead of 2. */
+#ifndef GCGE_VECTOR_CLONES
+#if defined(__GNUC__) && !defined(__clang__) && defined(__x86_64__)
+#define GCGE_VECTOR_CLONES __attribute__((target_clones("avx2", "default")))
+#else
+#define GCGE_VECTOR_CLONES
+#endif
+#endif
+#define GCGE_INLINE static inline __attribute__((always_inline))
+
+GCGE_INLINE double hypot_safe(double a, double b)
 {
     double aa = fabs(a), bb = fabs(b);
     if (aa > bb) {
@@ -176,7 +191,7 @@ static double hypot_safe(double a, double b)
 /* sqrt(a^2 + b^2) with one square root when the squares neither overflow
  * nor underflow (the division of the scaled form sits on the QL
  * recurrence's dependency chain), the scaled form otherwise */
-static inline double hypot_fast(double a, double b)
+GCGE_INLINE double hypot_fast(double a, double b)
 {
     const double s = a * a + b * b;
     if (s > 1e-290 && s < 1e290) return sqrt(s);
@@ -184,7 +199,7 @@ static inline double hypot_fast(double a, double b)
 }
 
 /* 2-norm of x[0..m-1]: sum of squares, scaled form when out of range */
-static double norm2(const double *x, int m)
+GCGE_INLINE double norm2(const double *x, int m)
 {
     double s = 0.0;
     for (int i = 0; i < m; ++i) s += x[i] * x[i];
@@ -198,7 +213,7 @@ static double norm2(const double *x, int m)
  * A = Q T Q^T; e[i] couples i and i+1.  The Householder vectors are left in
  * the columns of A below the subdiagonal (unit leading entry implied),
  * their factors in tau. */
-static void sym_tridiag(int n, double *A, double *d, double *e, double *tau, double *p)
+GCGE_INLINE void sym_tridiag(int n, double *A, double *d, double *e, double *tau, double *p)
 {
 #define AC(i, j) A[(size_t)(j) * n + (i)]
     for (int j = 0; j < n - 2; ++j) {
@@ -249,7 +264,7 @@ static void sym_tridiag(int n, double *A, double *d, double *e, double *tau, dou
 
 /* Q = H_0 H_1 ... H_{n-3} into Z (column-major, ldz), reflectors applied
  * from the last to the first so each touches only the trailing block. */
-static void form_q(int n, const double *A, const double *tau, double *Z, int ldz)
+GCGE_INLINE void form_q(int n, const double *A, const double *tau, double *Z, int ldz)
 {
     for (int c = 0; c < n; ++c)
         for (int i = 0; i < n; ++i) Z[(size_t)c * ldz + i] = (i == c) ? 1.0 : 0.0;
@@ -273,7 +288,7 @@ static void form_q(int n, const double *A, const double *tau, double *Z, int ldz
  *     (z_i, z_{i+1}) <- (c_i z_i - s_i z_{i+1}, s_i z_i + c_i z_{i+1}).
  * Rows are taken in chunks whose running column stays in registers, so every
  * element is loaded and stored once per sweep. */
-static void apply_sweep(double *Z, int ldz, int n, int lo, int hi, const double *cs,
+GCGE_INLINE void apply_sweep(double *Z, int ldz, int n, int lo, int hi, const double *cs,
                         const double *sn)
 {
     enum { CH = 16 };
@@ -301,7 +316,7 @@ static void apply_sweep(double *Z, int ldz, int n, int lo, int hi, const double 
  * rotations of a sweep are recorded in cs/sn (n entries each) and applied
  * together.  Returns 0 on success, 1 when an eigenvalue needs more than 60
  * sweeps. */
-static int tridiag_ql(int n, double *d, double *e, double *Z, int ldz, double *cs, double *sn)
+GCGE_INLINE int tridiag_ql(int n, double *d, double *e, double *Z, int ldz, double *cs, double *sn)
 {
     for (int l = 0; l < n; ++l) {
         int iter = 0, m;
@@ -353,6 +368,7 @@ static int tridiag_ql(int n, double *d, double *e, double *Z, int ldz, double *c
 /* All eigenpairs of the symmetric n x n matrix whose upper triangle is in
  * a (column-major, lda).  Eigenvalues ascending in eval; eigenvector j in
  * evec[j*lde .. j*lde+n-1] (lde >= n).  work needs n*n + 5n doubles. */
+GCGE_VECTOR_CLONES
 void GCGE_Dense_SymEig(GCGE_DOUBLE *a, GCGE_INT lda, GCGE_INT n, GCGE_DOUBLE *eval,
                        GCGE_DOUBLE *evec, GCGE_INT lde, GCGE_DOUBLE *work)
 {
