@@ -512,8 +512,8 @@ def permuted_workload(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the row-partitioned multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: runs on one GPU; --gpus {args.gpus} is not supported (the "
+              "multi-GPU row partition is a design, not built)", file=sys.stderr)
         sys.exit(2)
     import gcge_old_amd as gcge
     from gcge_old_amd import _native, problems

@@ -1217,6 +1217,27 @@ __global__ __launch_bounds__(kBlock) void k_reduce_cols_wave(const double *__res
     if (lane == 0) out[c] = v;
 }
 
+// out[c] = sum of the nbx partials of column c, 1024 threads per column
+// (the SpMM's per-row-block partials: a few thousand per column)
+constexpr int kWideBlock = 1024;
+__global__ __launch_bounds__(kWideBlock) void k_reduce_rows_wide(const double *__restrict__ partial,
+                                                                int nbx, double *__restrict__ out)
+{
+    __shared__ double lds[kWideBlock / kWave];
+    const int c = blockIdx.x;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nbx; b += kWideBlock) s += partial[(long)c * nbx + b];
+    s = wave_sum(s);
+    const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+    if (lane == 0) lds[wid] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tot = 0.0;
+        for (int w = 0; w < kWideBlock / kWave; ++w) tot += lds[w];
+        out[c] = tot;
+    }
+}
+
 // out[c] = sum of the nbx partials of column c
 __global__ __launch_bounds__(kBlock) void k_reduce_rows(const double *__restrict__ partial,
                                                        int nbx, double *__restrict__ out)
@@ -1404,17 +1425,18 @@ __global__ __launch_bounds__(kBlock) void k_bcg_control(BcgControl d, int init, 
         }
     }
     if (t == 0) s_count = init ? len : *d.count;
-    __syncthreads();
-    const int nu = s_count;
     if (finished_step > 0 && d.rr_part) {
-        // r.r of the step's columns from the partial sums, one wavefront per column
+        // r.r of the step's columns from the partial sums, one wavefront per
+        // column, issued with the state loads above (all len columns: the
+        // partial buffer holds len of them; only the first count are used)
         const int lane = t % kWave;
-        for (int id = t / kWave; id < nu; id += kBlock / kWave) {
+        for (int id = t / kWave; id < len; id += kBlock / kWave) {
             const double v = wave_col_sum(d.rr_part + (long)id * d.rr_nbx, d.rr_nbx, lane);
             if (lane == 0) s_rr[id] = v;
         }
-        __syncthreads();
     }
+    __syncthreads();
+    const int nu = s_count;
     if (finished_step > 0) {
         // rho1 <- rho2, rho2 <- r.r, |p| for the scaled directions, then
         // last_err from the new rho2 (each position touches its own entries)
@@ -1587,7 +1609,8 @@ void spmm_dot(const Mat *A, const double *P, long ldp, double *W, long ldw, int 
             launch_sell_spmm<1>(A, A->d_val, P, ldp, W, ldw, k, d_pmap, d_wmap, dpart, d_count,
                                 prof.start_event(), prof.stop_event());
     }
-    hipLaunchKernelGGL(k_reduce_rows, dim3(k), dim3(kBlock), 0, state().stream, dpart, nparts, d_out);
+    hipLaunchKernelGGL(k_reduce_rows_wide, dim3(k), dim3(kWideBlock), 0, state().stream, dpart,
+                       nparts, d_out);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -1615,8 +1638,8 @@ void spmm_dot(const Mat *A, const float *P, long ldp, float *W, long ldw, int k,
             launch_sell_spmm<2>(A, A->d_val32, P, ldp, W, ldw, k, d_pmap, d_wmap, dpart, d_count,
                                 prof.start_event(), prof.stop_event());
     }
-    hipLaunchKernelGGL(k_reduce_rows, dim3(2 * k), dim3(kBlock), 0, state().stream, dpart, nparts,
-                       d_out);
+    hipLaunchKernelGGL(k_reduce_rows_wide, dim3(2 * k), dim3(kWideBlock), 0, state().stream, dpart,
+                       nparts, d_out);
     HIP_CHECK(hipGetLastError());
 }
 
