@@ -629,6 +629,73 @@ __global__ __launch_bounds__(kBlock) void k_sell_rows_spmm(int n, int nslices, i
     }
 }
 
+// The same product with the column passes of a row group in one workgroup:
+// wave w takes columns 64 w .. 64 w + 63 (k <= 256) of every row group of the
+// workgroup's slices, so the four 512-byte pieces of a gathered row of XT
+// are read by the four waves close together in time, instead of by
+// workgroups of different passes far apart.  Each row is summed as in
+// k_sell_rows_spmm; the p.w partial of a column comes from the one wave that
+// owns it.
+template <typename VT, typename XT, int DOT>
+__global__ __launch_bounds__(kBlock) void k_sell_rows_spmm_passes(
+    int n, int nslices, int slices_per_block, const int *__restrict__ sptr,
+    const int *__restrict__ sw, const int *__restrict__ scol, const VT *__restrict__ sval,
+    const XT *__restrict__ XTm, int kp, XT *__restrict__ Y, long ldy, int k,
+    const int *__restrict__ ymap, double *__restrict__ dpart, int nparts,
+    const int *__restrict__ count)
+{
+    const int wid = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int kc = count ? min(k, *count) : k;
+    const int c0 = wid * kWave;
+    if (c0 >= kc) return;  // the whole wave: one column pass
+    const int c = c0 + lane;
+    const bool live = c < kc;
+    XT *y = live ? Y + (long)(ymap ? ymap[c] : c) * ldy : nullptr;
+    const int s0 = blockIdx.x * slices_per_block, s1 = min(nslices, s0 + slices_per_block);
+    double pw = 0.0, pp = 0.0;
+    for (int sl = s0; sl < s1; ++sl) {
+        const int w = sw[sl];
+        for (int g = 0; g < kWave / kRowsPerWave; ++g) {
+            const int base = sptr[sl] + g * kRowsPerWave;
+            XT acc[kRowsPerWave];
+#pragma unroll
+            for (int i = 0; i < kRowsPerWave; ++i) acc[i] = XT(0);
+            for (int j = 0; j < w; ++j) {
+                const int e0 = base + j * kWave;
+                XT xv[kRowsPerWave];
+#pragma unroll
+                for (int i = 0; i < kRowsPerWave; ++i) {
+                    const int col = scol[e0 + i];
+                    xv[i] = live ? XTm[(long)col * kp + c] : XT(0);
+                }
+#pragma unroll
+                for (int i = 0; i < kRowsPerWave; ++i) acc[i] += (XT)sval[e0 + i] * xv[i];
+            }
+            const int rbase = sl * kWave + g * kRowsPerWave;
+            if (live) {
+#pragma unroll
+                for (int i = 0; i < kRowsPerWave; ++i)
+                    if (rbase + i < n) y[rbase + i] = acc[i];
+            }
+            if constexpr (DOT > 0) {
+#pragma unroll
+                for (int i = 0; i < kRowsPerWave; ++i) {
+                    const int r = rbase + i;
+                    const double pv = (live && r < n) ? (double)XTm[(long)r * kp + c] : 0.0;
+                    pw += pv * (double)acc[i];
+                    if constexpr (DOT > 1) pp += pv * pv;
+                }
+            }
+        }
+    }
+    if constexpr (DOT > 0) {
+        if (live) {
+            dpart[(long)c * nparts + blockIdx.x] = pw;
+            if constexpr (DOT > 1) dpart[(long)(k + c) * nparts + blockIdx.x] = pp;
+        }
+    }
+}
+
 // slices per workgroup of the row-gather product: about 2048 workgroups per
 // column pass
 inline int rows_slices_per_block(const Mat *A) { return std::max(1, (A->nslices + 2047) / 2048); }
@@ -650,10 +717,21 @@ void launch_rows_spmm(const Mat *A, const VT *vals, const XT *X, long ldx, XT *Y
                        state().stream, A->ncols, k, X, ldx, d_xmap, XTm, kp, d_count);
     const int spb = rows_slices_per_block(A);
     const int nparts = rows_parts(A);
-    hipExtLaunchKernelGGL((k_sell_rows_spmm<VT, XT, DOT>), dim3(nparts, (k + kWave - 1) / kWave),
-                          dim3(kBlock), 0, state().stream, ev0, ev1, 0, n, A->nslices, spb,
-                          A->d_slice_ptr, A->d_slice_w, A->d_col, vals, XTm, kp, Y, ldy, k, d_ymap,
-                          dpart, nparts, d_count);
+    static const int passes_mode = [] {
+        const char *e = getenv("GCGE_HIP_ROWS_PASSES");  // "0": one pass per workgroup (measurement)
+        return e ? atoi(e) : 1;
+    }();
+    const int npass = (k + kWave - 1) / kWave;
+    if (passes_mode != 0 && npass >= 2 && npass <= kBlock / kWave)
+        hipExtLaunchKernelGGL((k_sell_rows_spmm_passes<VT, XT, DOT>), dim3(nparts), dim3(kBlock), 0,
+                              state().stream, ev0, ev1, 0, n, A->nslices, spb, A->d_slice_ptr,
+                              A->d_slice_w, A->d_col, vals, XTm, kp, Y, ldy, k, d_ymap, dpart,
+                              nparts, d_count);
+    else
+        hipExtLaunchKernelGGL((k_sell_rows_spmm<VT, XT, DOT>), dim3(nparts, npass), dim3(kBlock), 0,
+                              state().stream, ev0, ev1, 0, n, A->nslices, spb, A->d_slice_ptr,
+                              A->d_slice_w, A->d_col, vals, XTm, kp, Y, ldy, k, d_ymap, dpart,
+                              nparts, d_count);
     HIP_CHECK(hipGetLastError());
 }
 
