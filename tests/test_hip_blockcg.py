@@ -132,18 +132,20 @@ def test_device_bookkeeping_matches_host_bookkeeping(fp32, monkeypatch):
         np.testing.assert_array_equal(a, h)
 
 
+@pytest.mark.parametrize("cols", [20, 150])
 @pytest.mark.parametrize("fp32", [False, True])
-def test_device_block_cg_row_gathers_on_permuted_matrix(fp32):
-    """A randomly permuted Q1 pencil (row gathers, 20 columns: the p.w
-    epilogue of the row-gather product) against the restatement on the same
-    matrices: float64 to rounding, float32 within float32 rounding."""
+def test_device_block_cg_row_gathers_on_permuted_matrix(fp32, cols):
+    """A randomly permuted Q1 pencil (row gathers with the p.w epilogue: 20
+    columns, one pass per workgroup; 150 columns, three passes in one
+    workgroup) against the restatement on the same matrices: float64 to
+    rounding, float32 within float32 rounding."""
     K, M = problems.q1_fem_3d(9, problems.lognormal_kappa(9, 5))
     perm = np.random.default_rng(36).permutation(K.nrows)
     Kp = problems.symmetric_permutation(K, perm)
     rng = np.random.default_rng(37)
-    b = rng.standard_normal((K.nrows, 20)) * np.logspace(0, -3, 20)
-    x0 = 1e-3 * rng.standard_normal((K.nrows, 20))
-    print(f"block CG row gathers fp32={fp32}", flush=True)
+    b = rng.standard_normal((K.nrows, cols)) * np.logspace(0, -3, cols)
+    x0 = 1e-3 * rng.standard_normal((K.nrows, cols))
+    print(f"block CG row gathers fp32={fp32} cols={cols}", flush=True)
     x, _ = device_block_cg(Kp, None, b, x0, 12, 1e-3, fp32=fp32)
     xo, _ = oracle.block_cg(Kp, None, b, x0, 12, 1e-3)
     if fp32:
