@@ -20,15 +20,33 @@
  * leading dimension lde.
  */
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
 #include "gcge.h"
 
+/* The dense products and the eigensolver are compiled twice, for baseline
+ * x86-64 and for AVX2 (helpers inlined into each clone), and the clone is
+ * picked at load time where the host has AVX2.  Their vector loops are
+ * element-wise and their sums keep their order (no reassociation), and the
+ * clone targets AVX2 without fused multiply-adds, so both compute the same
+ * roundings: the results are bit-identical, the AVX2 one 4 doubles per
+ * instruction instead of 2. */
+#ifndef GCGE_VECTOR_CLONES
+#if defined(__GNUC__) && !defined(__clang__) && defined(__x86_64__)
+#define GCGE_VECTOR_CLONES __attribute__((target_clones("avx2", "default")))
+#else
+#define GCGE_VECTOR_CLONES
+#endif
+#endif
+#define GCGE_INLINE static inline __attribute__((always_inline))
+
 static inline int lsame(const char *c, char u) { return (*c == u) || (*c == (char)(u + 32)); }
 
 /* c = alpha op(a) op(b) + beta c, all column-major (Fortran dgemm contract:
  * with beta == 0, c is not read). */
+GCGE_VECTOR_CLONES
 void GCGE_Default_DenseMatDotDenseMat(char *transa, char *transb, GCGE_INT *nrows,
                                       GCGE_INT *ncols, GCGE_INT *mid, GCGE_DOUBLE *alpha,
                                       GCGE_DOUBLE *a, GCGE_INT *lda, GCGE_DOUBLE *b,
@@ -59,8 +77,28 @@ void GCGE_Default_DenseMatDotDenseMat(char *transa, char *transb, GCGE_INT *nrow
                 for (int i = 0; i < M; ++i) cj[i] += t * al_col[i];
             }
         } else {
-            /* dot form: c(i,j) += alpha * a(:,i) . op(b)(:,j) */
-            for (int i = 0; i < M; ++i) {
+            /* dot form: c(i,j) += alpha * a(:,i) . op(b)(:,j); four rows i at a
+             * time (independent sums, each over l in order) */
+            int i = 0;
+            if (!tb) {
+                const double *bj = b + (size_t)j * LB;
+                for (; i + 4 <= M; i += 4) {
+                    const double *a0 = a + (size_t)i * LA, *a1 = a0 + LA, *a2 = a1 + LA, *a3 = a2 + LA;
+                    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+                    for (int l = 0; l < K; ++l) {
+                        const double bl = bj[l];
+                        s0 += a0[l] * bl;
+                        s1 += a1[l] * bl;
+                        s2 += a2[l] * bl;
+                        s3 += a3[l] * bl;
+                    }
+                    cj[i] += al * s0;
+                    cj[i + 1] += al * s1;
+                    cj[i + 2] += al * s2;
+                    cj[i + 3] += al * s3;
+                }
+            }
+            for (; i < M; ++i) {
                 const double *ai = a + (size_t)i * LA;
                 double s = 0.0;
                 if (!tb) {
@@ -76,7 +114,11 @@ void GCGE_Default_DenseMatDotDenseMat(char *transa, char *transb, GCGE_INT *nrow
 }
 
 /* c = alpha a b + beta c (side 'L') or alpha b a + beta c (side 'R'), with
- * a symmetric and only its `uplo` triangle referenced. */
+ * a symmetric and only its `uplo` triangle referenced.  The triangle is first
+ * expanded to a full square in scratch memory; every c(i,j) is then the sum
+ * over k in increasing order of a(i,k) b(k,j) (side 'L') or b(i,k) a(k,j)
+ * (side 'R'), formed for a whole column of c at a time. */
+GCGE_VECTOR_CLONES
 void GCGE_Default_DenseSymMatDotDenseMat(char *side, char *uplo, GCGE_INT *nrows, GCGE_INT *ncols,
                                          GCGE_DOUBLE *alpha, GCGE_DOUBLE *a, GCGE_INT *lda,
                                          GCGE_DOUBLE *b, GCGE_INT *ldb, GCGE_DOUBLE *beta,
@@ -86,22 +128,36 @@ void GCGE_Default_DenseSymMatDotDenseMat(char *side, char *uplo, GCGE_INT *nrows
     const int left = lsame(side, 'L'), upper = lsame(uplo, 'U');
     const double al = *alpha, be = *beta;
     if (M <= 0 || N <= 0) return;
-#define ASYM(i, k)                                                                           \
-    (upper ? ((i) <= (k) ? a[(i) + (size_t)(k) * LA] : a[(k) + (size_t)(i) * LA])          \
-           : ((i) >= (k) ? a[(i) + (size_t)(k) * LA] : a[(k) + (size_t)(i) * LA]))
-    for (int j = 0; j < N; ++j) {
-        for (int i = 0; i < M; ++i) {
-            double s = 0.0;
-            if (left) {
-                for (int k = 0; k < M; ++k) s += ASYM(i, k) * b[k + (size_t)j * LB];
-            } else {
-                for (int k = 0; k < N; ++k) s += b[i + (size_t)k * LB] * ASYM(k, j);
-            }
-            double *cij = c + i + (size_t)j * LC;
-            *cij = (be == 0.0 ? 0.0 : be * *cij) + al * s;
-        }
+    const int S = left ? M : N; /* order of a */
+    double *full = (double *)malloc(((size_t)S * S + (size_t)M) * sizeof(double));
+    if (!full) {
+        fprintf(stderr, "GCGE_Default_DenseSymMatDotDenseMat: out of memory\n");
+        abort();
     }
-#undef ASYM
+    double *s = full + (size_t)S * S; /* one column of sums */
+    for (int k = 0; k < S; ++k)
+        for (int i = 0; i < S; ++i)
+            full[i + (size_t)k * S] = (upper ? (i <= k) : (i >= k)) ? a[i + (size_t)k * LA]
+                                                                  : a[k + (size_t)i * LA];
+    for (int j = 0; j < N; ++j) {
+        for (int i = 0; i < M; ++i) s[i] = 0.0;
+        if (left) {
+            for (int k = 0; k < M; ++k) {
+                const double bkj = b[k + (size_t)j * LB];
+                const double *ak = full + (size_t)k * S;
+                for (int i = 0; i < M; ++i) s[i] += ak[i] * bkj;
+            }
+        } else {
+            for (int k = 0; k < N; ++k) {
+                const double akj = full[k + (size_t)j * S];
+                const double *bk = b + (size_t)k * LB;
+                for (int i = 0; i < M; ++i) s[i] += bk[i] * akj;
+            }
+        }
+        double *cj = c + (size_t)j * LC;
+        for (int i = 0; i < M; ++i) cj[i] = (be == 0.0 ? 0.0 : be * cj[i]) + al * s[i];
+    }
+    free(full);
 }
 
 GCGE_DOUBLE GCGE_Default_ArrayDotArray(GCGE_DOUBLE *x, GCGE_DOUBLE *y, GCGE_INT length)
@@ -161,20 +217,7 @@ void GCGE_Default_ArrayScale(GCGE_DOUBLE alpha, GCGE_DOUBLE *a, GCGE_INT length)
  * dsyev to rounding, eigenvectors up to sign and the choice of basis inside
  * multiple eigenvalues. */
 
-/* The eigensolver is compiled twice, for baseline x86-64 and for AVX2
- * (helpers inlined into each clone), and the clone is picked at load time
- * where the host has AVX2.  Its vector loops are element-wise and its sums
- * keep their order (no reassociation), and the clone targets AVX2 without
- * fused multiply-adds, so both compute the same roundings: the results are
- * bit-identical, the AVX2 one 4 doubles per instruction instead of 2. */
-#ifndef GCGE_VECTOR_CLONES
-#if defined(__GNUC__) && !defined(__clang__) && defined(__x86_64__)
-#define GCGE_VECTOR_CLONES __attribute__((target_clones("avx2", "default")))
-#else
-#define GCGE_VECTOR_CLONES
-#endif
-#endif
-#define GCGE_INLINE static inline __attribute__((always_inline))
+
 
 GCGE_INLINE double hypot_safe(double a, double b)
 {
