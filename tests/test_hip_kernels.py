@@ -165,11 +165,11 @@ def test_gram_matches_oracle(n, m, k):
 
 
 @pytest.mark.parametrize("m,k", [(1, 1), (5, 7), (90, 8), (90, 9), (300, 50), (70, 16), (3, 17),
-                                 (20, 24), (21, 70), (6, 130)])
+                                 (20, 24), (21, 70), (6, 130), (300, 100), (41, 200)])
 @pytest.mark.parametrize("beta", [0.0, 1.0, -0.5])
 def test_linear_comb_matches_oracle(m, k, beta):
     """Vector kernel (k <= 8, 17..32) and matrix-core kernel (9..16, > 32:
-    one to three 64-column groups, m not a multiple of the 4-column step)."""
+    one to four 64-column groups, m not a multiple of the 4-column step)."""
     n = 3001
     rng = _rng(m * k)
     V = rng.standard_normal((n, m + 2))
