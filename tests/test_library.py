@@ -96,3 +96,16 @@ def test_torch_import_after_the_libraries():
     p = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=600,
                        cwd=ROOT)
     assert p.returncode == 0 and p.stdout.startswith("ok"), (p.returncode, p.stderr[-2000:])
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py runs on one GPU: --gpus 2 exits with status 2 and says so,
+    before any device work (CPU)."""
+    import subprocess
+    import sys
+
+    p = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120, cwd=ROOT)
+    assert p.returncode == 2
+    assert "one GPU" in p.stderr and "--gpus 2" in p.stderr
+    assert p.stdout == ""
