@@ -206,7 +206,12 @@ void GCGE_Solve(void *A, void *B, GCGE_DOUBLE *eval, void **evec, GCGE_PARA *par
         mv_s[0] = ws->num_soft_locked_in_last_iter; mv_e[0] = nev;
         mv_s[1] = ws->num_soft_locked_in_last_iter; mv_e[1] = nev;
         t1 = tick(ops);
-        ops->MultiVecSwap(V, evec, mv_s, mv_e, ops);
+        /* The reference swaps V(:, nsl:nev) and evec(:, nsl:nev) here
+         * (src/gcge_eigsol.c:478); only the V side is read afterwards: evec is
+         * the block CG's scratch in GCGE_ComputeW and is rewritten by
+         * GCGE_ComputeX before it is read, and its locked columns come back
+         * from V at the end.  One copy moves half the bytes of the swap. */
+        ops->MultiVecAxpby(1.0, evec, 0.0, V, mv_s, mv_e, ops);
         t2 = tick(ops);
         PART(x_axpy_time, t1, t2);
 
