@@ -99,6 +99,8 @@ def parse():
                         "(default here: -gcge_w_orth_type svqb)")
     p.add_argument("--no-profile", action="store_true",
                    help="time without per-kernel events (no roofline numbers)")
+    p.add_argument("--mfma-probe", action="store_true",
+                   help="isolated Gram / linear-combination launches for matrix-core counters")
     p.add_argument("--pmc-probe", action="store_true",
                    help="only run 5 SpMM launches (K, k=10) and 5 plain copies of n x 64 doubles, "
                         "for rocprofv3 --pmc counter passes; prints one line")
@@ -144,6 +146,39 @@ def pmc_probe(K):
     spmm_bytes = K.nnz * 12 + 4 * (n + 1) + 8 * 10 * 2 * n
     print(json.dumps({"pmc_probe": True, "spmm_algorithmic_bytes": spmm_bytes,
                       "copy_bytes_read": 8 * n * 64, "copy_bytes_written": 8 * n * 64}), flush=True)
+
+
+def mfma_probe():
+    """Isolated Gram and linear-combination launches at the solver's shapes
+    (n, m, k) = (2048383, 300, 100) (128^3 projection), (250047, 60, 10) and
+    (250047, 70, 50) (64^3), three of each, for the matrix-core counter
+    passes; prints their event times."""
+    import ctypes as C
+
+    import gcge_old_amd as gcge
+    from gcge_old_amd import _native
+
+    lib = _native.hip()
+    for n, m, k in ((2048383, 300, 100), (250047, 60, 10), (250047, 70, 50)):
+        V, W = gcge.DeviceMultiVec(n, m), gcge.DeviceMultiVec(n, k)
+        lib.GCGE_HIP_RandomFill(V.handle, 0, m)
+        lib.GCGE_HIP_RandomFill(W.handle, 0, k)
+        G = np.zeros((m, k), order="F")
+        Cm = np.asfortranarray(np.random.default_rng(1).standard_normal((m, k)))
+        lib.GCGE_HIP_ProfileReset()
+        lib.GCGE_HIP_ProfileEnable(1)
+        for _ in range(3):
+            lib.GCGE_HIP_Gram(V.handle, 0, m, W.handle, 0, k, _native.as_dptr(G), m)
+        for _ in range(3):
+            lib.GCGE_HIP_LinearComb(V.handle, 0, m, _native.as_dptr(Cm), m, 0.0, W.handle, 0, k)
+        lib.GCGE_HIP_ProfileEnable(0)
+        res = {"n": n, "m": m, "k": k, "gflop": round(2.0 * n * m * k / 1e9, 2)}
+        for name in ("gram", "lincomb"):
+            ms, cnt, by = C.c_double(), C.c_int(), C.c_double()
+            lib.GCGE_HIP_KernelStats(name.encode(), C.byref(ms), C.byref(cnt), C.byref(by))
+            res[name + "_us"] = round(ms.value * 1e3 / max(cnt.value, 1), 1)
+        print(json.dumps(res), flush=True)
+        del V, W
 
 
 def kernel_sweep(K, M):
@@ -534,6 +569,9 @@ def main():
     from gcge_old_amd import _native, problems
 
     gcge.init(0)
+    if args.mfma_probe:
+        mfma_probe()
+        return
     lib = _native.hip()
     if args.workload == "q1-64-adjoint":
         adjoint_workload(args)
