@@ -1,40 +1,52 @@
 """Benchmark: seconds to nev eigenpairs of the Q1 FEM pencil on one MI355X.
 
-Workload (one "step" = one complete GCG solve): trilinear Q1 hexahedral
-elements on the uniform 64^3 mesh of the unit cube, Dirichlet nodes
-eliminated (n = 63^3 = 250047), stiffness K and consistent mass M with
-kappa = 1, nev = 50 smallest eigenpairs of K x = lambda M x, float64
-throughout, converged to ||Kx - lambda Mx|| / (|lambda| ||Mx||) <= 1e-8.
-Matrices are uploaded before timing; the initial block is drawn on the
-device inside each solve from the seed.  The solver runs with
--gcge_full_rr 1: the converged prefix stays in the Rayleigh-Ritz step with
-its computed coupling.  The reference's soft locking zeroes that coupling,
-which at this tolerance stalls some starts short of nev pairs (the reference
-build does the same); --reference-locking restores it.  The W block is
-B-orthonormalised with -gcge_w_orth_type svqb (the reference's projection
-against [X, P], then two SVQB passes instead of the recursive single-column
-leaves: 5 host round trips instead of about 40, same span);
+Default workload (one "step" = one complete GCG solve): trilinear Q1
+hexahedral elements on the uniform 128^3 mesh of the unit cube, Dirichlet
+nodes eliminated (n = 127^3 = 2048383), stiffness K with a piecewise-constant
+coefficient log kappa_e ~ N(0, 1) i.i.d. from the seed and consistent mass M,
+nev = 100 smallest eigenpairs of K x = lambda M x (block size 200 requested,
+which the reference's rule clamps to nev), converged to
+||Kx - lambda Mx|| / (|lambda| ||Mx||) <= 1e-8.  K and M are assembled on the
+device from kappa before timing; the initial block is drawn on the device
+inside each solve from the seed.
+
+value is the mixed-precision form: the inner block CG (directions, products,
+residuals and its correction of the Ritz vectors) in float32, float64 for the
+B-Gram blocks, their orthonormalisation, the Rayleigh-Ritz matrices and dense
+eigensolve, and the residual check.  The plain float64 solve of the same
+seeds is timed beside it (fp64), with the largest relative eigenvalue
+difference, the largest principal angle between the two converged subspaces
+and the largest final residual of each.
+
+--workload q1-64 times the 64^3 kappa = 1 pencil (nev 50) with the float64
+solve as value (eigenvalues against the closed form), mixed beside it.
+
+The solver runs with -gcge_full_rr 1: the converged prefix stays in the
+Rayleigh-Ritz step with its computed coupling.  The reference's soft locking
+zeroes that coupling, which at this tolerance stalls some starts short of nev
+pairs (the reference build does the same); --reference-locking restores it.
+The W block is B-orthonormalised with -gcge_w_orth_type svqb (the reference's
+projection against [X, P], then two SVQB passes instead of the recursive
+single-column leaves: 5 host round trips instead of about 40, same span);
 --reference-orth restores "multi".
 
 The JSON line carries:
   value          mean seconds per solve over the timed steps (lower is better)
-  roofline       the block SpMM (dominant kernel): algorithmic bytes over its
-                 HIP-event time summed over the timed region, against 8 TB/s
-                 (the events of the SpMM launches are recorded by the
+  roofline       the block-CG products (the dominant kernel; float32 in the
+                 mixed form): algorithmic bytes over their HIP-event time
+                 summed over the timed region (events recorded by the
                  dispatch packets, hipExtLaunchKernelGGL; nothing else is
-                 bracketed during the timed steps); traffic from the
-                 committed rocprofv3 counter summary in profiles/
+                 bracketed during the timed steps), against 8 TB/s; traffic
+                 from the committed rocprofv3 counter summary in profiles/
   kernels_one_solve  per-kernel event time of one extra, untimed solve with
                  every launch bracketed
-  cpu_baseline   the reference CSR path built from its own sources (or the
-                 oracle restatement when that binary is absent), one core,
-                 timed on a bounded number of GCG iterations of the same
-                 workload and scaled to the device run's iteration count
-  mixed          the mixed-precision form on the same workload and seeds
-                 (float32 products and directions in the inner block CG,
-                 float64 everywhere else): seconds per solve, speed-up over
-                 the float64 solve and the largest relative eigenvalue
-                 difference from it
+  cpu_baseline   the reference's CSR path for the same work on one host
+                 core: the solve's operations counted as the single-vector
+                 CSR calls the reference's column loops make (products,
+                 inner products, axpbys, dense eigensolves), each priced by
+                 the oracle port's own call timed at the workload's size
+                 (the reference's vendored LAPACK for the dense eigensolve)
+  fp64 / mixed   the other precision on the same seeds
 """
 from __future__ import annotations
 
@@ -57,17 +69,17 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--steps", type=int, default=2)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--workload", choices=("q1-64", "q1-128-lognormal", "q1-64-adjoint",
+    p.add_argument("--workload", choices=("q1-128-lognormal", "q1-64", "q1-64-adjoint",
                                           "q1-192-permuted"),
-                   default="q1-64",
-                   help="q1-64: Q1 64^3, kappa = 1, nev 50 (default, float64 value); "
-                        "q1-128-lognormal: Q1 128^3, lognormal kappa, nev 100, block 200 "
-                        "requested (the mixed-precision comparison on one seed); "
+                   default="q1-128-lognormal",
+                   help="q1-128-lognormal: Q1 128^3, lognormal kappa, nev 100, block 200 "
+                        "requested, value = mixed-precision solve (default); "
+                        "q1-64: Q1 64^3, kappa = 1, nev 50, value = float64 solve; "
                         "q1-64-adjoint: Q1 64^3 lognormal kappa, nev 20, gradient of "
                         "sum w_k lambda_k + ||P x_1||^2 with respect to all kappa_e, "
-                        "checked by fourth-order central differences on --fd elements; "
+                        "checked by central differences on --fd elements; "
                         "q1-192-permuted: Q1 192^3 lognormal kappa, rows and columns "
                         "symmetrically permuted by a seeded random permutation, nev 200, block "
                         "400 requested, a bounded number of GCG iterations (--max-it, default "
@@ -75,12 +87,14 @@ def parse():
                         "all elements")
     p.add_argument("--fd", type=int, default=8, help="elements checked by finite differences "
                                                      "(q1-64-adjoint)")
+    p.add_argument("--fd-step", type=float, default=None,
+                   help="relative step of the central differences (q1-64-adjoint)")
     p.add_argument("--reorder", choices=("none", "rcm"), default="none",
                    help="q1-192-permuted: reorder the permuted pencil by reverse Cuthill-McKee "
                         "(computed from the matrix graph, a preprocessing step) before the solve")
     p.add_argument("--assembly", choices=("device", "host"), default="device",
-                   help="q1-192-permuted: assemble K and M on the device from kappa and the "
-                        "node -> row map, or on the host (numpy) and copy the CSR arrays")
+                   help="assemble K and M on the device from kappa (default), or on the host "
+                        "(numpy) and copy the CSR arrays")
     p.add_argument("--mesh", type=int, default=None, help="elements per axis (overrides the workload)")
     p.add_argument("--nev", type=int, default=None)
     p.add_argument("--kappa-seed", type=int, default=1)
@@ -88,10 +102,15 @@ def parse():
     p.add_argument("--max-it", type=int, default=None,
                    help="GCG iteration cap (a bounded run for profiling; solves stopped by it "
                         "report all_converged false)")
-    p.add_argument("--cpu-iters", type=int, default=3,
-                   help="GCG iterations timed for the CPU baseline (0: skip)")
-    p.add_argument("--mixed-steps", type=int, default=-1,
-                   help="solves of the mixed-precision form (default: --steps; 0: skip)")
+    p.add_argument("--other-steps", type=int, default=-1,
+                   help="solves of the other precision (default: min(--steps, 2); 0: skip)")
+    p.add_argument("--cpu", choices=("model", "none"), default="model",
+                   help="CPU baseline: op-count model priced by the port's timed calls")
+    p.add_argument("--cpu-reference-iters", type=int, default=0,
+                   help="also time the reference binary (oracle/_ref) on 1 and 1 + this many "
+                        "GCG iterations of the same pencil and scale the difference to the "
+                        "device iteration count (reference mode: its soft locking, multi "
+                        "orthonormalisation and R residual)")
     p.add_argument("--reference-locking", action="store_true",
                    help="soft locking as in the reference (no -gcge_full_rr)")
     p.add_argument("--reference-orth", action="store_true",
@@ -99,53 +118,73 @@ def parse():
                         "(default here: -gcge_w_orth_type svqb)")
     p.add_argument("--no-profile", action="store_true",
                    help="time without per-kernel events (no roofline numbers)")
+    p.add_argument("--no-breakdown", action="store_true",
+                   help="skip the extra untimed solve with every launch bracketed")
     p.add_argument("--mfma-probe", action="store_true",
                    help="isolated Gram / linear-combination launches for matrix-core counters")
     p.add_argument("--pmc-probe", action="store_true",
-                   help="only run 5 SpMM launches (K, k=10) and 5 plain copies of n x 64 doubles, "
-                        "for rocprofv3 --pmc counter passes; prints one line")
+                   help="only the workload's product launches (the block-CG float32 product, "
+                        "float64 products) and plain copies of known size, for rocprofv3 --pmc "
+                        "counter passes; prints one line")
     p.add_argument("--kernels", action="store_true",
                    help="also time SpMM, Gram, linear combination and a plain copy in isolation "
                         "(printed on stderr; the copy is the calibration for HBM counters)")
     return p.parse_args()
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "spmm_q1_64_k10_hbm_traffic.md")
+def traffic_file(E, lognormal):
+    return os.path.join(ROOT, "profiles", f"spmm_q1_{E}{'_lognormal' if lognormal else ''}_hbm_traffic.md")
 
 
-def read_traffic():
-    """HBM bytes per SpMM launch (K, k = 10) from the committed counter
-    summary: the line 'traffic_bytes_per_launch: <value>'."""
+def read_traffic(E, lognormal, key):
+    """HBM bytes per launch of the product `key` from the committed counter
+    summary: the line '<key>: <bytes>'."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(traffic_file(E, lognormal)) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
+                if line.startswith(key + ":"):
                     return float(line.split(":", 1)[1])
     except OSError:
         pass
     return None
 
 
-def pmc_probe(K):
-    """Isolated launches for the counter passes: SpMM on K with 10 columns
-    (the block-CG shape of the workload) and a plain copy of known size."""
+def pmc_probe(dK, k_cg):
+    """Isolated launches for the counter passes, three of each: the block-CG
+    float32 product on K with k_cg columns (through the block CG with a
+    fixed step count: one direction update, product and x/r update per
+    step), the float64 products with k_cg and 2 k_cg columns, and a plain
+    copy of n x 64 doubles (the calibration)."""
     import gcge_old_amd as gcge
     from gcge_old_amd import _native
 
     lib = _native.hip()
-    n = K.nrows
-    dK = gcge.DeviceMatrix(K)
-    X = gcge.DeviceMultiVec(n, 64)
-    Y = gcge.DeviceMultiVec(n, 64)
-    lib.GCGE_HIP_RandomFill(X.handle, 0, 64)
-    for _ in range(5):
-        lib.GCGE_HIP_SpMM(dK.handle, X.handle, 0, Y.handle, 0, 10)
-    for _ in range(5):
+    n = dK.nrows
+    X = gcge.DeviceMultiVec(n, 2 * k_cg)
+    Y = gcge.DeviceMultiVec(n, 2 * k_cg)
+    lib.GCGE_HIP_RandomFill(X.handle, 0, 2 * k_cg)
+    lib.GCGE_HIP_RandomFill(Y.handle, 0, 2 * k_cg)
+    # float32 block CG: 4 iterations = 3 products of k_cg columns (rate 0:
+    # no column stops)
+    dK.set_precision(True)
+    lib.GCGE_HIP_SetLinearSolverParams(4, 0.0, 0, 0.0, None)
+    lib.GCGE_HIP_MultiLinearSolve(dK.handle, Y.handle, 0, X.handle, 0, k_cg)
+    lib.GCGE_HIP_SetLinearSolverParams(20, 1e-2, 0, 0.0, None)
+    dK.set_precision(False)
+    for k in (k_cg, 2 * k_cg):
+        for _ in range(3):
+            lib.GCGE_HIP_SpMM(dK.handle, X.handle, 0, Y.handle, 0, k)
+    for _ in range(3):
         lib.GCGE_HIP_Axpby(1.0, X.handle, 0, 0.0, Y.handle, 0, 64)
     lib.GCGE_HIP_Synchronize()
-    spmm_bytes = K.nnz * 12 + 4 * (n + 1) + 8 * 10 * 2 * n
-    print(json.dumps({"pmc_probe": True, "spmm_algorithmic_bytes": spmm_bytes,
-                      "copy_bytes_read": 8 * n * 64, "copy_bytes_written": 8 * n * 64}), flush=True)
+    fixed64 = dK.nnz * 12 + 4 * (n + 1)
+    fixed32 = dK.nnz * 8 + 4 * (n + 1)
+    print(json.dumps({"pmc_probe": True, "n": n, "nnz": dK.nnz,
+                      "spmm32_cg_algorithmic_bytes": fixed32 + 8 * k_cg * n,
+                      "spmm64_algorithmic_bytes": {str(k): fixed64 + 16 * k * n
+                                                   for k in (k_cg, 2 * k_cg)},
+                      "copy_bytes_read": 8 * n * 64, "copy_bytes_written": 8 * n * 64}),
+          flush=True)
 
 
 def mfma_probe():
@@ -176,12 +215,14 @@ def mfma_probe():
         for name in ("gram", "lincomb"):
             ms, cnt, by = C.c_double(), C.c_int(), C.c_double()
             lib.GCGE_HIP_KernelStats(name.encode(), C.byref(ms), C.byref(cnt), C.byref(by))
-            res[name + "_us"] = round(ms.value * 1e3 / max(cnt.value, 1), 1)
+            us = ms.value * 1e3 / max(cnt.value, 1)
+            res[name + "_us"] = round(us, 1)
+            res[name + "_tflops"] = round(2.0 * n * m * k / (us * 1e-6) / 1e12, 1) if us else None
         print(json.dumps(res), flush=True)
         del V, W
 
 
-def kernel_sweep(K, M):
+def kernel_sweep(dK, dM):
     """Isolated kernel timings on the workload's matrices (stderr)."""
     import ctypes as C
 
@@ -189,13 +230,12 @@ def kernel_sweep(K, M):
     from gcge_old_amd import _native
 
     lib = _native.hip()
-    n = M.nrows
-    dK, dM = gcge.DeviceMatrix(K), gcge.DeviceMatrix(M)
-    X = gcge.DeviceMultiVec(n, 96)
-    Y = gcge.DeviceMultiVec(n, 96)
-    lib.GCGE_HIP_RandomFill(X.handle, 0, 96)
+    n = dM.nrows
+    X = gcge.DeviceMultiVec(n, 200)
+    Y = gcge.DeviceMultiVec(n, 200)
+    lib.GCGE_HIP_RandomFill(X.handle, 0, 200)
 
-    def timed(name, fn, reps=20, extra=()):
+    def timed(name, fn, reps=10, extra=()):
         """us per launch and GB/s of kernel `name`; `extra` kernels of the
         same calls (a second reduction pass) are added to the time"""
         lib.GCGE_HIP_ProfileReset()
@@ -215,21 +255,22 @@ def kernel_sweep(K, M):
         return us, gbs
 
     for mat, label in ((dK, "K"), (dM, "M")):
-        for k in (1, 4, 10, 16, 50):
+        for k in (1, 10, 50, 100, 200):
             us, gbs = timed("spmm", lambda: lib.GCGE_HIP_SpMM(mat.handle, X.handle, 0, Y.handle, 0, k))
             print(f"# kernel spmm {label} k={k}: {us:.1f} us/launch, {gbs:.0f} GB/s", file=sys.stderr)
-    G = np.zeros((96, 96), order="F")
-    for m, k in ((70, 10), (60, 60), (90, 10), (10, 10), (1, 1)):
+    G = np.zeros((200, 200), order="F")
+    for m, k in ((70, 10), (60, 60), (200, 100), (300, 100), (1, 1)):
         us, gbs = timed("gram", lambda: lib.GCGE_HIP_Gram(X.handle, 0, m, Y.handle, 0, k,
-                                                          _native.as_dptr(G), 96),
+                                                          _native.as_dptr(G), 200),
                         extra=("gram_reduce",))
-        print(f"# kernel gram m={m} k={k} (with its reduction pass): {us:.1f} us/call, {gbs:.0f} GB/s",
-              file=sys.stderr)
-    Cm = np.asfortranarray(np.random.default_rng(0).standard_normal((96, 96)))
-    for m, k in ((90, 50), (80, 30), (70, 10)):
+        print(f"# kernel gram m={m} k={k} (with its reduction pass): {us:.1f} us/call, {gbs:.0f} GB/s, "
+              f"{2.0 * n * m * k / (us * 1e-6) / 1e12:.1f} TFLOP/s", file=sys.stderr)
+    Cm = np.asfortranarray(np.random.default_rng(0).standard_normal((200, 200)))
+    for m, k in ((90, 50), (200, 100), (70, 10)):
         us, gbs = timed("lincomb", lambda: lib.GCGE_HIP_LinearComb(X.handle, 0, m, _native.as_dptr(Cm),
-                                                                   96, 0.0, Y.handle, 0, k))
-        print(f"# kernel lincomb m={m} k={k}: {us:.1f} us/launch, {gbs:.0f} GB/s", file=sys.stderr)
+                                                                   200, 0.0, Y.handle, 0, k))
+        print(f"# kernel lincomb m={m} k={k}: {us:.1f} us/launch, {gbs:.0f} GB/s, "
+              f"{2.0 * n * m * k / (us * 1e-6) / 1e12:.1f} TFLOP/s", file=sys.stderr)
     # plain copy of 64 columns (n x 64 doubles read and written): the counter
     # calibration dispatch
     us, _ = timed("axpby", lambda: lib.GCGE_HIP_Axpby(1.0, X.handle, 0, 0.0, Y.handle, 0, 64))
@@ -239,7 +280,9 @@ def kernel_sweep(K, M):
 
 
 KERNELS = ("spmm", "spmm32", "gram", "gram_reduce", "lincomb", "axpby", "axpby_list", "swap",
-           "coldots", "bcg_p", "bcg_xr", "axpy_cols", "residual", "random")
+           "coldots", "bcg_p", "bcg_xr", "axpy_cols", "residual", "random", "convert")
+WORK_NAMES = ("spmv", "spmv_nnz", "spmv_rows", "dot", "axpby", "dense_eig", "dense_eig_n3",
+              "vec_len")
 
 
 def kernel_stats(lib):
@@ -255,71 +298,395 @@ def kernel_stats(lib):
     return kernels
 
 
-def host_transfer(gcge, lib, n, nev, upload_s, value):
-    """The solve with its host transfers: the matrices' upload (host CSR to
-    the device layouts, once) and the eigenvectors' download (n x nev
-    doubles, once per solve), next to the resident-input value."""
-    X = gcge.DeviceMultiVec(n, nev)
-    lib.GCGE_HIP_Synchronize()
-    t0 = time.perf_counter()
-    X.get()
-    down = time.perf_counter() - t0
-    return {"matrix_upload_s": round(upload_s, 4), "evec_download_s": round(down, 5),
-            "s_per_solve_incl_transfers": round(value + upload_s + down, 4),
-            "note": "upload includes the host-side sliced-ELLPACK/diagonal conversion; "
-                    "value excludes both transfers (inputs resident)"}
+def work_counts(lib, reset):
+    from gcge_old_amd import _native
+
+    w = np.zeros(len(WORK_NAMES))
+    lib.GCGE_HIP_WorkCounts(_native.as_dptr(w), len(w), 1 if reset else 0)
+    return dict(zip(WORK_NAMES, w.tolist()))
 
 
-def mixed_leg(gcge, lib, dK, dM, nev, flags, steps, fp64_outs, fp64_value):
-    """The mixed-precision form on the same seeds as the float64 steps."""
-    lib.GCGE_HIP_ProfileReset()
-    lib.GCGE_HIP_ProfileEnable(2)
-    times, iters, errs = [], [], []
-    for s in range(steps):
-        out = gcge.solve(dK, dM, nev, flags, seed=1 + s, precision="mixed")
-        times.append(out["time"])
-        iters.append(out["num_iter"])
-        ref = fp64_outs[s % len(fp64_outs)]["eval"]
-        errs.append(float(np.max(np.abs(np.sort(out["eval"]) - np.sort(ref)) / np.abs(ref))))
-        print(f"# mixed step {s}: {out['time']:.4f}s iters={out['num_iter']} "
-              f"conv={out['num_conv']} max rel eig diff {errs[-1]:.2e}", file=sys.stderr, flush=True)
-    lib.GCGE_HIP_ProfileEnable(0)
-    k = kernel_stats(lib)
-    dK.set_precision(False)
-    dM.set_precision(False)
-    value = float(np.mean(times))
-    out = {"value": round(value, 5), "unit": "s", "speedup_vs_fp64": round(fp64_value / value, 3),
-           "iterations": iters, "max_rel_eig_diff_vs_fp64": max(errs),
-           "dtype": "f32 inner block CG products and directions, f64 elsewhere"}
-    if "spmm32" in k and k["spmm32"]["ms"] > 0:
-        sp = k["spmm32"]
-        out["spmm32_gbs"] = round(sp["bytes"] / (sp["ms"] * 1e-3) / 1e9, 1)
-    return out
+def cpu_model(K, M, work, reps=10):
+    """Seconds the reference's CSR path needs for `work` (counts per solve,
+    work_counts) on one host core: every count priced by the oracle port's
+    own single-vector call (app/csr/gcge_app_csr.c:297-378 restated, serial
+    loops), timed here at the workload's size; the dense eigensolves by the
+    reference's vendored LAPACK dsyev (oracle/_ref) at the solve's mean order,
+    or the core's host QL solver when that library is absent."""
+    import ctypes as C
 
-
-def cpu_baseline(K, M, nev, tol, iters, dev_iters):
-    """Time `iters` GCG iterations of the reference CSR path on one core and
-    scale per-iteration cost to the device run's iteration count."""
     import oracle
 
-    flags = ["-gcge_ev_max_it", str(iters), "-gcge_ev_tol", str(tol)]
-    if oracle.ref_available():
-        kind = "reference"
-        r = oracle.ref_solve(K, M, nev, flags)
-        secs, done = r["time"], r["num_iter"]
-    else:
-        kind = "port"
-        r = oracle.solve(K, M, nev, flags + ["-gcge_conv_type", "RB"])
-        secs, done = r["time"], r["num_iter"]
-    per_iter = secs / max(done, 1)
+    L = oracle.lib()
+    dp = C.POINTER(C.c_double)
+    n = K.nrows
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal(n)
+    y = rng.standard_normal(n)
+    r = np.zeros(n)
+    t_start = time.perf_counter()
+    spmv = {}
+    for name, A in (("K", K), ("M", M)):
+        Am = oracle.Matrix(A)
+        L.ORACLE_SpMM(Am.handle, 1, x.ctypes.data_as(dp), n, r.ctypes.data_as(dp), n)  # warm
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            L.ORACLE_SpMM(Am.handle, 1, x.ctypes.data_as(dp), n, r.ctypes.data_as(dp), n)
+        spmv[name] = (time.perf_counter() - t0) / reps
+        del Am
+    per_nnz = (spmv["K"] + spmv["M"]) / (K.nnz + M.nnz)
+    g = np.zeros(1)
+    one = np.ones(1)
+    L.ORACLE_Gram(n, 1, 1, x.ctypes.data_as(dp), n, y.ctypes.data_as(dp), n, g.ctypes.data_as(dp), 1)
+    t0 = time.perf_counter()
+    for _ in range(5 * reps):
+        L.ORACLE_Gram(n, 1, 1, x.ctypes.data_as(dp), n, y.ctypes.data_as(dp), n,
+                      g.ctypes.data_as(dp), 1)
+    t_dot = (time.perf_counter() - t0) / (5 * reps)
+    t0 = time.perf_counter()
+    for _ in range(5 * reps):
+        L.ORACLE_LinearComb(n, 1, 1, x.ctypes.data_as(dp), n, one.ctypes.data_as(dp), 1, 1.0,
+                            y.ctypes.data_as(dp), n)
+    t_axpby = (time.perf_counter() - t0) / (5 * reps)
+    # dense eigensolve at the mean order of the solve's calls
+    n_eig = int(round((work["dense_eig_n3"] / work["dense_eig"]) ** (1.0 / 3.0))) if work["dense_eig"] else 0
+    t_eig, eig_kind = 0.0, "none"
+    if n_eig > 0:
+        a = rng.standard_normal((n_eig, n_eig))
+        a = np.asfortranarray(a + a.T)
+        t_eig, eig_kind = _time_dense_eig(a)
+    secs = (work["spmv_nnz"] * per_nnz + work["dot"] * t_dot + work["axpby"] * t_axpby
+            + work["dense_eig"] * t_eig)
+    sample_s = time.perf_counter() - t_start
     return {
-        "value": per_iter * dev_iters,
+        "value": round(secs, 1),
         "unit": "s",
         "cores": 1,
-        "kind": kind,
-        "sample": f"{done} GCG iterations of the same workload in {secs:.1f} s on one host core "
-                  f"({per_iter:.2f} s/iteration), scaled to the device run's {dev_iters} iterations",
+        "kind": "port",
+        "sample": (f"the solve's work counted as the reference's single-vector CSR calls "
+                   f"({work['spmv']:.0f} SpMV, {work['dot']:.0f} inner products, "
+                   f"{work['axpby']:.0f} axpbys of length {n}, {work['dense_eig']:.0f} dense "
+                   f"eigensolves of mean order {n_eig} per solve), each priced by the port's own "
+                   f"call timed on this host at this size: SpMV K {spmv['K'] * 1e3:.1f} ms, "
+                   f"M {spmv['M'] * 1e3:.1f} ms, inner product {t_dot * 1e3:.2f} ms, axpby "
+                   f"{t_axpby * 1e3:.2f} ms, dense eigensolve {t_eig * 1e3:.1f} ms ({eig_kind}); "
+                   f"{sample_s:.1f} s of timed CPU work"),
+        "unit_costs_s": {"spmv_K": spmv["K"], "spmv_M": spmv["M"], "dot": t_dot, "axpby": t_axpby,
+                         "dense_eig": t_eig},
+        "work_per_solve": {k: round(v, 1) for k, v in work.items()},
     }
+
+
+def _time_dense_eig(a, reps=3):
+    """Seconds per dsyev of the symmetric matrix a: the reference's vendored
+    netlib LAPACK when built, else the core's host QL solver."""
+    import ctypes as C
+
+    dp = C.POINTER(C.c_double)
+    n = a.shape[0]
+    lapack = os.path.join(ROOT, "oracle", "_ref", "liblapack_ref.so")
+    w = np.zeros(n)
+    if os.path.exists(lapack):
+        L = C.CDLL(lapack)
+        info = C.c_int(0)
+        work = np.zeros(1)
+        b = a.copy(order="F")
+        L.dsyev_(b"V", b"U", C.byref(C.c_int(n)), b.ctypes.data_as(dp), C.byref(C.c_int(n)),
+                 w.ctypes.data_as(dp), work.ctypes.data_as(dp), C.byref(C.c_int(-1)), C.byref(info))
+        lw = int(work[0])
+        work = np.zeros(lw)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            b = a.copy(order="F")
+            L.dsyev_(b"V", b"U", C.byref(C.c_int(n)), b.ctypes.data_as(dp), C.byref(C.c_int(n)),
+                     w.ctypes.data_as(dp), work.ctypes.data_as(dp), C.byref(C.c_int(lw)),
+                     C.byref(info))
+        return (time.perf_counter() - t0) / reps, "reference netlib dsyev"
+    from gcge_old_amd import _native
+
+    core = _native.core()
+    f = core.GCGE_Default_DenseMatEigenSolver
+    ip = C.POINTER(C.c_int)
+    f.argtypes = [dp, C.c_int, C.c_int, dp, dp, C.c_int, C.c_int, C.c_int, ip, dp]
+    z = np.zeros((n, n), order="F")
+    iw = np.zeros(10 * n, dtype=np.int32)
+    dw = np.zeros(30 * n * n)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b = a.copy(order="F")
+        f(b.ctypes.data_as(dp), n, n, w.ctypes.data_as(dp), z.ctypes.data_as(dp), n, 1, n,
+          iw.ctypes.data_as(ip), dw.ctypes.data_as(dp))
+    return (time.perf_counter() - t0) / reps, "host QL of the core"
+
+
+def cpu_reference_run(K, M, nev, tol, iters, dev_iters):
+    """The reference binary (oracle/_ref/gcge_ref, built from its sources, one
+    core) on 1 and 1 + iters GCG iterations of the same pencil; the
+    difference per iteration scaled to the device run's iteration count."""
+    import oracle
+
+    if not oracle.ref_available():
+        return None
+    flags = ["-gcge_ev_tol", str(tol)]
+    r1 = oracle.ref_solve(K, M, nev, flags + ["-gcge_ev_max_it", "1"])
+    r2 = oracle.ref_solve(K, M, nev, flags + ["-gcge_ev_max_it", str(1 + iters)])
+    d_it = max(r2["num_iter"] - r1["num_iter"], 1)
+    per_iter = (r2["time"] - r1["time"]) / d_it
+    return {"value": round(per_iter * dev_iters, 1), "unit": "s", "cores": 1, "kind": "reference",
+            "sample": f"reference binary: {r1['num_iter']} and {r2['num_iter']} GCG iterations in "
+                      f"{r1['time']:.1f} s and {r2['time']:.1f} s on one host core, "
+                      f"{per_iter:.2f} s per steady-state iteration, scaled to the device run's "
+                      f"{dev_iters} iterations"}
+
+
+def principal_angle(gcge, lib, dM, X1, X2):
+    """Largest principal angle between span(X1) and span(X2), both
+    M-orthonormal: the singular values of X1^T M X2 are the cosines."""
+    from gcge_old_amd import _native
+
+    k = X1.shape[1]
+    A = gcge.DeviceMultiVec.from_host(X1)
+    B = gcge.DeviceMultiVec.from_host(X2)
+    MB = gcge.DeviceMultiVec(X2.shape[0], k)
+    lib.GCGE_HIP_SpMM(dM.handle, B.handle, 0, MB.handle, 0, k)
+    G = np.zeros((k, k), order="F")
+    lib.GCGE_HIP_Gram(A.handle, 0, k, MB.handle, 0, k, _native.as_dptr(G), k)
+    s = np.linalg.svd(G, compute_uv=False)
+    smin = float(min(s.min(), 1.0))
+    return float(np.sqrt(max(0.0, 1.0 - smin * smin)))
+
+
+def solve_workload(args):
+    """One complete GCG solve per step on the Q1 pencil, in the value's
+    precision (mixed at 128^3 lognormal, float64 at 64^3), the other precision
+    on the same seeds beside it."""
+    import gcge_old_amd as gcge
+    from gcge_old_amd import _native, problems
+
+    lib = _native.hip()
+    lognormal = args.workload == "q1-128-lognormal"
+    E = args.mesh or (128 if lognormal else 64)
+    nev = args.nev or (100 if lognormal else 50)
+    max_it = args.max_it or 500
+    value_prec, other_prec = ("mixed", "fp64") if lognormal else ("fp64", "mixed")
+    t0 = time.perf_counter()
+    kappa = problems.lognormal_kappa(E, args.kappa_seed) if lognormal else None
+    if args.assembly == "device":
+        dK = gcge.DeviceMatrix.q1(E, kappa, "stiffness")
+        dM = gcge.DeviceMatrix.q1(E, None, "mass")
+        lib.GCGE_HIP_Synchronize()
+        setup_s = time.perf_counter() - t0
+        K = M = None
+    else:
+        K, M = problems.q1_fem_3d(E, kappa)
+        t1 = time.perf_counter()
+        dK, dM = gcge.DeviceMatrix(K), gcge.DeviceMatrix(M)
+        lib.GCGE_HIP_Synchronize()
+        setup_s = time.perf_counter() - t1
+    n = dK.nrows
+    flags = ["-gcge_conv_type", "RB", "-gcge_ev_tol", str(args.tol), "-gcge_ev_max_it", str(max_it)]
+    if lognormal:
+        flags += ["-gcge_block_size", str(2 * nev)]
+    if not args.reference_locking:
+        flags += ["-gcge_full_rr", "1"]
+    if not args.reference_orth:
+        flags += ["-gcge_w_orth_type", "svqb"]
+    print(f"# workload: Q1 {E}^3 {'lognormal kappa ' if lognormal else ''}n={n} nnz(K)={dK.nnz} "
+          f"nnz(M)={dM.nnz} nev={nev} ({args.assembly} assembly {setup_s:.2f}s, storage "
+          f"{dK.storage_info()})", file=sys.stderr, flush=True)
+    if args.pmc_probe:
+        pmc_probe(dK, nev)
+        return
+    if args.kernels:
+        kernel_sweep(dK, dM)
+
+    def run(prec, seed, mode, its=max_it):
+        lib.GCGE_HIP_ProfileEnable(mode)
+        f = flags if its == max_it else flags[:5] + [str(its)] + flags[6:]
+        out = gcge.solve(dK, dM, nev, f, seed=seed, precision=prec)
+        lib.GCGE_HIP_ProfileEnable(0)
+        return out
+
+    prof_mode = 0 if args.no_profile else 2
+    warm_launches = 0
+    # buffers of the other precision allocated before anything is timed
+    run(other_prec, 99, 0, its=2)
+    for w in range(args.warmup):
+        lib.GCGE_HIP_ProfileReset()
+        out = run(value_prec, 100 + w, prof_mode)
+        k = kernel_stats(lib)
+        warm_launches = max(warm_launches, sum(v["launches"] for n_, v in k.items()
+                                               if n_ in ("spmm", "spmm32")))
+        print(f"# warmup {w} ({value_prec}): {out['time']:.4f}s iters={out['num_iter']} "
+              f"conv={out['num_conv']}", file=sys.stderr, flush=True)
+
+    # timed region: events on the product launches only (recorded by the
+    # dispatches themselves), so the solve time carries no profiling
+    # packets; the events are created beforehand (two per launch, with a
+    # margin)
+    lib.GCGE_HIP_ProfileReset()
+    if not args.no_profile:
+        lib.GCGE_HIP_ProfileReserve(int(2 * 1.5 * max(warm_launches, 2000) * args.steps))
+    work_counts(lib, reset=True)
+    lib.GCGE_HIP_Synchronize()
+    times, iters, outs = [], [], []
+    t_start = time.perf_counter()
+    for s in range(args.steps):
+        out = run(value_prec, 1 + s, prof_mode)
+        times.append(out["time"])
+        iters.append(out["num_iter"])
+        outs.append(out)
+        print(f"# step {s} ({value_prec}): {out['time']:.4f}s iters={out['num_iter']} "
+              f"conv={out['num_conv']}", file=sys.stderr, flush=True)
+    lib.GCGE_HIP_Synchronize()
+    wall = time.perf_counter() - t_start
+    kernels = kernel_stats(lib)
+    work = {k: v / max(args.steps, 1) for k, v in work_counts(lib, reset=True).items()}
+    work["vec_len"] = n
+    value = float(np.mean(times))
+
+    # the other precision on the same seeds
+    other = None
+    other_steps = min(args.steps, 2) if args.other_steps < 0 else args.other_steps
+    if other_steps > 0:
+        lib.GCGE_HIP_ProfileReset()
+        o_times, o_iters, diffs, angles, o_outs = [], [], [], [], []
+        for s in range(other_steps):
+            o = run(other_prec, 1 + s, prof_mode)
+            o_times.append(o["time"])
+            o_iters.append(o["num_iter"])
+            ref = outs[s % len(outs)]
+            both = min(o["num_conv"], ref["num_conv"])
+            d = np.abs(o["eval"][:both] - ref["eval"][:both]) / np.abs(ref["eval"][:both])
+            diffs.append(float(d.max()) if both else None)
+            if both == nev:
+                angles.append(principal_angle(gcge, lib, dM, ref["evec"], o["evec"]))
+            print(f"# other {s} ({other_prec}): {o['time']:.4f}s iters={o['num_iter']} "
+                  f"conv={o['num_conv']} max rel eig diff {diffs[-1]} angle "
+                  f"{angles[-1] if angles else None}", file=sys.stderr, flush=True)
+            o_outs.append({"num_conv": o["num_conv"], "res": o["res"]})
+            del o
+        ko = kernel_stats(lib)
+        o_value = float(np.mean(o_times))
+        other = {"precision": other_prec, "value": round(o_value, 5), "unit": "s",
+                 "steps": other_steps, "iterations": o_iters,
+                 "all_converged": all(o["num_conv"] == nev for o in o_outs),
+                 "max_rel_eig_diff_vs_value": max(d for d in diffs if d is not None) if any(
+                     d is not None for d in diffs) else None,
+                 "max_principal_angle_vs_value": max(angles) if angles else None,
+                 "max_final_residual": float(max(np.max(o["res"]) for o in o_outs))}
+        fp64_s = o_value if other_prec == "fp64" else value
+        mixed_s = value if other_prec == "fp64" else o_value
+        other["mixed_speedup_vs_fp64"] = round(fp64_s / mixed_s, 3)
+        for name in ("spmm", "spmm32"):
+            if name in ko and ko[name]["ms"] > 0:
+                other[name + "_gbs"] = round(ko[name]["bytes"] / (ko[name]["ms"] * 1e-3) / 1e9, 1)
+
+    # per-kernel breakdown: one more solve, untimed, every launch bracketed
+    breakdown = {}
+    if not args.no_profile and not args.no_breakdown:
+        lib.GCGE_HIP_ProfileReset()
+        run(value_prec, 1, 1)
+        breakdown = kernel_stats(lib)
+
+    dom = "spmm32" if value_prec == "mixed" else "spmm"
+    roofline = None
+    if dom in kernels and kernels[dom]["ms"] > 0:
+        sp = kernels[dom]
+        achieved = sp["bytes"] / (sp["ms"] * 1e-3) / 1e9
+        key = "spmm32_cg" if dom == "spmm32" else "spmm64_k10"
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": read_traffic(E, lognormal, key + "_traffic_bytes"),
+                    "traffic_algorithmic_bytes": read_traffic(E, lognormal, key + "_algorithmic_bytes"),
+                    "traffic_note": "HBM bytes (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated) "
+                                    "of one isolated launch of this product at "
+                                    + ("the block-CG shape (float32, nev columns)" if dom == "spmm32"
+                                       else "10 columns (the block-CG shape)")
+                                    + ", next to that launch's algorithmic bytes",
+                    "kernel": ("k_dia_spmm float32 block-CG products (direction block times K, "
+                               "with p.w)" if dom == "spmm32" else "k_dia_spmm float64 products"),
+                    "launches": sp["launches"],
+                    "avg_launch_us": round(sp["ms"] * 1e3 / sp["launches"], 2),
+                    "algorithmic_bytes_per_launch": round(sp["bytes"] / sp["launches"]),
+                    "bytes_formula": "nnz*(s_val+4) + 4(n+1) + s_x*k*(n_cols+n_rows), k = columns "
+                                     "of the launch (the block CG records them on the device)"}
+    spmm_rates = {name: {"gbs": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1),
+                         "launches": v["launches"],
+                         "avg_launch_us": round(v["ms"] * 1e3 / v["launches"], 2)}
+                  for name, v in kernels.items() if name in ("spmm", "spmm32") and v["ms"] > 0}
+
+    last = outs[-1]
+    eig_err = None
+    if not lognormal:
+        exact = problems.q1_eigs(E, nev)
+        eig_err = float(np.max(np.abs(np.sort(last["eval"]) - exact) / exact))
+
+    cpu = None
+    if args.cpu == "model":
+        try:
+            Kh = K if K is not None else dK.to_csr()
+            Mh = M if M is not None else dM.to_csr()
+            cpu = cpu_model(Kh, Mh, work)
+            cpu["value"] = round(cpu["value"], 1)
+            if args.cpu_reference_iters > 0:
+                cpu["reference_run"] = cpu_reference_run(Kh, Mh, nev, args.tol, args.cpu_reference_iters,
+                                                         int(np.mean(iters)))
+            del Kh, Mh
+        except Exception as e:  # the baseline is reported, never required
+            print(f"# cpu baseline skipped: {e}", file=sys.stderr)
+
+    X = gcge.DeviceMultiVec(n, nev)
+    lib.GCGE_HIP_Synchronize()
+    t = time.perf_counter()
+    X.get()
+    down = time.perf_counter() - t
+    del X
+    line = {
+        "metric": METRIC,
+        "value": round(value, 5),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(value * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": ("f32 inner block CG (directions, products, residuals, correction), f64 Gram/"
+                  "orthonormalisation/Rayleigh-Ritz/residual check" if value_prec == "mixed" else "f64"),
+        "data": "synthetic",
+        "config": {"workload": f"Q1 FEM {E}^3 unit cube, "
+                               f"{'lognormal kappa (seed %d)' % args.kappa_seed if lognormal else 'kappa=1'}, "
+                               f"K x = lambda M x, nev={nev}"
+                               f"{', block 200 requested' if lognormal else ''}, "
+                               f"{'mixed precision' if value_prec == 'mixed' else 'FP64'}, "
+                               f"||r||/(|lambda| ||Mx||) <= {args.tol:g}",
+                   "precision": value_prec,
+                   "rayleigh_ritz": "reference soft locking" if args.reference_locking else "full (-gcge_full_rr 1)",
+                   "w_orth": "multi (reference)" if args.reference_orth else "svqb (-gcge_w_orth_type svqb)",
+                   "n": n, "nnz_K": dK.nnz, "nnz_M": dM.nnz, "nev": nev,
+                   "block_size": last["block_size"], "assembly": args.assembly,
+                   "parallelism": "1 GPU"},
+        "iterations": iters,
+        "all_converged": all(o["num_conv"] == nev for o in outs),
+        "max_final_residual": float(max(np.max(o["res"]) for o in outs)),
+        "max_rel_eig_err_vs_closed_form": eig_err,
+        "wall_s_timed_region": round(wall, 4),
+        "roofline": roofline,
+        "spmm_rates": spmm_rates,
+        "kernels_one_solve": {k: {"ms": round(v["ms"], 3), "launches": v["launches"]}
+                              for k, v in breakdown.items()},
+        "cpu_baseline": cpu,
+        "host_transfer": {"setup_s": round(setup_s, 4), "assembly": args.assembly,
+                          "evec_download_s": round(down, 5),
+                          "s_per_solve_incl_transfers": round(value + down, 4),
+                          "note": "setup = device assembly from kappa (or host CSR upload with "
+                                  "--assembly host), once; value excludes it and the download "
+                                  "of the eigenvectors (inputs resident)"},
+        "fp64" if other_prec == "fp64" else "mixed": other,
+    }
+    print(json.dumps(line), flush=True)
 
 
 def adjoint_workload(args):
@@ -559,6 +926,7 @@ def permuted_workload(args):
     print(json.dumps(line), flush=True)
 
 
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -566,160 +934,18 @@ def main():
               "multi-GPU row partition is a design, not built)", file=sys.stderr)
         sys.exit(2)
     import gcge_old_amd as gcge
-    from gcge_old_amd import _native, problems
 
     gcge.init(0)
     if args.mfma_probe:
         mfma_probe()
         return
-    lib = _native.hip()
     if args.workload == "q1-64-adjoint":
         adjoint_workload(args)
         return
     if args.workload == "q1-192-permuted":
         permuted_workload(args)
         return
-    if args.max_it is None:
-        args.max_it = 500
-    lognormal = args.workload == "q1-128-lognormal"
-    E = args.mesh or (128 if lognormal else 64)
-    if args.nev is None:
-        args.nev = 100 if lognormal else 50
-    t0 = time.time()
-    kappa = problems.lognormal_kappa(E, args.kappa_seed) if lognormal else None
-    K, M = problems.q1_fem_3d(E, kappa)
-    assemble_s = time.time() - t0
-    # the host CSR arrays cross PCIe once (with the host-side layout conversion)
-    t0 = time.perf_counter()
-    dK, dM = gcge.DeviceMatrix(K), gcge.DeviceMatrix(M)
-    lib.GCGE_HIP_Synchronize()
-    upload_s = time.perf_counter() - t0
-    flags = ["-gcge_conv_type", "RB", "-gcge_ev_tol", str(args.tol), "-gcge_ev_max_it", str(args.max_it)]
-    if lognormal:
-        flags += ["-gcge_block_size", str(2 * args.nev)]
-    if not args.reference_locking:
-        flags += ["-gcge_full_rr", "1"]
-    if not args.reference_orth:
-        flags += ["-gcge_w_orth_type", "svqb"]
-    print(f"# workload: Q1 {E}^3 {'lognormal kappa ' if lognormal else ''}n={K.nrows} nnz(K)={K.nnz} "
-          f"nnz(M)={M.nnz} nev={args.nev} (assembly {assemble_s:.1f}s)", file=sys.stderr, flush=True)
-    if args.pmc_probe:
-        pmc_probe(K)
-        return
-    if args.kernels:
-        kernel_sweep(K, M)
-
-    warm_launches = 0
-    for w in range(args.warmup):
-        lib.GCGE_HIP_ProfileReset()
-        lib.GCGE_HIP_ProfileEnable(0 if args.no_profile else 2)
-        out = gcge.solve(dK, dM, args.nev, flags, seed=100 + w)
-        lib.GCGE_HIP_ProfileEnable(0)
-        warm_launches = max(warm_launches, kernel_stats(lib).get("spmm", {}).get("launches", 0))
-        print(f"# warmup {w}: {out['time']:.4f}s iters={out['num_iter']} conv={out['num_conv']}",
-              file=sys.stderr, flush=True)
-
-    # timed region: events on the SpMM launches only (recorded by the
-    # dispatches themselves), so the solve time carries no profiling packets;
-    # the events are created beforehand (two per launch, with a margin)
-    lib.GCGE_HIP_ProfileReset()
-    if not args.no_profile:
-        lib.GCGE_HIP_ProfileReserve(int(2 * 1.5 * max(warm_launches, 2000) * args.steps))
-    lib.GCGE_HIP_ProfileEnable(0 if args.no_profile else 2)
-    lib.GCGE_HIP_Synchronize()
-    times, iters, outs = [], [], []
-    t_start = time.perf_counter()
-    for s in range(args.steps):
-        out = gcge.solve(dK, dM, args.nev, flags, seed=1 + s)
-        times.append(out["time"])
-        iters.append(out["num_iter"])
-        outs.append(out)
-        print(f"# step {s}: {out['time']:.4f}s iters={out['num_iter']} conv={out['num_conv']}",
-              file=sys.stderr, flush=True)
-    lib.GCGE_HIP_Synchronize()
-    wall = time.perf_counter() - t_start
-    lib.GCGE_HIP_ProfileEnable(0)
-    kernels = kernel_stats(lib)
-
-    # per-kernel breakdown: one more solve, untimed, every launch bracketed
-    breakdown = {}
-    if not args.no_profile:
-        lib.GCGE_HIP_ProfileReset()
-        lib.GCGE_HIP_ProfileEnable(1)
-        gcge.solve(dK, dM, args.nev, flags, seed=1)
-        lib.GCGE_HIP_ProfileEnable(0)
-        breakdown = kernel_stats(lib)
-
-    value = float(np.mean(times))
-    roofline = None
-    if "spmm" in kernels and kernels["spmm"]["ms"] > 0:
-        sp = kernels["spmm"]
-        achieved = sp["bytes"] / (sp["ms"] * 1e-3) / 1e9
-        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    # the counter summary covers the 64^3 K product only
-                    "traffic": read_traffic() if (E == 64 and not lognormal) else None,
-                    "traffic_note": "HBM bytes per SpMM launch on K with 10 columns (the block-CG "
-                                    "shape), rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a copy "
-                                    "calibration; algorithmic bytes of that launch (CSR formula): "
-                                    f"{K.nnz * 12 + 4 * (K.nrows + 1) + 160 * K.nrows}; the "
-                                    "diagonal-storage kernel reads no column indices",
-                    "kernel": "spmm", "launches": sp["launches"],
-                    "avg_launch_us": round(sp["ms"] * 1e3 / sp["launches"], 2),
-                    "algorithmic_bytes_per_launch": round(sp["bytes"] / sp["launches"])}
-
-    # accuracy of the last solve against the closed form (kappa = 1 only)
-    last = outs[-1]
-    eig_err = None
-    if not lognormal:
-        exact = problems.q1_eigs(E, args.nev)
-        eig_err = float(np.max(np.abs(np.sort(last["eval"]) - exact) / exact))
-
-    mixed = None
-    mixed_steps = args.steps if args.mixed_steps < 0 else args.mixed_steps
-    if mixed_steps > 0:
-        mixed = mixed_leg(gcge, lib, dK, dM, args.nev, flags, mixed_steps, outs, value)
-
-    cpu = None
-    if args.cpu_iters > 0:
-        try:
-            cpu = cpu_baseline(K, M, args.nev, args.tol, args.cpu_iters, int(np.mean(iters)))
-        except Exception as e:  # the baseline is reported, never required
-            print(f"# cpu baseline skipped: {e}", file=sys.stderr)
-
-    line = {
-        "metric": METRIC,
-        "value": round(value, 5),
-        "unit": "s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(value * 1e3, 3),
-        "higher_is_better": False,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": f"Q1 FEM {E}^3 unit cube, "
-                               f"{'lognormal kappa (seed %d)' % args.kappa_seed if lognormal else 'kappa=1'}, "
-                               f"K x = lambda M x, nev={args.nev}, FP64, "
-                               f"||r||/(|lambda| ||Mx||) <= {args.tol:g}",
-                   "rayleigh_ritz": "reference soft locking" if args.reference_locking else "full (-gcge_full_rr 1)",
-                   "w_orth": "multi (reference)" if args.reference_orth else "svqb (-gcge_w_orth_type svqb)",
-                   "n": K.nrows, "nnz_K": K.nnz, "nnz_M": M.nnz, "nev": args.nev,
-                   "block_size": last["block_size"], "parallelism": "1 GPU"},
-        "iterations": iters,
-        "all_converged": all(o["num_conv"] == args.nev for o in outs),
-        "max_rel_eig_err_vs_closed_form": eig_err,
-        "wall_s_timed_region": round(wall, 4),
-        "roofline": roofline,
-        "kernels_one_solve": {k: {"ms": round(v["ms"], 3), "launches": v["launches"]}
-                              for k, v in breakdown.items()},
-        "cpu_baseline": cpu,
-        "host_transfer": host_transfer(gcge, lib, K.nrows, args.nev, upload_s, value),
-        "mixed": mixed,
-    }
-    print(json.dumps(line), flush=True)
+    solve_workload(args)
 
 
 if __name__ == "__main__":

@@ -204,12 +204,16 @@ def solve(A, B, nev: int, flags: list[str] | None = None, x0: np.ndarray | None 
     x0p = None
     if x0 is not None:
         x0 = np.asfortranarray(x0, dtype=np.float64)
+        if x0.shape != (n, nev):
+            raise ValueError(f"x0 must have shape ({n}, {nev}), got {x0.shape}")
         x0p = _native.as_dptr(x0)
     rc = core.GCGE_SolveWithBackend(lib.GCGE_HIP_Backend(), Ad.handle,
                                     None if Bd is None else Bd.handle, n, nev, len(flags) + 1,
                                     argv, x0p, _native.as_dptr(ev), _native.as_dptr(vec),
                                     _native.as_dptr(res), _native.as_dptr(info),
                                     _native.as_dptr(hist) if history_cap else None, history_cap)
+    if rc == 2:
+        raise ValueError("a -gcge_nev flag must equal the nev argument")
     if rc != 0:
         raise RuntimeError("operations table incomplete")
     h = hist[: int(np.count_nonzero(hist[:, 0]))] if history_cap else None

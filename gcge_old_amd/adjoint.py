@@ -8,7 +8,7 @@ For B-normalised eigenpairs ``A x_c = lambda_c B x_c`` of symmetric matrices:
   Q1 stiffness ``K = sum_e kappa_e h Khat`` of ``problems.q1_fem_3d``,
   ``dL/dkappa_e = sum_c gbar_c x_c|e^T (h Khat) x_c|e``
   (``q1_kappa_gradient``);
-* eigenvector losses ``L(x)`` of a simple pair: with ``g = dL/dx`` solve
+* eigenvector losses ``L(x)`` of the smallest pair (simple): with ``g = dL/dx`` solve
   ``(A - lambda B) y = (I - B x x^T) g`` with ``x^T B y = 0``; then
   ``dL/dA = -(y x^T + x y^T)/2`` on the pattern and
   ``dL/dkappa_e = -y|e^T (h Khat) x|e`` (``q1_eigvec_kappa_gradient``; the
@@ -20,6 +20,8 @@ moves data and sets parameters.  ``Q1Eigenvalues`` wraps the solve and the
 eigenvalue adjoint as a ``torch.autograd.Function`` (float64).
 """
 from __future__ import annotations
+
+import ctypes as C
 
 import numpy as np
 
@@ -75,9 +77,20 @@ def q1_kappa_gradient(E: int, X, gbar, Y=None, node_row=None) -> np.ndarray:
 
 
 def eigvec_adjoint_solve(A: DeviceMatrix, B: DeviceMatrix, x: np.ndarray, lam: float,
-                         gx: np.ndarray, rate: float = 1e-12, max_it: int = 20000) -> np.ndarray:
+                         gx: np.ndarray, rate: float = 1e-12, max_it: int = 20000,
+                         index: int = 0) -> np.ndarray:
     """y with (A - lam B) y = (I - B x x^T) gx and x^T B y = 0 (device block
-    CG on the shifted matrix from y = 0, then the component along x removed)."""
+    CG on the shifted matrix from y = 0, then the component along x removed).
+
+    Only the smallest eigenpair (``index`` 0) is supported: conjugate
+    gradients need A - lam B positive semidefinite, which holds for the
+    smallest eigenvalue lam_1 of a pencil with B positive definite; for a
+    higher eigenvalue the shifted matrix is indefinite and CG can break down,
+    so any other index raises ValueError.  The linear-solver parameters in
+    force before the call are restored afterwards."""
+    if index != 0:
+        raise ValueError("the eigenvector adjoint is solved by CG on A - lam B, which is "
+                         "indefinite unless lam is the smallest eigenvalue (index 0)")
     lib = _native.hip()
     x = np.asarray(x, dtype=np.float64)
     n = len(x)
@@ -88,19 +101,26 @@ def eigvec_adjoint_solve(A: DeviceMatrix, B: DeviceMatrix, x: np.ndarray, lam: f
     rhs = gx - bx * (x @ gx)
     R = DeviceMultiVec.from_host(rhs[:, None])
     Y = DeviceMultiVec.from_host(np.zeros((n, 1)))
+    saved = (np.zeros(1, dtype=np.int32), np.zeros(1), np.zeros(1, dtype=np.int32), np.zeros(1),
+             C.c_void_p())
+    lib.GCGE_HIP_GetLinearSolverParams(_native.as_iptr(saved[0]), _native.as_dptr(saved[1]),
+                                       _native.as_iptr(saved[2]), _native.as_dptr(saved[3]),
+                                       C.byref(saved[4]))
     lib.GCGE_HIP_SetLinearSolverParams(max_it, rate, 1, -float(lam), B.handle)
     try:
         lib.GCGE_HIP_MultiLinearSolve(A.handle, R.handle, 0, Y.handle, 0, 1)
     finally:
-        lib.GCGE_HIP_SetLinearSolverParams(20, 1e-2, 0, 0.0, None)
+        lib.GCGE_HIP_SetLinearSolverParams(int(saved[0][0]), float(saved[1][0]), int(saved[2][0]),
+                                           float(saved[3][0]), saved[4].value)
     y = Y.get()[:, 0]
     return y - x * (bx @ y)
 
 
 def q1_eigvec_kappa_gradient(E: int, K: DeviceMatrix, M: DeviceMatrix, x: np.ndarray, lam: float,
                              gx: np.ndarray, **solve_args):
-    """dL/dkappa for a loss on the B-normalised eigenvector x of the simple
-    eigenvalue lam, gx = dL/dx; returns (gradient, y)."""
+    """dL/dkappa for a loss on the B-normalised eigenvector x of the smallest
+    eigenvalue lam (simple), gx = dL/dx; returns (gradient, y).  Other
+    eigenpairs are refused (see eigvec_adjoint_solve)."""
     y = eigvec_adjoint_solve(K, M, x, lam, gx, **solve_args)
     g = q1_kappa_gradient(E, x[:, None], np.array([-1.0]), Y=y[:, None])
     return g, y

@@ -10,9 +10,13 @@
  * here; the large tall-skinny products live in the device backend.
  *
  * Eigensolver: Householder reduction to tridiagonal form followed by the
- * implicit-shift QL iteration (the Wilkinson-Reinsch method, Handbook for
- * Automatic Computation vol. II, procedures tred2/tql2), then an ascending
- * sort.  GCGE_Default_DenseMatEigenSolver keeps the reference dispatch
+ * implicit-shift QL iteration, then an ascending sort.  The QL sweep
+ * (tridiag_ql below) follows the statement order of the tqli procedure in
+ * Press, Teukolsky, Vetterling and Flannery, Numerical Recipes in C, 2nd ed.,
+ * section 11.3 (itself a restatement of the Wilkinson-Reinsch tql2 of the
+ * Handbook for Automatic Computation vol. II), including its early exit when
+ * a rotation radius underflows to zero; the rotations of a sweep are recorded
+ * and applied to the eigenvector columns in one pass.  GCGE_Default_DenseMatEigenSolver keeps the reference dispatch
  * (src/gcge_ops.c:401-432): the whole spectrum when il == 1 and iu == n is
  * computed in place in `a` and then copied to evec as n*n contiguous
  * doubles (which is what the reference's memcpy after dsyev does, also

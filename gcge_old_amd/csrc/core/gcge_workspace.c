@@ -165,7 +165,13 @@ int GCGE_SolveWithBackend(const GCGE_BACKEND *backend, void *A, void *B, GCGE_IN
     s->para->print_result = 0;
     s->para->print_final_part_time = 0;
     GCGE_PARA_SetFromCommandLine(s->para, argc, argv);
-    nev = s->para->nev;
+    if (s->para->nev != nev) {
+        /* eval/evec/res/x0 are the caller's arrays of nev columns */
+        fprintf(stderr, "GCGE_SolveWithBackend: -gcge_nev %d differs from nev %d\n",
+                (int)s->para->nev, (int)nev);
+        GCGE_SOLVER_Free(&s);
+        return 2;
+    }
     backend->SetOps(s->ops);
     if (GCGE_OPS_Setup(s->ops) != 0) {
         GCGE_SOLVER_Free(&s);

@@ -693,6 +693,30 @@ static double dot_dev(int n, const double *x, const double *y)
     return h[0];
 }
 
+// ---- work counts in the reference's single-vector terms ------------------------------------------------
+// Every block operation of this backend counted as the CSR calls the
+// reference's column loops make for it (src/gcge_ops.c:115-309 over
+// app/csr/gcge_app_csr.c:297-378; block CG src/gcge_cg.c:302-501; residual
+// check src/gcge_eigsol.c:663-887): single-vector products (and their stored
+// entries), inner products and axpbys of length n, and the dense
+// eigensolves with the sum of their orders cubed.  Host-side counts only.
+enum WorkSlot { kWorkSpmv, kWorkSpmvNnz, kWorkSpmvRows, kWorkDot, kWorkAxpby, kWorkEig,
+                kWorkEigN3, kWorkVecLen, kWorkSlots };
+static double g_work[kWorkSlots];
+
+static void work_spmv(const Mat *A, double cols)
+{
+    g_work[kWorkSpmv] += cols;
+    g_work[kWorkSpmvNnz] += cols * A->nnz;
+    g_work[kWorkSpmvRows] += cols * A->nrows;
+}
+static void work_vec(double dots, double axpbys, int n)
+{
+    g_work[kWorkDot] += dots;
+    g_work[kWorkAxpby] += axpbys;
+    g_work[kWorkVecLen] = n;
+}
+
 // ---- operations table: single vectors ------------------------------------------------------------
 static void op_vec_random(void *vec, GCGE_OPS *)
 {
@@ -706,18 +730,21 @@ static void op_mat_dot_vec(void *mat, void *x, void *r, GCGE_OPS *)
     Mat *A = as_mat(mat);
     Vec *xv = as_vec(x), *rv = as_vec(r);
     if (xv->n != A->ncols || rv->n != A->nrows) fail_msg("MatDotVec: matrix and vector sizes differ");
+    work_spmv(A, 1);
     spmm(A, xv->data, padded_ld(A->ncols), rv->data, padded_ld(A->nrows), 1);
 }
 
 static void op_vec_axpby(double a, void *x, double b, void *y, GCGE_OPS *)
 {
     Vec *yv = as_vec(y);
+    work_vec(0, 1, yv->n);
     axpby(yv->n, a, as_vec(x)->data, yv->n, b, yv->data, yv->n, 1);
 }
 
 static void op_vec_inner(void *x, void *y, double *v, GCGE_OPS *)
 {
     Vec *xv = as_vec(x);
+    work_vec(1, 0, xv->n);
     *v = dot_dev(xv->n, xv->data, as_vec(y)->data);
 }
 
@@ -784,6 +811,7 @@ static void op_mat_dot_mv(void *mat, void **x, void **y, GCGE_INT *start, GCGE_I
     check_range(X, start[0], k, "MatDotMultiVec");
     check_range(Y, start[1], k, "MatDotMultiVec");
     if (X->n != A->ncols || Y->n != A->nrows) fail_msg("MatDotMultiVec: matrix and multivec sizes differ");
+    work_spmv(A, k);
     spmm(A, col_ptr(X, start[0]), X->ld, col_ptr(Y, start[1]), Y->ld, k);
 }
 
@@ -794,6 +822,7 @@ static void op_mv_axpby(double a, void **x, double b, void **y, GCGE_INT *start,
     const int k = end[0] - start[0];
     check_range(X, start[0], k, "MultiVecAxpby");
     check_range(Y, start[1], k, "MultiVecAxpby");
+    work_vec(0, k, Y->n);
     axpby(Y->n, a, col_ptr(X, start[0]), X->ld, b, col_ptr(Y, start[1]), Y->ld, k);
 }
 
@@ -803,6 +832,7 @@ static void op_mv_axpby_column(double a, void **x, GCGE_INT cx, double b, void *
     MultiVec *X = as_mv(x), *Y = as_mv(y);
     check_range(X, cx, 1, "MultiVecAxpbyColumn");
     check_range(Y, cy, 1, "MultiVecAxpbyColumn");
+    work_vec(0, 1, Y->n);
     axpby(Y->n, a, col_ptr(X, cx), X->ld, b, col_ptr(Y, cy), Y->ld, 1);
 }
 
@@ -817,6 +847,7 @@ static void op_mv_lincomb(void **x, void **y, GCGE_INT *start, GCGE_INT *end, do
     check_range(X, start[0], std::max(m, 0), "MultiVecLinearComb");
     check_range(Y, start[1], k, "MultiVecLinearComb");
     if (m <= 0) fail_msg("MultiVecLinearComb: empty input range");
+    work_vec(0, (double)m * k, Y->n);  // one axpby per coefficient (src/gcge_ops.c:190-221)
     // coefficients row by row, zero-padded to the kernel's column group
     const int g = lincomb_group(k);
     const int ldct = (k + g - 1) / g * g;
@@ -839,6 +870,7 @@ static void op_mv_inner(void **V, void **W, double *a, char *is_sym, GCGE_INT *s
     if (m <= 0 || k <= 0) return;
     check_range(Vm, start[0], m, "MultiVecInnerProd");
     check_range(Wm, start[1], k, "MultiVecInnerProd");
+    work_vec(sym ? 0.5 * k * (k + 1.0) : (double)m * k, 0, Vm->n);
     std::vector<double> g((size_t)m * k);
     gram(Vm->n, col_ptr(Vm, start[0]), Vm->ld, m, col_ptr(Wm, start[1]), Wm->ld, k, g.data(), m);
     if (!sym) {
@@ -945,6 +977,9 @@ static void bcg_iterate(Mat *A, Mat *B, int if_shift, double shift, int n, int l
     int niter = 1;
     while (num_unlock > 0 && niter < max_it) {
         const int nu = num_unlock;
+        // per column: p update, product, p.w, x and r updates, r.r
+        work_spmv(A, nu);
+        work_vec(2.0 * nu, 3.0 * nu, n);
         for (int id = 0; id < nu; ++id) {
             const int idx = unlock[id];
             maps[id] = p0 + idx;            // p
@@ -1115,6 +1150,11 @@ static void bcg_iterate_dev(Mat *A, int n, int len, PT *Rd, long ldr, int r0, do
     memcpy(last_err, h + 3 * len, (size_t)len * sizeof(double));
     *num_unlock = hi[len];
     memcpy(unlock, hi, (size_t)*num_unlock * sizeof(int));
+    for (int it = 0; it < steps; ++it) {
+        // per column of a step: p update, product, p.w, x and r updates, r.r
+        work_spmv(A, hi[len + 1 + it]);
+        work_vec(2.0 * hi[len + 1 + it], 3.0 * hi[len + 1 + it], n);
+    }
     for (int it = 0; it < steps; ++it)
         if (slots[it] >= 0) {
             const int cols = hi[len + 1 + it];
@@ -1183,6 +1223,8 @@ static void block_cg(void *Amat, int if_shift, double shift, void *Bmat, void **
     const long ldr = R->ld, ldx = X->ld;
 
     // r = A x (+ shift B x) into W, b -= r, rho2 = b.b   (all columns, float64)
+    work_spmv(A, len);
+    work_vec(len, len, n);
     spmm(A, Xd + (long)start[1] * ldx, ldx, Wd + (long)w0 * ldw, ldw, len);
     if (if_shift == 1) {
         if (!B) {
@@ -1275,6 +1317,9 @@ static void ext_residual_norms(void *Amat, void *Bmat, void **Xh, GCGE_INT start
     if (k <= 0) return;
     check_range(X, start, k, "MultiVecResidualNorms");
     const int n = X->n;
+    work_spmv(A, k);
+    if (Bmat) work_spmv(as_mat(Bmat), k);
+    work_vec(3.0 * k, k, n);
     long ldax, ldbx;
     double *AX = scratch(0, n, k, &ldax);
     const double *Xc = col_ptr(X, start);
@@ -1324,6 +1369,7 @@ static void ext_mat_dot_cols(void *mat, void **x, const GCGE_INT *xcols, void **
         check_range(Y, ycols[c], 1, "MatDotMultiVecCols");
     }
     if (X->n != A->ncols || Y->n != A->nrows) fail_msg("MatDotMultiVecCols: sizes differ");
+    work_spmv(A, k);
     int *d = upload_maps(xcols, ycols, k);
     spmm(A, X->data, X->ld, Y->data, Y->ld, k, d, d + k);
 }
@@ -1338,6 +1384,7 @@ static void ext_axpby_cols(const GCGE_DOUBLE *a, void **x, const GCGE_INT *xcols
         check_range(X, xcols[c], 1, "MultiVecAxpbyCols");
         check_range(Y, ycols[c], 1, "MultiVecAxpbyCols");
     }
+    work_vec(0, k, Y->n);
     int *d = upload_maps(xcols, ycols, k);
     std::vector<double> ab((size_t)2 * k);
     memcpy(ab.data(), a, (size_t)k * sizeof(double));
@@ -1363,6 +1410,8 @@ static DenseEig g_eig;
 static int ext_dense_eig(GCGE_DOUBLE *a, GCGE_INT lda, GCGE_INT n, GCGE_DOUBLE *eval,
                          GCGE_DOUBLE *evec, GCGE_INT lde, GCGE_INT il, GCGE_INT iu)
 {
+    g_work[kWorkEig] += 1;
+    g_work[kWorkEigN3] += (double)n * n * n;
     if (n < kDeviceEigMinN) return 1;
     State &s = state();
     DenseEig &e = g_eig;
@@ -1787,6 +1836,26 @@ void GCGE_HIP_SetLinearSolverParams(int max_it, double rate, int if_shift, doubl
     g_cg.if_shift = if_shift;
     g_cg.shift = shift;
     g_cg.B = B;
+}
+
+int GCGE_HIP_WorkCounts(double *out, int len, int reset)
+{
+    const int m = std::min(std::max(len, 0), (int)kWorkSlots);
+    if (out)
+        for (int i = 0; i < m; ++i) out[i] = g_work[i];
+    if (reset)
+        for (double &w : g_work) w = 0.0;
+    return kWorkSlots;
+}
+
+void GCGE_HIP_GetLinearSolverParams(int *max_it, double *rate, int *if_shift, double *shift,
+                                    void **B)
+{
+    if (max_it) *max_it = g_cg.max_it;
+    if (rate) *rate = g_cg.rate;
+    if (if_shift) *if_shift = g_cg.if_shift;
+    if (shift) *shift = g_cg.shift;
+    if (B) *B = g_cg.B;
 }
 
 void GCGE_HIP_MultiLinearSolve(void *mat, void **b, int bs, void **x, int xs, int k)

@@ -357,7 +357,9 @@ typedef struct GCGE_BACKEND_ {
  * which case num_init_evec = nev.  Outputs (each may be NULL): eval[nev],
  * evec[n x nev], res[nev] (last residuals), info[4] = {num_iter,
  * num_conv_continuous, solve seconds, block_size}, history[cap x (nev+2)].
- * Returns 0, or 1 when the operations table is incomplete. */
+ * The output arrays are sized by nev, so a -gcge_nev flag must repeat the
+ * nev argument.  Returns 0, 1 when the operations table is incomplete, or 2
+ * when a -gcge_nev flag differs from nev (nothing is solved). */
 int GCGE_SolveWithBackend(const GCGE_BACKEND *backend, void *A, void *B, GCGE_INT n,
                           GCGE_INT nev, int argc, char **argv, const GCGE_DOUBLE *x0,
                           GCGE_DOUBLE *eval, GCGE_DOUBLE *evec, GCGE_DOUBLE *res,

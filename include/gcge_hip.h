@@ -147,6 +147,18 @@ void GCGE_HIP_SetOps(GCGE_OPS *ops);
 /* Parameters MultiLinearSolver uses when a driver calls it without them
  * (reference defaults: cg_max_it 20, cg_rate 1e-2, no shift). */
 void GCGE_HIP_SetLinearSolverParams(int max_it, double rate, int if_shift, double shift, void *B);
+/* Work done since the last reset, counted as the single-vector CSR calls the
+ * reference's column loops would make for it (no reference counterpart: the
+ * input of a CPU cost model).  out[0..len) receives: SpMV calls, their stored
+ * entries summed, their rows summed, inner products, axpbys, dense
+ * eigensolves, the sum of their orders cubed, the vector length of the last
+ * counted vector operation.  reset != 0 clears the counts after reading.
+ * Returns the number of counts (8). */
+int GCGE_HIP_WorkCounts(double *out, int len, int reset);
+/* The parameters last set by GCGE_HIP_SetLinearSolverParams (outputs may be
+ * NULL), so that a caller can restore them. */
+void GCGE_HIP_GetLinearSolverParams(int *max_it, double *rate, int *if_shift, double *shift,
+                                    void **B);
 /* The table's MultiLinearSolver on columns [bs, bs+k) of b (right-hand
  * sides, overwritten by the residuals as in src/gcge_cg.c:302-501) and
  * [xs, xs+k) of x (initial guess in, solution out), with the parameters of

@@ -165,10 +165,14 @@ def _native_solve(_native, backend, Ah, Bh, n, nev, flags, x0, history_cap, keep
     x0p = None
     if x0 is not None:
         x0 = np.asfortranarray(x0, dtype=np.float64)
+        if x0.shape != (n, nev):
+            raise ValueError(f"x0 must have shape ({n}, {nev}), got {x0.shape}")
         x0p = _d(x0)
     rc = core.GCGE_SolveWithBackend(backend, Ah, Bh, n, nev, len(flags) + 1, argv, x0p, _d(ev),
                                     _d(vec), _d(res), _d(info), _d(hist) if history_cap else None,
                                     history_cap)
+    if rc == 2:
+        raise ValueError("a -gcge_nev flag must equal the nev argument")
     if rc != 0:
         raise RuntimeError("operations table incomplete")
     h = hist[: int(np.count_nonzero(hist[:, 0]))] if history_cap else None

@@ -83,6 +83,25 @@ def test_core_parameters_and_block_size_rules():
     assert np.all(np.isfinite(out["eval"]))
 
 
+def test_nev_flag_must_match_the_nev_argument():
+    """The output arrays are sized by the nev argument: a -gcge_nev flag with
+    another value is refused before any solve (it used to overwrite nev and
+    write past the caller's arrays); the same value is accepted.  x0 must be
+    n x nev."""
+    import numpy as np
+
+    import oracle
+    from gcge_old_amd import problems
+
+    A = problems.fd_laplacian_2d(12)
+    with pytest.raises(ValueError, match="gcge_nev"):
+        oracle.solve(A, None, 2, ["-gcge_nev", "40"])
+    out = oracle.solve(A, None, 4, ["-gcge_nev", "4", "-gcge_ev_max_it", "2"])
+    assert out["eval"].shape == (4,)
+    with pytest.raises(ValueError, match="x0 must have shape"):
+        oracle.solve(A, None, 4, ["-gcge_ev_max_it", "1"], x0=np.zeros((A.nrows, 3)))
+
+
 def test_torch_import_after_the_libraries():
     """Loading both libraries and then importing torch in one process must
     work (the libraries are opened with local symbol binding; opened
