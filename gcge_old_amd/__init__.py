@@ -114,12 +114,14 @@ class DeviceMatrix:
         "sliced" ELLPACK with element gathers, or sliced ELLPACK with "rows"
         gathers from a row-major copy of the column block (from 16 columns);
         True/False mean "diagonal"/"sliced".  "diagonal-consecutive" and
-        "diagonal-planes" fix the order of the diagonal row blocks (same
-        results bit for bit; "diagonal" chooses).  Returns whether diagonal
+        "diagonal-planes" fix the order of the diagonal row blocks,
+        "diagonal-tiles" uses the LDS plane tiles of 27-point plane layouts
+        for every column count (same products bit for bit; "diagonal"
+        chooses: plane tiles from 8 columns).  Returns whether diagonal
         storage is now used."""
         if storage is True or storage is False:
             storage = "diagonal" if storage else "sliced"
-        code = {"diagonal-consecutive": 3, "diagonal-planes": 4}.get(storage)
+        code = {"diagonal-consecutive": 3, "diagonal-planes": 4, "diagonal-tiles": 5}.get(storage)
         if code is None:
             code = self.STORAGES.index(storage)
         used = _native.hip().GCGE_HIP_MatSetStorage(self.handle, code)

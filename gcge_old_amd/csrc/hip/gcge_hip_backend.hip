@@ -275,6 +275,48 @@ static int dia_plane_stride(const DiaDesc &dd, int n)
     return 0;
 }
 
+// The 27-point table of a plane layout: line stride N (tried around the
+// centre of the (a, b) = (1, 0) run) such that every stored offset is
+// a N + b S + d with a, b, d in {-1, 0, 1}.
+static void dia_tile_table(Mat *m)
+{
+    m->dia_tile_ok = 0;
+    const int S = m->dia_plane;
+    const DiaDesc &dd = m->dia;
+    if (S <= 0 || dd.ngroups != 9) return;
+    for (int dN = -1; dN <= 1; ++dN) {
+        const int N = dd.center[5] + dN;
+        if (N <= 2 || S <= 2 * N + 2) continue;
+        int t[27], used = 0, stored = 0;
+        for (int j = 0; j < 27; ++j) t[j] = -1;
+        bool ok = true;
+        for (int g = 0; g < dd.ngroups && ok; ++g)
+            for (int q = 0; q < 3 && ok; ++q) {
+                if (dd.slot[g][q] < 0) continue;
+                ++stored;
+                const int off = dd.center[g] - 1 + q;
+                // off = a N + b S + d
+                int b = off > S / 2 ? 1 : (off < -S / 2 ? -1 : 0);
+                int r = off - b * S;
+                int a = r > N / 2 ? 1 : (r < -N / 2 ? -1 : 0);
+                int d = r - a * N;
+                if (d < -1 || d > 1) {
+                    ok = false;
+                    break;
+                }
+                const int j = 9 * (b + 1) + 3 * (a + 1) + d + 1;
+                if (t[j] >= 0) ok = false;
+                t[j] = dd.slot[g][q];
+                ++used;
+            }
+        if (!ok || used != stored) continue;
+        m->dia_N = N;
+        memcpy(m->tile27, t, sizeof(t));
+        m->dia_tile_ok = 1;
+        return;
+    }
+}
+
 static void build_dia(Mat *m, const int *rowptr, const int *colidx, const double *vals)
 {
     const int n = m->nrows;
@@ -319,6 +361,7 @@ static void build_dia(Mat *m, const int *rowptr, const int *colidx, const double
     HIP_CHECK(hipStreamSynchronize(state().stream));
     m->dia = dd;
     m->dia_plane = dia_plane_stride(dd, n);
+    dia_tile_table(m);
     m->ndiag = nd;
     m->use_dia = 1;
 }
@@ -537,6 +580,7 @@ Mat *create_mat_q1(int E, const double *kappa, int which, const int *node_row)
         HIP_CHECK(hipMemsetAsync(m->d_dia, 0, (size_t)doffs.nd * dia_ld * sizeof(double), st.stream));
         m->dia = dd;
         m->dia_plane = dia_plane_stride(dd, n);
+        dia_tile_table(m);
         m->ndiag = doffs.nd;
         m->use_dia = 1;
     }
@@ -1076,8 +1120,8 @@ static void block_to_double(const float *src, long lds, double *dst, long ldd, i
 // included: the reference reads positions this call never writes) go up
 // once and come back once, so dtmp/itmp end as after the host loop.
 // d_rr0 holds the initial r.r of the len columns.
-template <typename PT>
-static void bcg_iterate_dev(Mat *A, int n, int len, PT *Rd, long ldr, int r0, double *Xd,
+template <typename PT, typename XT>
+static void bcg_iterate_dev(Mat *A, int n, int len, PT *Rd, long ldr, int r0, XT *Xd,
                             long ldx, int x0, PT *Pd, long ldp, int p0, PT *Wd, long ldw, int w0,
                             int max_it, double rate, double *rho1, double *rho2, double *err,
                             double *last_err, int *unlock, int *num_unlock, double *d_st,
@@ -1245,19 +1289,25 @@ static void block_cg(void *Amat, int if_shift, double shift, void *Bmat, void **
                 d_st + 12 * len);
         int num_unlock = 0;
         if (f32) {
-            long lpf, lwf, lrf;
+            // float32 directions, products and residuals, and a float32
+            // correction D of the initial guess: x = x0 + D is formed once at
+            // the end (x0 keeps its float64 digits, D is small against it)
+            long lpf, lwf, lrf, ldf;
             float *Pf = scratch_f32(0, n, len, &lpf);
             float *Wf = scratch_f32(1, n, len, &lwf);
+            float *Df = scratch_f32(2, n, len, &ldf);
             float *Rf = scratch_f32(3, n, len, &lrf);
             block_to_float(Rd + (long)start[0] * ldr, ldr, Rf, lrf, n, len);
-            bcg_iterate_dev<float>(A, n, len, Rf, lrf, 0, Xd, ldx, start[1], Pf, lpf, 0, Wf, lwf, 0,
-                                   max_it, rate, rho1, rho2, err, last_err, unlock, &num_unlock,
-                                   d_st, d_it, by_column);
+            zero_float(n, Df, ldf, len);
+            bcg_iterate_dev<float, float>(A, n, len, Rf, lrf, 0, Df, ldf, 0, Pf, lpf, 0, Wf, lwf, 0,
+                                          max_it, rate, rho1, rho2, err, last_err, unlock,
+                                          &num_unlock, d_st, d_it, by_column);
+            add_float(n, Df, ldf, Xd + (long)start[1] * ldx, ldx, len);
             block_to_double(Rf, lrf, Rd + (long)start[0] * ldr, ldr, n, len);
         } else {
-            bcg_iterate_dev<double>(A, n, len, Rd, ldr, start[0], Xd, ldx, start[1], Pd, ldp, p0, Wd,
-                                    ldw, w0, max_it, rate, rho1, rho2, err, last_err, unlock,
-                                    &num_unlock, d_st, d_it, by_column);
+            bcg_iterate_dev<double, double>(A, n, len, Rd, ldr, start[0], Xd, ldx, start[1], Pd, ldp,
+                                            p0, Wd, ldw, w0, max_it, rate, rho1, rho2, err, last_err,
+                                            unlock, &num_unlock, d_st, d_it, by_column);
         }
         return;
     }
@@ -1693,9 +1743,9 @@ int GCGE_HIP_MatSetStorage(void *mat, int storage)
 {
     Mat *m = as_mat(mat);
     HIP_CHECK(hipStreamSynchronize(state().stream));
-    m->use_dia = ((storage == 1 || storage == 3 || storage == 4) && m->d_dia) ? 1 : 0;
+    m->use_dia = ((storage == 1 || storage == 3 || storage == 4 || storage == 5) && m->d_dia) ? 1 : 0;
     m->use_rows = storage == 2 ? 1 : 0;
-    m->dia_order = storage == 3 ? 1 : (storage == 4 ? 2 : 0);
+    m->dia_order = storage == 3 ? 1 : (storage == 4 ? 2 : (storage == 5 ? 3 : 0));
     return m->use_dia ? 1 : (m->use_rows ? 2 : 0);
 }
 

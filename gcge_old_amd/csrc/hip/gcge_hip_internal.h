@@ -110,8 +110,16 @@ struct Mat {
     int dia_plane;  // rows per plane of a 27-point pattern (centers a N + b S,
                     // a, b in {-1, 0, 1}, S = dia_plane dividing nrows), else 0
     int dia_order;  // row blocks of the diagonal products: 0 automatic (plane
-                    // order from a plane stride of 8192 rows), 1 consecutive,
-                    // 2 plane order whenever dia_plane > 0
+                    // tiles from kTileMinCols columns, else plane order from a
+                    // plane stride of 8192 rows), 1 consecutive, 2 plane order
+                    // whenever dia_plane > 0, 3 plane tiles whenever possible
+    // the same layout as a 27-point plane stencil (dia_plane > 0): line
+    // stride N and, for offset a N + b S + d (a, b, d in {-1, 0, 1}), the
+    // diagonal holding it in tile27[9 (b + 1) + 3 (a + 1) + d + 1] (-1: none);
+    // dia_tile_ok when every stored diagonal has its place there
+    int dia_N;
+    int dia_tile_ok;
+    int tile27[27];
     int ndiag;
     double *d_dia;
     float *d_dia32;
@@ -160,6 +168,9 @@ void spmm(const Mat *A, const float *X, long ldx, float *Y, long ldy, int k,
           const int *d_xmap = nullptr, const int *d_ymap = nullptr);
 void to_float(const double *src, float *dst, size_t len);
 void to_double(const float *src, double *dst, size_t len);
+// X(:, c) += D(:, c), c < k (float32 into float64); D = 0 (float32 block)
+void add_float(int n, const float *D, long ldd, double *X, long ldx, int k);
+void zero_float(int n, float *X, long ldx, int k);
 // block-CG product: W(:, wmap[c]) = A P(:, pmap[c]) and, from the same pass,
 // d_out[c] = P(:, pmap[c]) . W(:, wmap[c]); the float32 form also gives
 // d_out[k + c] = P(:, pmap[c]) . P(:, pmap[c]) (sums in float64)
@@ -193,7 +204,9 @@ void coldots(int n, const double *A, long lda, const int *d_amap, const double *
 void residual_sq(int n, const double *AX, long ldax, const double *BX, long ldbx, const double *X,
                  long ldx, const double *d_lam, int k, double *d_out);
 // block CG pieces (column lists on the device); the direction block P, its
-// product W and the residual block R are float64 or float32, x stays float64.  The float32
+// product W and the residual block R are float64 or float32; x is float64,
+// or, in the device-driven float32 form, a float32 correction that is added
+// to the float64 initial guess once at the end.  The float32
 // directions are stored scaled, p = s p_stored with per-column s (d_sold,
 // d_snew, d_scale: one entry per listed column).
 void bcg_p_update(int n, double *P, long ldp, const int *d_pmap, const double *R, long ldr,
@@ -211,6 +224,12 @@ void bcg_xr_update(int n, double *X, long ldx, const int *d_xmap, double *R, lon
                    const double *d_ptw, int k, double *d_rr_out, const int *d_count = nullptr,
                    const double **rr_part = nullptr, int *rr_nbx = nullptr);
 void bcg_xr_update(int n, double *X, long ldx, const int *d_xmap, float *R, long ldr,
+                   const int *d_rmap, const float *P, long ldp, const int *d_pmap, const float *W,
+                   long ldw, const int *d_wmap, const double *d_rho2, const double *d_ptw,
+                   const double *d_scale, int k, double *d_rr_out, const int *d_count = nullptr,
+                   const double **rr_part = nullptr, int *rr_nbx = nullptr);
+// the same with a float32 x (the correction of the initial guess)
+void bcg_xr_update(int n, float *X, long ldx, const int *d_xmap, float *R, long ldr,
                    const int *d_rmap, const float *P, long ldp, const int *d_pmap, const float *W,
                    long ldw, const int *d_wmap, const double *d_rho2, const double *d_ptw,
                    const double *d_scale, int k, double *d_rr_out, const int *d_count = nullptr,

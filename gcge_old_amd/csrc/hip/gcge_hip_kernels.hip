@@ -521,6 +521,255 @@ void launch_dia_spmm(const Mat *A, const VT *vals, const XT *X, long ldx, XT *Y,
     HIP_CHECK(hipGetLastError());
 }
 
+// ---- SpMM on diagonal storage, plane tiles (wide column blocks) ------------------------
+// For a diagonal layout that is a 27-point plane stencil (offsets a N + b S + d,
+// a, b, d in {-1, 0, 1}; S rows per plane, N per line).  A workgroup of 256
+// threads owns in-plane rows [q0, q0 + 256) of PZ consecutive planes; thread
+// t computes in-plane row q0 + t of each of them and keeps their PZ x 27
+// coefficients in registers for every column.  Per chunk of CK columns the
+// PZ + 2 planes the tile reads (in-plane rows q0 - N - 1 .. q0 + 256 + N) are
+// staged in LDS once, the CK values of a position side by side (one vector
+// LDS access per stencil read); each staged value serves the stencil
+// positions of up to 27 rows of the tile.  Two workgroups per CU overlap one
+// chunk's global loads with the other's products.  Positions outside the grid
+// are staged as zero.  Products are added in increasing column order
+// (b, then a, then d), as the CSR rows list them and as k_dia_spmm adds
+// them; diagonals the matrix lacks contribute zero products (finite X), so
+// both kernels give the same sums.
+// Workgroup b runs on XCD b % 8; XCD x takes plane groups x, x + 8, ...,
+// each with all its in-plane tiles in order: the in-plane halo of a tile is
+// read by its neighbours on the same XCD, the planes shared by two plane
+// groups are read by neighbouring XCDs at the same time.
+constexpr int kTileRows = 256;
+struct TileDesc {
+    int S, N, planes, tq, groups, nparts;  // strides, planes, in-plane tiles, plane groups,
+                                           // partial sums per column (DOT)
+    int slot[27];
+};
+
+template <typename XT, int CK>
+struct alignas(sizeof(XT) * CK) TilePack {
+    XT v[CK];
+};
+
+template <typename VT, typename XT, int PZ, int CK, int J, int DOT>
+__global__ __launch_bounds__(kTileRows, 2) void k_dia_tile(TileDesc td, const VT *__restrict__ dval,
+                                                          long ldd, const XT *__restrict__ X, long ldx,
+                                                          XT *__restrict__ Y, long ldy, int k,
+                                                          const int *__restrict__ xmap,
+                                                          const int *__restrict__ ymap,
+                                                          double *__restrict__ dpart,
+                                                          const int *__restrict__ count)
+{
+    using Pack = TilePack<XT, CK>;
+    extern __shared__ __attribute__((aligned(16))) unsigned char tile_smem[];
+    Pack *lds = reinterpret_cast<Pack *>(tile_smem);
+    const int xcd = blockIdx.x % kXcds, loc = blockIdx.x / kXcds;
+    const int zg = kXcds * (loc / td.tq) + xcd, qt = loc % td.tq;
+    if (zg >= td.groups) return;
+    const int W = kTileRows + 2 * td.N + 2;
+    const int t = threadIdx.x;
+    const int q0 = qt * kTileRows, z0 = zg * PZ;
+    const int q = q0 + t;
+    const long S = td.S;
+    bool own[PZ];
+    XT v[PZ][27];
+#pragma unroll
+    for (int p = 0; p < PZ; ++p) {
+        own[p] = q < td.S && z0 + p < td.planes;
+        const long row = (z0 + p) * S + q;
+#pragma unroll
+        for (int j = 0; j < 27; ++j) {
+            const int s = td.slot[j];
+            v[p][j] = (s >= 0 && own[p]) ? (XT)dval[(long)s * ldd + row] : XT(0);
+        }
+    }
+    const int kc = count ? min(k, *count) : k;
+    // staged positions: plane pl (grid plane z0 - 1 + pl), in-plane offset
+    // o = t + 256 jj (in-plane row q0 - N - 1 + o)
+    Pack stage[PZ + 2][J];
+    auto fetch = [&](int c0) {
+        const XT *col[CK];
+#pragma unroll
+        for (int cc = 0; cc < CK; ++cc) {
+            const int c = min(c0 + cc, kc - 1);
+            col[cc] = X + (long)(xmap ? xmap[c] : c) * ldx;
+        }
+#pragma unroll
+        for (int pl = 0; pl < PZ + 2; ++pl) {
+            const int z = z0 - 1 + pl;
+#pragma unroll
+            for (int jj = 0; jj < J; ++jj) {
+                const int o = t + kTileRows * jj;
+                const int ip = q0 - td.N - 1 + o;
+                const bool ok = o < W && z >= 0 && z < td.planes && ip >= 0 && ip < td.S;
+                const long g = ok ? (long)z * S + ip : 0;
+#pragma unroll
+                for (int cc = 0; cc < CK; ++cc)
+                    stage[pl][jj].v[cc] = (ok && c0 + cc < kc) ? col[cc][g] : XT(0);
+            }
+        }
+    };
+    for (int c0 = 0; c0 < kc; c0 += CK) {
+        fetch(c0);
+        __syncthreads();  // the previous chunk's products are done with the LDS
+#pragma unroll
+        for (int pl = 0; pl < PZ + 2; ++pl)
+#pragma unroll
+            for (int jj = 0; jj < J; ++jj) {
+                const int o = t + kTileRows * jj;
+                if (o < W) lds[pl * W + o] = stage[pl][jj];
+            }
+        __syncthreads();
+        XT acc[PZ][CK];
+#pragma unroll
+        for (int p = 0; p < PZ; ++p)
+#pragma unroll
+            for (int cc = 0; cc < CK; ++cc) acc[p][cc] = XT(0);
+        const int base = t + td.N + 1;
+#pragma unroll
+        for (int pl = 0; pl < PZ + 2; ++pl) {
+            // the nine staged values of this plane, then their products (a
+            // diagonal the matrix lacks has zero coefficients)
+            Pack xv[3][3];
+#pragma unroll
+            for (int a = -1; a <= 1; ++a)
+#pragma unroll
+                for (int d = -1; d <= 1; ++d) xv[a + 1][d + 1] = lds[pl * W + base + a * td.N + d];
+#pragma unroll
+            for (int a = -1; a <= 1; ++a)
+#pragma unroll
+                for (int d = -1; d <= 1; ++d)
+#pragma unroll
+                    for (int b = -1; b <= 1; ++b) {
+                        const int p = pl - 1 - b;
+                        if (p < 0 || p >= PZ) continue;
+                        const int j = 9 * (b + 1) + 3 * (a + 1) + d + 1;
+#pragma unroll
+                        for (int cc = 0; cc < CK; ++cc) acc[p][cc] += v[p][j] * xv[a + 1][d + 1].v[cc];
+                    }
+            // the sums as they stand after this plane (an empty asm that reads
+            // and writes them): the products are not sunk to the stores, so
+            // only this plane's nine vectors are live
+#pragma unroll
+            for (int p = 0; p < PZ; ++p)
+#pragma unroll
+                for (int cc = 0; cc < CK; ++cc) asm volatile("" : "+v"(acc[p][cc]));
+        }
+        const int cn = min(CK, kc - c0);
+#pragma unroll
+        for (int cc = 0; cc < CK; ++cc) {
+            if (cc >= cn) continue;
+            XT *y = Y + (long)(ymap ? ymap[c0 + cc] : c0 + cc) * ldy;
+#pragma unroll
+            for (int p = 0; p < PZ; ++p)
+                if (own[p]) y[(z0 + p) * S + q] = acc[p][cc];
+        }
+        if constexpr (DOT > 0) {
+            const int part = (zg * td.tq + qt) * (kTileRows / kWave) + t / kWave;
+#pragma unroll
+            for (int cc = 0; cc < CK; ++cc) {
+                if (cc >= cn) continue;
+                double pw = 0.0, pp = 0.0;
+#pragma unroll
+                for (int p = 0; p < PZ; ++p) {
+                    const double pv = own[p] ? (double)lds[(p + 1) * W + base].v[cc] : 0.0;
+                    pw += pv * (own[p] ? (double)acc[p][cc] : 0.0);
+                    if constexpr (DOT > 1) pp += pv * pv;
+                }
+                pw = wave_sum(pw);
+                if constexpr (DOT > 1) pp = wave_sum(pp);
+                if (t % kWave == 0) {
+                    dpart[(long)(c0 + cc) * td.nparts + part] = pw;
+                    if constexpr (DOT > 1) dpart[(long)(k + c0 + cc) * td.nparts + part] = pp;
+                }
+            }
+        }
+    }
+}
+
+// Plane tiles apply to matrices with a complete 27-point table from
+// kTileMinCols columns (automatic order), or always (order 3).  Planes per
+// tile: 4 for float32 values, 2 for float64 (register budget of the
+// coefficients); GCGE_HIP_TILE_PZ and GCGE_HIP_TILE_MIN override (for
+// measurement).
+constexpr int kTileMinCols = 8;
+inline int tile_min_cols()
+{
+    static const int v = [] {
+        const char *e = getenv("GCGE_HIP_TILE_MIN");
+        return e ? atoi(e) : kTileMinCols;
+    }();
+    return v;
+}
+template <typename XT>
+inline int tile_pz()
+{
+    if (sizeof(XT) == 8) return 2;
+    static const int v = [] {
+        const char *e = getenv("GCGE_HIP_TILE_PZ");
+        return (e && atoi(e) == 2) ? 2 : 4;
+    }();
+    return v;
+}
+template <typename XT>
+inline bool use_tiles(const Mat *A, int k)
+{
+    if (!A->use_dia || !A->dia_tile_ok || A->dia_plane <= 0 || A->dia_order == 1 || A->dia_order == 2)
+        return false;
+    // staged planes x positions x 16 bytes within 64 KiB of LDS
+    const long W = kTileRows + 2L * A->dia_N + 2;
+    if (W > 3 * kTileRows || (tile_pz<XT>() + 2) * W * 16 > 65536) return false;
+    return A->dia_order == 3 || k >= tile_min_cols();
+}
+template <typename XT>
+inline TileDesc tile_desc(const Mat *A)
+{
+    TileDesc td{};
+    td.S = A->dia_plane;
+    td.N = A->dia_N;
+    td.planes = A->nrows / A->dia_plane;
+    td.tq = (td.S + kTileRows - 1) / kTileRows;
+    td.groups = (td.planes + tile_pz<XT>() - 1) / tile_pz<XT>();
+    td.nparts = td.groups * td.tq * (kTileRows / kWave);
+    for (int j = 0; j < 27; ++j) td.slot[j] = A->tile27[j];
+    return td;
+}
+template <typename XT>
+inline int tile_parts(const Mat *A) { return tile_desc<XT>(A).nparts; }
+
+template <int DOT, typename VT, typename XT>
+void launch_dia_tile(const Mat *A, const VT *vals, const XT *X, long ldx, XT *Y, long ldy, int k,
+                     const int *d_xmap, const int *d_ymap, double *dpart, const int *d_count,
+                     hipEvent_t ev0, hipEvent_t ev1)
+{
+    constexpr int CK = 16 / sizeof(XT);
+    const TileDesc td = tile_desc<XT>(A);
+    const int W = kTileRows + 2 * td.N + 2;
+    const int J = (W + kTileRows - 1) / kTileRows;
+    const int pz = tile_pz<XT>();
+    const int rounds = (td.groups + kXcds - 1) / kXcds;
+    const dim3 grid(td.tq * rounds * kXcds);
+    const size_t lds = (size_t)(pz + 2) * W * 16;
+    const long ldd = (long)A->nslices * kWave;
+#define GCGE_TILE_LAUNCH(PZ, JJ)                                                                    \
+    hipExtLaunchKernelGGL((k_dia_tile<VT, XT, PZ, CK, JJ, DOT>), grid, dim3(kTileRows), lds,         \
+                          state().stream, ev0, ev1, 0, td, vals, ldd, X, ldx, Y, ldy, k, d_xmap,    \
+                          d_ymap, dpart, d_count)
+    if constexpr (sizeof(XT) == 4) {
+        if (pz == 4) {
+            if (J <= 2) GCGE_TILE_LAUNCH(4, 2);
+            else GCGE_TILE_LAUNCH(4, 3);
+            HIP_CHECK(hipGetLastError());
+            return;
+        }
+    }
+    if (J <= 2) GCGE_TILE_LAUNCH(2, 2);
+    else GCGE_TILE_LAUNCH(2, 3);
+#undef GCGE_TILE_LAUNCH
+    HIP_CHECK(hipGetLastError());
+}
+
 // ---- SpMM with whole-row gathers (matrices with scattered columns) ------------------
 // The column block X (column-major, optional column map) is first copied to a
 // row-major block XT (row i: the k values of row i, row stride kp), through
@@ -743,6 +992,22 @@ __global__ __launch_bounds__(kBlock) void k_to_float(size_t len, const double *_
 {
     const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
     if (i < len) dst[i] = (float)src[i];
+}
+
+// X(:, c) += D(:, c) for c < k (float32 correction into a float64 block)
+__global__ __launch_bounds__(kBlock) void k_add_float(int n, const float *__restrict__ D, long ldd,
+                                                     double *__restrict__ X, long ldx)
+{
+    const int row = blockIdx.x * kBlock + threadIdx.x;
+    const int c = blockIdx.y;
+    if (row < n) X[(long)c * ldx + row] += (double)D[(long)c * ldd + row];
+}
+
+// X(:, c) = 0 for c < k (float32 block)
+__global__ __launch_bounds__(kBlock) void k_zero_float(int n, float *__restrict__ X, long ldx)
+{
+    const int row = blockIdx.x * kBlock + threadIdx.x;
+    if (row < n) X[(long)blockIdx.y * ldx + row] = 0.0f;
 }
 
 __global__ __launch_bounds__(kBlock) void k_to_double(size_t len, const float *__restrict__ src,
@@ -1420,9 +1685,9 @@ __global__ __launch_bounds__(kBlock) void k_coldots_pw(int n, int rows_per_block
 // (float64 P), or alpha = rho2[c] / (scale[c] ptw[c]) for the scaled float32
 // directions (p = scale p_stored, ptw the stored p . A p_stored); x float64,
 // r float64 or float32 (with float32 directions); arithmetic and r.r in float64
-template <typename PT, typename RT>
+template <typename PT, typename RT, typename XT>
 __global__ __launch_bounds__(kBlock) void k_bcg_xr(int n, int rows_per_block,
-                                                  double *__restrict__ X, long ldx,
+                                                  XT *__restrict__ X, long ldx,
                                                   const int *__restrict__ xmap,
                                                   RT *__restrict__ R, long ldr,
                                                   const int *__restrict__ rmap,
@@ -1440,14 +1705,14 @@ __global__ __launch_bounds__(kBlock) void k_bcg_xr(int n, int rows_per_block,
     const int bx = blockIdx.x, c = blockIdx.y;
     if (count && c >= *count) return;
     const double alpha = scale ? rho2[c] / (scale[c] * ptw[c]) : rho2[c] / ptw[c];
-    double *x = X + (long)xmap[c] * ldx;
+    XT *x = X + (long)xmap[c] * ldx;
     RT *r = R + (long)rmap[c] * ldr;
     const PT *p = P + (long)pmap[c] * ldp;
     const PT *w = W + (long)wmap[c] * ldw;
     const int r0 = bx * rows_per_block, r1 = min(n, r0 + rows_per_block);
     double s = 0.0;
     for (int i = r0 + threadIdx.x; i < r1; i += kBlock) {
-        x[i] = alpha * (double)p[i] + x[i];
+        x[i] = (XT)(alpha * (double)p[i] + (double)x[i]);
         const RT rn = (RT)(-alpha * (double)w[i] + (double)r[i]);
         r[i] = rn;
         s += (double)rn * (double)rn;
@@ -1634,7 +1899,10 @@ void spmm(const Mat *A, const double *X, long ldx, double *Y, long ldy, int k, c
     const double bytes = (double)A->nnz * (8 + 4) + 4.0 * (A->nrows + 1) +
                          8.0 * (double)k * (A->ncols + A->nrows);
     ProfScope prof("spmm", bytes, true);
-    if (A->use_dia)
+    if (A->use_dia && use_tiles<double>(A, k))
+        launch_dia_tile<0>(A, A->d_dia, X, ldx, Y, ldy, k, d_xmap, d_ymap, nullptr, nullptr,
+                           prof.start_event(), prof.stop_event());
+    else if (A->use_dia)
         launch_dia_spmm<0>(A, A->d_dia, X, ldx, Y, ldy, k, d_xmap, d_ymap, nullptr, nullptr,
                            prof.start_event(), prof.stop_event());
     else if (A->use_rows && k >= kRowsMinCols)
@@ -1653,7 +1921,10 @@ void spmm(const Mat *A, const float *X, long ldx, float *Y, long ldy, int k, con
     const double bytes = (double)A->nnz * (4 + 4) + 4.0 * (A->nrows + 1) +
                          4.0 * (double)k * (A->ncols + A->nrows);
     ProfScope prof("spmm32", bytes, true);
-    if (A->use_dia && A->d_dia32)
+    if (A->use_dia && A->d_dia32 && use_tiles<float>(A, k))
+        launch_dia_tile<0>(A, A->d_dia32, X, ldx, Y, ldy, k, d_xmap, d_ymap, nullptr, nullptr,
+                           prof.start_event(), prof.stop_event());
+    else if (A->use_dia && A->d_dia32)
         launch_dia_spmm<0>(A, A->d_dia32, X, ldx, Y, ldy, k, d_xmap, d_ymap, nullptr, nullptr,
                            prof.start_event(), prof.stop_event());
     else if (A->use_rows && k >= kRowsMinCols)
@@ -1672,12 +1943,17 @@ void spmm_dot(const Mat *A, const double *P, long ldp, double *W, long ldw, int 
     const double bytes = (double)A->nnz * (8 + 4) + 4.0 * (A->nrows + 1) +
                          8.0 * (double)k * (A->ncols + 2.0 * A->nrows);
     const bool rows = !A->use_dia && A->use_rows && k >= kRowsMinCols;
-    const int nparts = A->use_dia ? dia_blocks(A) : (rows ? rows_parts(A) : A->nslices);
+    const bool tiles = A->use_dia && use_tiles<double>(A, k);
+    const int nparts = tiles ? tile_parts<double>(A)
+                             : (A->use_dia ? dia_blocks(A) : (rows ? rows_parts(A) : A->nslices));
     double *dpart = partial_buffer((size_t)nparts * k);
     {
         ProfScope prof("spmm", bytes, true);
         if (prof_slot) *prof_slot = prof.slot;
-        if (A->use_dia)
+        if (tiles)
+            launch_dia_tile<1>(A, A->d_dia, P, ldp, W, ldw, k, d_pmap, d_wmap, dpart, d_count,
+                               prof.start_event(), prof.stop_event());
+        else if (A->use_dia)
             launch_dia_spmm<1>(A, A->d_dia, P, ldp, W, ldw, k, d_pmap, d_wmap, dpart, d_count,
                                prof.start_event(), prof.stop_event());
         else if (rows)
@@ -1701,12 +1977,17 @@ void spmm_dot(const Mat *A, const float *P, long ldp, float *W, long ldw, int k,
                          4.0 * (double)k * (A->ncols + 2.0 * A->nrows);
     const bool dia = A->use_dia && A->d_dia32;
     const bool rows = !dia && A->use_rows && k >= kRowsMinCols;
-    const int nparts = dia ? dia_blocks(A) : (rows ? rows_parts(A) : A->nslices);
+    const bool tiles = dia && use_tiles<float>(A, k);
+    const int nparts = tiles ? tile_parts<float>(A)
+                             : (dia ? dia_blocks(A) : (rows ? rows_parts(A) : A->nslices));
     double *dpart = partial_buffer((size_t)nparts * 2 * k);
     {
         ProfScope prof("spmm32", bytes, true);
         if (prof_slot) *prof_slot = prof.slot;
-        if (dia)
+        if (tiles)
+            launch_dia_tile<2>(A, A->d_dia32, P, ldp, W, ldw, k, d_pmap, d_wmap, dpart, d_count,
+                               prof.start_event(), prof.stop_event());
+        else if (dia)
             launch_dia_spmm<2>(A, A->d_dia32, P, ldp, W, ldw, k, d_pmap, d_wmap, dpart, d_count,
                                prof.start_event(), prof.stop_event());
         else if (rows)
@@ -1734,6 +2015,23 @@ void to_float(const double *src, float *dst, size_t len)
     if (!len) return;
     hipLaunchKernelGGL(k_to_float, dim3(ceil_div((long)len, kBlock)), dim3(kBlock), 0,
                        state().stream, len, src, dst);
+    HIP_CHECK(hipGetLastError());
+}
+
+void add_float(int n, const float *D, long ldd, double *X, long ldx, int k)
+{
+    if (k <= 0 || n <= 0) return;
+    ProfScope prof("convert");
+    hipLaunchKernelGGL(k_add_float, dim3(ceil_div(n, kBlock), k), dim3(kBlock), 0, state().stream,
+                       n, D, ldd, X, ldx);
+    HIP_CHECK(hipGetLastError());
+}
+
+void zero_float(int n, float *X, long ldx, int k)
+{
+    if (k <= 0 || n <= 0) return;
+    hipLaunchKernelGGL(k_zero_float, dim3(ceil_div(n, kBlock), k), dim3(kBlock), 0, state().stream,
+                       n, X, ldx);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -2060,8 +2358,8 @@ void coldots_pw(int n, const float *P, long ldp, const int *d_pmap, const float 
     HIP_CHECK(hipGetLastError());
 }
 
-template <typename PT, typename RT>
-static void bcg_xr_t(int n, double *X, long ldx, const int *d_xmap, RT *R, long ldr,
+template <typename PT, typename RT, typename XT>
+static void bcg_xr_t(int n, XT *X, long ldx, const int *d_xmap, RT *R, long ldr,
                      const int *d_rmap, const PT *P, long ldp, const int *d_pmap, const PT *W,
                      long ldw, const int *d_wmap, const double *d_rho2, const double *d_ptw,
                      const double *d_scale, int k, double *d_rr_out, const int *d_count,
@@ -2072,7 +2370,7 @@ static void bcg_xr_t(int n, double *X, long ldx, const int *d_xmap, RT *R, long 
     const int nbx = std::max(1, ceil_div(n, rows));
     double *partial = partial_buffer((size_t)nbx * k);
     ProfScope prof("bcg_xr");
-    hipLaunchKernelGGL((k_bcg_xr<PT, RT>), dim3(nbx, k), dim3(kBlock), 0, state().stream, n, rows, X,
+    hipLaunchKernelGGL((k_bcg_xr<PT, RT, XT>), dim3(nbx, k), dim3(kBlock), 0, state().stream, n, rows, X,
                        ldx, d_xmap, R, ldr, d_rmap, P, ldp, d_pmap, W, ldw, d_wmap, d_rho2, d_ptw,
                        d_scale, partial, d_count);
     if (rr_part) {
@@ -2096,6 +2394,16 @@ void bcg_xr_update(int n, double *X, long ldx, const int *d_xmap, double *R, lon
 }
 
 void bcg_xr_update(int n, double *X, long ldx, const int *d_xmap, float *R, long ldr,
+                   const int *d_rmap, const float *P, long ldp, const int *d_pmap, const float *W,
+                   long ldw, const int *d_wmap, const double *d_rho2, const double *d_ptw,
+                   const double *d_scale, int k, double *d_rr_out, const int *d_count,
+                   const double **rr_part, int *rr_nbx)
+{
+    bcg_xr_t(n, X, ldx, d_xmap, R, ldr, d_rmap, P, ldp, d_pmap, W, ldw, d_wmap, d_rho2, d_ptw,
+             d_scale, k, d_rr_out, d_count, rr_part, rr_nbx);
+}
+
+void bcg_xr_update(int n, float *X, long ldx, const int *d_xmap, float *R, long ldr,
                    const int *d_rmap, const float *P, long ldp, const int *d_pmap, const float *W,
                    long ldw, const int *d_wmap, const double *d_rho2, const double *d_ptw,
                    const double *d_scale, int k, double *d_rr_out, const int *d_count,

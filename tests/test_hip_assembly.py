@@ -170,6 +170,27 @@ def test_plane_order_products_equal_consecutive_order(E, which):
     assert dA.storage_info()["plane_rows"] == 0
 
 
+@pytest.mark.parametrize("E,which", [(93, "stiffness"), (10, "stiffness"), (12, "mass"),
+                                     (130, "mass")])
+def test_plane_tiles_equal_consecutive_order(E, which):
+    """The LDS plane tiles add each row's products in the order of the
+    row-block kernel: equal bit for bit, column counts that are not a
+    multiple of the staged chunk and a partial last plane group included
+    (E = 130: line stride 129, three staged positions per thread)"""
+    n = (E - 1) ** 3
+    kap = problems.lognormal_kappa(E, 5) if which == "stiffness" else None
+    dA = DeviceMatrix.q1(E, kap, which)
+    ks = (1, 3, 10, 37) if E < 100 else (2, 9)
+    for k in ks:
+        X = np.random.default_rng(10 + k).standard_normal((n, k))
+        dA.set_storage("diagonal-tiles")
+        y_tiles = _spmm(dA, X)
+        dA.set_storage("diagonal-consecutive")
+        y_cons = _spmm(dA, X)
+        assert _same_bits(y_tiles, y_cons), (E, which, k)
+    dA.set_storage("diagonal")
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_plane_order_solve(precision):
     """solves through the block CG (products with the p.w epilogue, float32
@@ -179,7 +200,7 @@ def test_plane_order_solve(precision):
     E, nev = 12, 10
     flags = ["-gcge_conv_type", "RB", "-gcge_ev_tol", "1e-10", "-gcge_ev_max_it", "100"]
     out = {}
-    for order in ("diagonal-planes", "diagonal-consecutive"):
+    for order in ("diagonal-planes", "diagonal-consecutive", "diagonal-tiles"):
         K = DeviceMatrix.q1(E, problems.lognormal_kappa(E, 4), "stiffness")
         M = DeviceMatrix.q1(E, None, "mass")
         K.set_storage(order)
@@ -188,3 +209,4 @@ def test_plane_order_solve(precision):
         assert out[order]["num_conv"] == nev
     a, b = out["diagonal-planes"]["eval"], out["diagonal-consecutive"]["eval"]
     np.testing.assert_allclose(a, b, rtol=1e-10)
+    np.testing.assert_allclose(out["diagonal-tiles"]["eval"], b, rtol=1e-10)
