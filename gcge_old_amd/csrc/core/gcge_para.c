@@ -192,6 +192,8 @@ static const flag_row flag_table[] = {
     PF("-gcge_use_ops_ext", F_INT, use_ops_ext),
     PF("-gcge_full_rr", F_INT, full_rr),
     PF("-gcge_bcg_by_column", F_INT, bcg_by_column),
+    /* a GCGE_PARA field the reference sets only in code (example/csr_solver.c:51) */
+    PF("-gcge_dirichlet_boundary", F_INT, dirichlet_boundary),
 };
 
 static void print_usage(const char *prog)
@@ -211,6 +213,7 @@ static void print_usage(const char *prog)
     GCGE_Printf("  -gcge_use_ops_ext <i>       use backend block extensions (default 1)\n");
     GCGE_Printf("  -gcge_full_rr <i>           keep converged vectors in the Rayleigh-Ritz step (default 0)\n");
     GCGE_Printf("  -gcge_bcg_by_column <i>     block CG scalars kept per column (default 0)\n");
+    GCGE_Printf("  -gcge_dirichlet_boundary <i> zero the Dirichlet rows of the initial vectors (default 0)\n");
     GCGE_Printf("  (all other reference -gcge_* flags are accepted with the same meaning)\n\n");
 }
 

@@ -690,13 +690,13 @@ static void check_range(const MultiVec *mv, int start, int k, const char *who)
 static inline double *col_ptr(MultiVec *mv, int c) { return mv->data + (size_t)c * mv->ld; }
 
 // Backend scratch blocks (residual check, block CG without a workspace,
-// float32 directions of the mixed-precision block CG); slots 0-3 hold
-// float64 blocks, 4-7 float32 blocks
+// float32 blocks of the mixed-precision block CG); slots 0-3 hold float64
+// blocks, 4-8 float32 blocks
 struct Scratch {
     void *data = nullptr;
     size_t cap = 0;  // bytes
 };
-constexpr int kScratchSlots = 8;
+constexpr int kScratchSlots = 9;
 static Scratch g_scratch[kScratchSlots];
 
 static void *scratch_bytes(int which, size_t need)
@@ -989,7 +989,9 @@ static CGParams g_cg;
 // residual block R hold float64 (the caller's blocks or backend scratch) or
 // float32 (backend scratch; the products A p then use the float32 values;
 // the float64 initial residual is rounded to float32 once and the final one
-// written back); x, the scalars and every dot product stay float64.
+// written back); x is the float64 solution, or in the float32 form a float32
+// correction of the initial guess; the scalars and every dot product stay
+// float64.
 //
 // The float32 directions are stored scaled, p = s p_stored, with one scale
 // per column chosen so that |p_stored| <= 1: s_new = |r| + |beta| s_old
@@ -999,9 +1001,9 @@ static CGParams g_cg;
 // (scalars stored by position in the list of unconverged columns, beta read
 // by column, src/gcge_cg.c:405,467-476) gives a large beta, which float64
 // absorbs and unscaled float32 directions would overflow.
-template <typename PT>
+template <typename PT, typename XT>
 static void bcg_iterate(Mat *A, Mat *B, int if_shift, double shift, int n, int len, PT *Rd,
-                        long ldr, int r0, double *Xd, long ldx, int x0, PT *Pd, long ldp, int p0,
+                        long ldr, int r0, XT *Xd, long ldx, int x0, PT *Pd, long ldp, int p0,
                         PT *Wd, long ldw, int w0, PT *T, long ldt, int max_it, double rate,
                         double *rho1, double *rho2, const double *err, double *last_err,
                         int *unlock, int num_unlock, int by_column)
@@ -1295,7 +1297,7 @@ static void block_cg(void *Amat, int if_shift, double shift, void *Bmat, void **
             long lpf, lwf, lrf, ldf;
             float *Pf = scratch_f32(0, n, len, &lpf);
             float *Wf = scratch_f32(1, n, len, &lwf);
-            float *Df = scratch_f32(2, n, len, &ldf);
+            float *Df = scratch_f32(4, n, len, &ldf);
             float *Rf = scratch_f32(3, n, len, &lrf);
             block_to_float(Rd + (long)start[0] * ldr, ldr, Rf, lrf, n, len);
             zero_float(n, Df, ldf, len);
@@ -1324,20 +1326,25 @@ static void block_cg(void *Amat, int if_shift, double shift, void *Bmat, void **
         if (err[i] >= 1e-15) unlock[num_unlock++] = i;
     }
     if (f32) {
-        long lpf, lwf, ltf = 0, lrf;
+        // the same float32 form as the device-driven loop: x = x0 + D with a
+        // float32 correction D formed once at the end
+        long lpf, lwf, ltf = 0, lrf, ldf;
         float *Pf = scratch_f32(0, n, len, &lpf);
         float *Wf = scratch_f32(1, n, len, &lwf);
         float *Tf = (if_shift == 1 && B) ? scratch_f32(2, n, len, &ltf) : nullptr;
         float *Rf = scratch_f32(3, n, len, &lrf);
+        float *Df = scratch_f32(4, n, len, &ldf);
         block_to_float(Rd + (long)start[0] * ldr, ldr, Rf, lrf, n, len);
-        bcg_iterate<float>(A, B, if_shift, shift, n, len, Rf, lrf, 0, Xd, ldx, start[1], Pf, lpf,
-                           0, Wf, lwf, 0, Tf, ltf, max_it, rate, rho1, rho2, err, last_err, unlock,
-                           num_unlock, by_column);
+        zero_float(n, Df, ldf, len);
+        bcg_iterate<float, float>(A, B, if_shift, shift, n, len, Rf, lrf, 0, Df, ldf, 0, Pf, lpf,
+                                  0, Wf, lwf, 0, Tf, ltf, max_it, rate, rho1, rho2, err, last_err,
+                                  unlock, num_unlock, by_column);
+        add_float(n, Df, ldf, Xd + (long)start[1] * ldx, ldx, len);
         block_to_double(Rf, lrf, Rd + (long)start[0] * ldr, ldr, n, len);
     } else {
-        bcg_iterate<double>(A, B, if_shift, shift, n, len, Rd, ldr, start[0], Xd, ldx, start[1], Pd,
-                            ldp, p0, Wd, ldw, w0, T, ldt, max_it, rate, rho1, rho2, err, last_err,
-                            unlock, num_unlock, by_column);
+        bcg_iterate<double, double>(A, B, if_shift, shift, n, len, Rd, ldr, start[0], Xd, ldx,
+                                    start[1], Pd, ldp, p0, Wd, ldw, w0, T, ldt, max_it, rate, rho1,
+                                    rho2, err, last_err, unlock, num_unlock, by_column);
     }
 }
 

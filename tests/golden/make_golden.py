@@ -11,7 +11,12 @@ that holds the reference tree; the GPU box only reads the results).
    fixed initial block: final eigenvalues, residuals, iteration counts and
    the per-iteration convergence counts.  These pin the GCG core's control
    flow (soft locking, unlock lists, block CG, P construction) without the
-   reference tree at test time.
+   reference tree at test time.  Runs named *_dirichlet start from the
+   block with the Dirichlet rows zeroed as CSR_SetDirichletBoundary
+   (app/csr/gcge_app_csr.c:130-183) zeroes them: the reference applies it
+   only to randomly drawn blocks (src/gcge_eigsol.c:134-185), so the zeroed
+   block is what a solve with dirichlet_boundary = 1 and this random block
+   orthonormalises.
 
     python tests/golden/make_golden.py [/root/reference]
 """
@@ -38,7 +43,21 @@ RUNS = [
     ("A5M5_nev9", ("A_5", "M_5"), 9, [], 7),
     ("fd2d_24_nev12", "fd24", 12, ["-gcge_ev_max_it", "60"], 9),
     ("q1_e10_nev15", "q1e10", 15, ["-gcge_ev_max_it", "60", "-gcge_ev_tol", "1e-10"], 3),
+    ("A5M5_nev20_dirichlet", ("A_5", "M_5"), 20, ["-gcge_block_size", "10", "-gcge_ev_max_it", "40"], 7),
+    ("A5M5_nev30_dirichlet", ("A_5", "M_5"), 30, ["-gcge_block_size", "10", "-gcge_ev_max_it", "60"], 8),
 ]
+
+
+def dirichlet_rows(A, B):
+    """Rows whose B row has no entry above 1e-20 and whose A row exactly one
+    (the rule of app/csr/gcge_app_csr.c:130-183)."""
+    rows = []
+    for i in range(A.nrows):
+        b = B.vals[B.rowptr[i]:B.rowptr[i + 1]]
+        a = A.vals[A.rowptr[i]:A.rowptr[i + 1]]
+        if not np.any(np.abs(b) > 1e-20) and np.count_nonzero(np.abs(a) > 1e-20) == 1:
+            rows.append(i)
+    return np.array(rows, dtype=np.int64)
 
 
 # (name, matrices, k, max_it, rate, if_shift, shift, zero column or -1, seed)
@@ -108,6 +127,8 @@ def main():
     for name, spec, nev, flags, seed in RUNS:
         A, B = build(spec)
         x0 = oracle.random_fill(seed, 0, 0, nev, A.nrows)
+        if name.endswith("_dirichlet"):
+            x0[dirichlet_rows(A, B), :] = 0.0
         r = oracle.ref_solve(A, B, nev, flags, x0=x0, print_eval=True)
         out[name + "/eval"] = r["eval"]
         out[name + "/res"] = r["res"]

@@ -198,3 +198,37 @@ def test_c_program_through_solver_init():
     np.testing.assert_allclose(ev, problems.q1_eigs(E, nev), rtol=1e-10)
     last = [line for line in p.stdout.splitlines() if line.startswith("num_iter")][-1].split()
     assert int(last[3]) >= nev
+
+
+@pytest.mark.parametrize("nev,flags,seed",
+                         [(20, ["-gcge_block_size", "10", "-gcge_ev_max_it", "40"], 7),
+                          (30, ["-gcge_block_size", "10", "-gcge_ev_max_it", "60"], 8)])
+def test_dirichlet_boundary_on_device(nev, flags, seed, golden_dir):
+    """The reference's shipped pencil A_5/M_5 with dirichlet_boundary = 1 (the
+    setting of its canonical example, example/csr_solver.c:51): the randomly
+    drawn initial block gets its Dirichlet rows zeroed by the device
+    SetDirichletBoundary (app/csr/gcge_app_csr.c:130-183) before the
+    orthonormalisation.  Against the oracle run from the same random block
+    (same counter-based generator, its own CPU zeroing) and against the
+    reference binary started from that block with the rows zeroed
+    (ref_runs.npz *_dirichlet): eigenvalues, iteration and per-iteration
+    convergence counts; the Dirichlet rows of the eigenvectors stay zero."""
+    A, B = _fixture_matrices(golden_dir, "A_5"), _fixture_matrices(golden_dir, "M_5")
+    fl = flags + ["-gcge_dirichlet_boundary", "1"]
+    dev = gcge.solve(A, B, nev, fl, seed=seed, history_cap=200)
+    cpu = oracle.solve(A, B, nev, fl, seed=seed, history_cap=200)
+    ref = np.load(os.path.join(golden_dir, "ref_runs.npz"))
+    key = f"A5M5_nev{nev}_dirichlet"
+    assert dev["num_iter"] == cpu["num_iter"] == int(ref[key + "/iters"][0])
+    assert dev["num_conv"] == cpu["num_conv"] == int(ref[key + "/iters"][1])
+    np.testing.assert_array_equal(dev["history"][:, :2], ref[key + "/conv_history"])
+    scale = np.abs(ref[key + "/eval"]).max()
+    np.testing.assert_allclose(dev["eval"], ref[key + "/eval"], rtol=1e-9, atol=1e-9 * scale)
+    np.testing.assert_allclose(dev["eval"], cpu["eval"], rtol=1e-10, atol=1e-10 * scale)
+    # the zeroed rows: B has no entry there, A exactly one (an eigenvector of
+    # the pencil with lambda = A_ii / 0 is excluded, so they stay zero)
+    Bs, As = B.to_scipy(), A.to_scipy()
+    rows = [i for i in range(A.nrows) if not np.any(np.abs(Bs[i].data) > 1e-20)
+            and np.count_nonzero(np.abs(As[i].data) > 1e-20) == 1]
+    assert rows, "the fixture has Dirichlet rows"
+    assert np.abs(dev["evec"][rows]).max() < 1e-12

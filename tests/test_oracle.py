@@ -244,6 +244,32 @@ def test_core_on_oracle_matches_reference_runs(case, golden_dir):
         np.testing.assert_allclose(out["history"][:, 2:], g[case + "/eval_history"], rtol=1e-9)
 
 
+@pytest.mark.parametrize("nev,flags,seed",
+                         [(20, ["-gcge_block_size", "10", "-gcge_ev_max_it", "40"], 7),
+                          (30, ["-gcge_block_size", "10", "-gcge_ev_max_it", "60"], 8)])
+def test_dirichlet_boundary_matches_reference_runs(nev, flags, seed, golden_dir):
+    """dirichlet_boundary = 1 on A_5/M_5 (the setting of the reference's
+    canonical example, example/csr_solver.c:51) with a randomly drawn
+    initial block: the core zeroes the Dirichlet rows through the oracle's
+    SetDirichletBoundary (app/csr/gcge_app_csr.c:130-183), then runs as the
+    reference binary did from that block with the rows zeroed by the
+    fixture script's own reading of the rule (ref_runs.npz *_dirichlet).  A
+    missing or wrong zeroing changes the first orthonormalisation and every
+    iterate after it."""
+    g = np.load(os.path.join(golden_dir, "ref_runs.npz"))
+    A, B = _fixture(golden_dir, "A_5"), _fixture(golden_dir, "M_5")
+    out = oracle.solve(A, B, nev, flags + ["-gcge_dirichlet_boundary", "1"], seed=seed,
+                       history_cap=200)
+    key = f"A5M5_nev{nev}_dirichlet"
+    assert [out["num_iter"], out["num_conv"]] == list(g[key + "/iters"])
+    np.testing.assert_array_equal(out["history"][:, :2], g[key + "/conv_history"])
+    np.testing.assert_allclose(out["eval"], g[key + "/eval"], rtol=1e-10)
+    # without the zeroing the same random block follows another trajectory
+    plain = oracle.solve(A, B, nev, flags, seed=seed, history_cap=200)
+    assert not np.allclose(plain["history"][:, 2:min(plain['history'].shape[1], 6)][:3],
+                           g[key + "/eval_history"][:3, :4], rtol=1e-6)
+
+
 def test_spurious_pairs_of_shipped_example(golden_dir):
     """The shipped example (A_5/M_5, nev = 9): block_size is clamped to 9, so
     after the first loop iteration the X block (max_dim_x = nev + 20 = 29
