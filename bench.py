@@ -231,9 +231,9 @@ def kernel_sweep(dK, dM):
 
     lib = _native.hip()
     n = dM.nrows
-    X = gcge.DeviceMultiVec(n, 200)
+    X = gcge.DeviceMultiVec(n, 300)
     Y = gcge.DeviceMultiVec(n, 200)
-    lib.GCGE_HIP_RandomFill(X.handle, 0, 200)
+    lib.GCGE_HIP_RandomFill(X.handle, 0, 300)
 
     def timed(name, fn, reps=10, extra=()):
         """us per launch and GB/s of kernel `name`; `extra` kernels of the
@@ -258,17 +258,17 @@ def kernel_sweep(dK, dM):
         for k in (1, 10, 50, 100, 200):
             us, gbs = timed("spmm", lambda: lib.GCGE_HIP_SpMM(mat.handle, X.handle, 0, Y.handle, 0, k))
             print(f"# kernel spmm {label} k={k}: {us:.1f} us/launch, {gbs:.0f} GB/s", file=sys.stderr)
-    G = np.zeros((200, 200), order="F")
+    G = np.zeros((300, 200), order="F")
     for m, k in ((70, 10), (60, 60), (200, 100), (300, 100), (1, 1)):
         us, gbs = timed("gram", lambda: lib.GCGE_HIP_Gram(X.handle, 0, m, Y.handle, 0, k,
-                                                          _native.as_dptr(G), 200),
+                                                          _native.as_dptr(G), 300),
                         extra=("gram_reduce",))
         print(f"# kernel gram m={m} k={k} (with its reduction pass): {us:.1f} us/call, {gbs:.0f} GB/s, "
               f"{2.0 * n * m * k / (us * 1e-6) / 1e12:.1f} TFLOP/s", file=sys.stderr)
-    Cm = np.asfortranarray(np.random.default_rng(0).standard_normal((200, 200)))
-    for m, k in ((90, 50), (200, 100), (70, 10)):
+    Cm = np.asfortranarray(np.random.default_rng(0).standard_normal((300, 200)))
+    for m, k in ((90, 50), (200, 100), (300, 100), (70, 10)):
         us, gbs = timed("lincomb", lambda: lib.GCGE_HIP_LinearComb(X.handle, 0, m, _native.as_dptr(Cm),
-                                                                   200, 0.0, Y.handle, 0, k))
+                                                                   300, 0.0, Y.handle, 0, k))
         print(f"# kernel lincomb m={m} k={k}: {us:.1f} us/launch, {gbs:.0f} GB/s, "
               f"{2.0 * n * m * k / (us * 1e-6) / 1e12:.1f} TFLOP/s", file=sys.stderr)
     # plain copy of 64 columns (n x 64 doubles read and written): the counter

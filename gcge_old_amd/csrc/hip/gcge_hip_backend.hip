@@ -189,7 +189,7 @@ static hipEvent_t pooled_event()
 ProfScope::ProfScope(const char *name, double bytes, bool pkt) : slot(-1), packet(pkt)
 {
     State &s = state();
-    if (!s.profiling) return;
+    if (!s.profiling || !name) return;  // no name: never recorded (allocation-free callers)
     // profile mode 2: only the SpMM launches (the roofline kernel)
     if (s.profile_mode == 2 && strncmp(name, "spmm", 4) != 0) return;
     ProfRecord r{name, pooled_event(), pooled_event(), bytes};
@@ -713,7 +713,7 @@ static void *scratch_bytes(int which, size_t need)
     return s.data;
 }
 
-static double *scratch(int which, int n, int k, long *ld)
+static double *scratch_block(int which, int n, int k, long *ld)
 {
     const long l = padded_ld(n);
     *ld = l;
@@ -1233,7 +1233,7 @@ static void block_cg(void *Amat, int if_shift, double shift, void *Bmat, void **
         Pd = P->data;
         ldp = P->ld;
     } else if (!f32) {
-        Pd = scratch(0, n, len, &ldp);
+        Pd = scratch_block(0, n, len, &ldp);
         p0 = 0;
     }
     if (CG_W) {
@@ -1242,11 +1242,11 @@ static void block_cg(void *Amat, int if_shift, double shift, void *Bmat, void **
         Wd = W->data;
         ldw = W->ld;
     } else {
-        Wd = scratch(1, n, len, &ldw);
+        Wd = scratch_block(1, n, len, &ldw);
         w0 = 0;
     }
     long ldt = 0;
-    double *T = (if_shift == 1 && B) ? scratch(2, n, len, &ldt) : nullptr;
+    double *T = (if_shift == 1 && B) ? scratch_block(2, n, len, &ldt) : nullptr;
     std::vector<double> host_scalars;
     double *rho1, *rho2, *err, *last_err;
     if (dtmp) {
@@ -1378,13 +1378,13 @@ static void ext_residual_norms(void *Amat, void *Bmat, void **Xh, GCGE_INT start
     if (Bmat) work_spmv(as_mat(Bmat), k);
     work_vec(3.0 * k, k, n);
     long ldax, ldbx;
-    double *AX = scratch(0, n, k, &ldax);
+    double *AX = scratch_block(0, n, k, &ldax);
     const double *Xc = col_ptr(X, start);
     spmm(A, Xc, X->ld, AX, ldax, k);
     const double *BX = Xc;
     ldbx = X->ld;
     if (Bmat) {
-        double *t = scratch(1, n, k, &ldbx);
+        double *t = scratch_block(1, n, k, &ldbx);
         spmm(as_mat(Bmat), Xc, X->ld, t, ldbx, k);
         BX = t;
     }

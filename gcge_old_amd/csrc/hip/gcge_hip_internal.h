@@ -183,6 +183,13 @@ void spmm_dot(const Mat *A, const double *P, long ldp, double *W, long ldw, int 
 void spmm_dot(const Mat *A, const float *P, long ldp, float *W, long ldw, int k, const int *d_pmap,
               const int *d_wmap, double *d_out, const int *d_count = nullptr,
               int *prof_slot = nullptr);
+// device d_out (m x k, leading dimension m) = V^T W; the partial sums of the
+// row ranges go to `partial` (caller memory of partial_cap doubles: fewer
+// ranges when needed) or, when partial is null, to the backend's grown
+// buffer.  profile = false records no profiling events.  Returns the number
+// of row ranges, or -1 when partial_cap cannot hold one m x k block.
+int gram_impl(int n, const double *V, long ldv, int m, const double *W, long ldw, int k,
+              double *d_out, double *partial, size_t partial_cap, bool profile);
 // host G(m x k, ldg) = V^T W
 void gram(int n, const double *V, long ldv, int m, const double *W, long ldw, int k, double *G,
           int ldg, bool sync_result = true);

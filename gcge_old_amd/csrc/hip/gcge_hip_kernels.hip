@@ -1448,6 +1448,78 @@ __global__ __launch_bounds__(kBlock) void k_lincomb_mfma(int n, const double *__
             }
 }
 
+// The same product for wide output blocks (up to 16 TJ columns per
+// wavefront, TJ = 7: 112 columns, so V is read once for k <= 112), with the
+// operands of the next U steps of 4 input columns loaded before the current
+// U steps' matrix-core products, so the loads' latency overlaps the
+// products.
+template <int TJ, int RT, int U>
+__global__ __launch_bounds__(kBlock) void k_lincomb_mfma_p(int n, const double *__restrict__ V, long ldv,
+                                                          int m, const double *__restrict__ Ct,
+                                                          int ldct, double beta,
+                                                          double *__restrict__ Y, long ldy, int k)
+{
+    const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+    const int r0 = (blockIdx.x * (kBlock / kWave) + wid) * 16 * RT;
+    if (r0 >= n) return;
+    const int j0 = blockIdx.y * 16 * TJ;
+    const int kk = lane / 16, li = lane % 16;
+    gcge_double4 acc[TJ][RT];
+#pragma unroll
+    for (int t = 0; t < TJ; ++t)
+#pragma unroll
+        for (int q = 0; q < RT; ++q) acc[t][q] = gcge_double4{0.0, 0.0, 0.0, 0.0};
+    double a[U][TJ], b[U][RT];
+    auto load = [&](int m0) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int mm = m0 + 4 * u + kk;
+            const bool mok = mm < m;
+#pragma unroll
+            for (int t = 0; t < TJ; ++t) {
+                const int j = j0 + t * 16 + li;
+                a[u][t] = (mok && j < k) ? Ct[(long)mm * ldct + j] : 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < RT; ++q) {
+                const int r = r0 + q * 16 + li;
+                b[u][q] = (mok && r < n) ? V[(long)mm * ldv + r] : 0.0;
+            }
+        }
+    };
+    load(0);
+    for (int m0 = 0; m0 < m; m0 += 4 * U) {
+        double ca[U][TJ], cb[U][RT];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+#pragma unroll
+            for (int t = 0; t < TJ; ++t) ca[u][t] = a[u][t];
+#pragma unroll
+            for (int q = 0; q < RT; ++q) cb[u][q] = b[u][q];
+        }
+        if (m0 + 4 * U < m) load(m0 + 4 * U);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int t = 0; t < TJ; ++t)
+#pragma unroll
+                for (int q = 0; q < RT; ++q)
+                    acc[t][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(ca[u][t], cb[u][q], acc[t][q], 0, 0, 0);
+    }
+#pragma unroll
+    for (int t = 0; t < TJ; ++t)
+#pragma unroll
+        for (int q = 0; q < RT; ++q)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int j = j0 + t * 16 + kk + 4 * v, r = r0 + q * 16 + li;
+                if (j < k && r < n) {
+                    double *y = Y + (long)j * ldy + r;
+                    *y = (beta == 0.0) ? acc[t][q][v] : beta * (*y) + acc[t][q][v];
+                }
+            }
+}
+
 // ---- streaming kernels ------------------------------------------------------------------
 // y = a x + b y with the reference special cases (a == 0: x not read;
 // b == 0: y not read)
@@ -2043,26 +2115,35 @@ void to_double(const float *src, double *dst, size_t len)
     HIP_CHECK(hipGetLastError());
 }
 
-void gram(int n, const double *V, long ldv, int m, const double *W, long ldw, int k, double *G,
-          int ldg, bool sync_result)
+// The row ranges of a Gram launch: as many as `want`, or fewer when the
+// caller's partial buffer (cap doubles, 0: the backend's grown buffer)
+// cannot hold that many m x k blocks; rows per range rounded up to `gran`.
+static int gram_ranges(int n, int want, long mk, size_t cap, int gran, int *rows)
 {
-    if (m <= 0 || k <= 0) return;
-    if (n <= 0) {
-        for (int j = 0; j < k; ++j)
-            for (int i = 0; i < m; ++i) G[(long)j * ldg + i] = 0.0;
-        return;
-    }
+    int nbx = std::max(1, want);
+    if (cap && (size_t)nbx * mk > cap) nbx = (int)std::max<size_t>(1, cap / mk);
+    int r = ceil_div(n, nbx);
+    r = (r + gran - 1) / gran * gran;
+    *rows = r;
+    return std::max(1, ceil_div(n, r));
+}
+
+int gram_impl(int n, const double *V, long ldv, int m, const double *W, long ldw, int k,
+              double *d_out, double *partial, size_t partial_cap, bool profile)
+{
     State &s = state();
     const long mk = (long)m * k;
+    if (partial && (size_t)mk > partial_cap) return -1;
+    const char *pname = profile ? "gram" : nullptr;
+    const char *rname = profile ? "gram_reduce" : nullptr;
+    const double gbytes = 8.0 * (double)n * (m + k);
+    int rows = 0, nbx = 0;
     if (m <= 8 && k <= 8) {
         // register-blocked path for the small blocks of the orthonormalisation
-        int nbx = std::max(1, std::min(ceil_div(n, 4 * kBlock), 512));
-        const int rows = ceil_div(n, nbx);
-        nbx = std::max(1, ceil_div(n, rows));
-        double *partial = partial_buffer((size_t)nbx * mk);
-        double *d_out = small_buffer2((size_t)mk);
+        nbx = gram_ranges(n, std::min(ceil_div(n, 4 * kBlock), 512), mk, partial_cap, 1, &rows);
+        if (!partial) partial = partial_buffer((size_t)nbx * mk);
         {
-            ProfScope prof("gram", 8.0 * (double)n * (m + k));
+            ProfScope prof(pname, gbytes);
             const int MR = m <= 1 ? 1 : m <= 2 ? 2 : m <= 4 ? 4 : 8;
             const int KR = k <= 1 ? 1 : k <= 2 ? 2 : k <= 4 ? 4 : 8;
 #define GCGE_GRAM_SMALL(A, B)                                                                        \
@@ -2076,17 +2157,9 @@ void gram(int n, const double *V, long ldv, int m, const double *W, long ldw, in
 #undef GCGE_GRAM_SMALL
             HIP_CHECK(hipGetLastError());
         }
-        {
-            ProfScope prof("gram_reduce");
-            reduce_partials(partial, mk, nbx, mk, d_out);
-        }
-        double *h = result_buffer((size_t)mk);
-        HIP_CHECK(hipMemcpyAsync(h, d_out, mk * sizeof(double), hipMemcpyDeviceToHost, s.stream));
-        if (!sync_result) return;
-        HIP_CHECK(hipStreamSynchronize(s.stream));
-        for (int j = 0; j < k; ++j)
-            for (int i = 0; i < m; ++i) G[(long)j * ldg + i] = h[(long)j * m + i];
-        return;
+        ProfScope prof(rname);
+        reduce_partials(partial, mk, nbx, mk, d_out);
+        return nbx;
     }
     static const int gram_mode = [] {
         const char *e = getenv("GCGE_HIP_GRAM");  // "fma": the vector-FMA kernels (measurement)
@@ -2107,43 +2180,30 @@ void gram(int n, const double *V, long ldv, int m, const double *W, long ldw, in
         // partials of at most a tenth of the input bytes (the reduction pass
         // reads them all)
         const long cap_parts = std::max(1L, (long)n * (m + k) / (10L * mk));
-        int nbx = std::max(1, std::min(ceil_div(n, 2 * kGmRows), std::max(1, 1024 / (gy * gz))));
-        nbx = (int)std::min<long>(nbx, cap_parts);
-        int rows = ceil_div(n, nbx);
-        rows = (rows + kGmRows - 1) / kGmRows * kGmRows;
-        nbx = std::max(1, ceil_div(n, rows));
-        double *partial = partial_buffer((size_t)nbx * mk);
-        double *d_out = small_buffer2((size_t)mk);
+        int want = std::max(1, std::min(ceil_div(n, 2 * kGmRows), std::max(1, 1024 / (gy * gz))));
+        want = (int)std::min<long>(want, cap_parts);
+        nbx = gram_ranges(n, want, mk, partial_cap, kGmRows, &rows);
+        if (!partial) partial = partial_buffer((size_t)nbx * mk);
         {
-            ProfScope prof("gram", 8.0 * (double)n * (m + k));
+            ProfScope prof(pname, gbytes);
             dim3 grid(nbx, gy, gz);
             launch_gram_mfma(nw, pw, ept, grid, lds, n, rows, V, ldv, m, W, ldw, k, partial);
             HIP_CHECK(hipGetLastError());
         }
-        {
-            ProfScope prof("gram_reduce");
-            reduce_partials(partial, mk, nbx, mk, d_out);
-        }
-        double *h = result_buffer((size_t)mk);
-        HIP_CHECK(hipMemcpyAsync(h, d_out, mk * sizeof(double), hipMemcpyDeviceToHost, s.stream));
-        if (!sync_result) return;
-        HIP_CHECK(hipStreamSynchronize(s.stream));
-        for (int j = 0; j < k; ++j)
-            for (int i = 0; i < m; ++i) G[(long)j * ldg + i] = h[(long)j * m + i];
-        return;
+        ProfScope prof(rname);
+        reduce_partials(partial, mk, nbx, mk, d_out);
+        return nbx;
     }
     if (k <= 16) {
         // column-block register path: 8 x 8 or 4 x 16 accumulators per lane
         const int MR = k <= 8 ? 8 : 4;
         const int colblocks = ceil_div(m, MR);
         const int gy = ceil_div(colblocks, kBlock / kWave);
-        int nbx = std::max(1, std::min(ceil_div(n, 4 * kWave), std::max(1, 1024 / gy)));
-        const int rows = ceil_div(n, nbx);
-        nbx = std::max(1, ceil_div(n, rows));
-        double *partial = partial_buffer((size_t)nbx * mk);
-        double *d_out = small_buffer2((size_t)mk);
+        nbx = gram_ranges(n, std::min(ceil_div(n, 4 * kWave), std::max(1, 1024 / gy)), mk, partial_cap, 1,
+                          &rows);
+        if (!partial) partial = partial_buffer((size_t)nbx * mk);
         {
-            ProfScope prof("gram", 8.0 * (double)n * (m + k));
+            ProfScope prof(pname, gbytes);
             if (k <= 8)
                 hipLaunchKernelGGL((k_gram_cols<8, 8>), dim3(nbx, gy), dim3(kBlock), 0, s.stream, n, rows, V,
                                    ldv, m, W, ldw, k, partial);
@@ -2152,31 +2212,20 @@ void gram(int n, const double *V, long ldv, int m, const double *W, long ldw, in
                                    ldv, m, W, ldw, k, partial);
             HIP_CHECK(hipGetLastError());
         }
-        {
-            ProfScope prof("gram_reduce");
-            reduce_partials(partial, mk, nbx, mk, d_out);
-        }
-        double *h = result_buffer((size_t)mk);
-        HIP_CHECK(hipMemcpyAsync(h, d_out, mk * sizeof(double), hipMemcpyDeviceToHost, s.stream));
-        if (!sync_result) return;
-        HIP_CHECK(hipStreamSynchronize(s.stream));
-        for (int j = 0; j < k; ++j)
-            for (int i = 0; i < m; ++i) G[(long)j * ldg + i] = h[(long)j * m + i];
-        return;
+        ProfScope prof(rname);
+        reduce_partials(partial, mk, nbx, mk, d_out);
+        return nbx;
     }
     const bool wide_m = m > 16, wide_k = k > 16;
     const int TM = wide_m ? 64 : 16, TN = wide_k ? 64 : 16;
     const int tm = ceil_div(m, TM), tn = ceil_div(k, TN);
     const int tiles = tm * tn;
     // about two workgroups per CU over all tiles, at least 256 rows each
-    int nbx = std::max(1, std::min(ceil_div(n, kBlock), std::max(1, 512 / tiles)));
-    int rows = ceil_div(n, nbx);
-    rows = (rows + kGramRows - 1) / kGramRows * kGramRows;
-    nbx = std::max(1, ceil_div(n, rows));
-    double *partial = partial_buffer((size_t)nbx * mk);
-    double *d_out = small_buffer2((size_t)mk);
+    nbx = gram_ranges(n, std::min(ceil_div(n, kBlock), std::max(1, 512 / tiles)), mk, partial_cap,
+                      kGramRows, &rows);
+    if (!partial) partial = partial_buffer((size_t)nbx * mk);
     {
-        ProfScope prof("gram", 8.0 * (double)n * (m + k));
+        ProfScope prof(pname, gbytes);
         dim3 grid(nbx, tm, tn);
         if (wide_m && wide_k)
             hipLaunchKernelGGL((k_gram<64, 64>), grid, dim3(kBlock), 0, s.stream, n, rows, V, ldv, m, W, ldw, k, partial);
@@ -2188,10 +2237,24 @@ void gram(int n, const double *V, long ldv, int m, const double *W, long ldw, in
             hipLaunchKernelGGL((k_gram<16, 16>), grid, dim3(kBlock), 0, s.stream, n, rows, V, ldv, m, W, ldw, k, partial);
         HIP_CHECK(hipGetLastError());
     }
-    {
-        ProfScope prof("gram_reduce");
-        reduce_partials(partial, mk, nbx, mk, d_out);
+    ProfScope prof(rname);
+    reduce_partials(partial, mk, nbx, mk, d_out);
+    return nbx;
+}
+
+void gram(int n, const double *V, long ldv, int m, const double *W, long ldw, int k, double *G,
+          int ldg, bool sync_result)
+{
+    if (m <= 0 || k <= 0) return;
+    if (n <= 0) {
+        for (int j = 0; j < k; ++j)
+            for (int i = 0; i < m; ++i) G[(long)j * ldg + i] = 0.0;
+        return;
     }
+    State &s = state();
+    const long mk = (long)m * k;
+    double *d_out = small_buffer2((size_t)mk);
+    gram_impl(n, V, ldv, m, W, ldw, k, d_out, nullptr, 0, true);
     double *h = result_buffer((size_t)mk);
     HIP_CHECK(hipMemcpyAsync(h, d_out, mk * sizeof(double), hipMemcpyDeviceToHost, s.stream));
     if (!sync_result) return;
@@ -2222,6 +2285,26 @@ void lincomb_dev(int n, const double *V, long ldv, int m, const double *d_Ct, in
     // m = 70, k = 10: 27.7 us against 34.1 (vector kernel); m = 90, k = 50:
     // 113.7 against 136.3; m = 80, k = 30: 57.5 against 47.9, so the vector
     // kernel keeps 17..32
+    static const int wide = [] {
+        const char *e = getenv("GCGE_HIP_LINCOMB_WIDE");  // variant of the wide kernel (measurement)
+        return e ? atoi(e) : 1;
+    }();
+    if (mode == 0 && k > 64 && wide > 0) {
+        // 112 output columns per wavefront (V read once up to k = 112)
+        const int wpb = kBlock / kWave;
+        const dim3 g(ceil_div(n, 32 * wpb), ceil_div(k, 112));
+        if (wide == 2)
+            hipLaunchKernelGGL((k_lincomb_mfma_p<7, 2, 1>), g, block, 0, st, n, V, ldv, m, d_Ct, ldct, beta,
+                               Y, ldy, k);
+        else if (wide == 3)
+            hipLaunchKernelGGL((k_lincomb_mfma_p<7, 2, 4>), g, block, 0, st, n, V, ldv, m, d_Ct, ldct, beta,
+                               Y, ldy, k);
+        else
+            hipLaunchKernelGGL((k_lincomb_mfma_p<7, 2, 2>), g, block, 0, st, n, V, ldv, m, d_Ct, ldct, beta,
+                               Y, ldy, k);
+        HIP_CHECK(hipGetLastError());
+        return;
+    }
     if (mode == 0 && k > 8 && (k <= 16 || k > 32)) {
         const int wpb = kBlock / kWave;
         if (k <= 16) {
