@@ -268,6 +268,34 @@ static int block_residuals(void *A, void *B, GCGE_DOUBLE *eval, void **evec, GCG
     return 1;
 }
 
+/* ||A x - lambda B x||, ||x||, ||B x|| of column idx by single-vector calls
+ * (the reference's loop, src/gcge_eigsol.c:683-721); tmp1, tmp2 scratch */
+static void column_residual(void *A, void *B, GCGE_DOUBLE lambda, void **evec, GCGE_INT idx,
+                            void *tmp1, void *tmp2, GCGE_OPS *ops, GCGE_DOUBLE *res_norm,
+                            GCGE_DOUBLE *evec_norm, GCGE_DOUBLE *bx_norm)
+{
+    void *ev;
+    ops->GetVecFromMultiVec(evec, idx, &ev, ops);
+    ops->MatDotVec(A, ev, tmp1, ops);
+    if (B == NULL) {
+        ops->VecAxpby(-lambda, ev, 1.0, tmp1, ops);
+    } else {
+        ops->MatDotVec(B, ev, tmp2, ops);
+        ops->VecAxpby(-lambda, tmp2, 1.0, tmp1, ops);
+    }
+    ops->VecInnerProd(tmp1, tmp1, res_norm, ops);
+    *res_norm = sqrt(*res_norm);
+    ops->VecInnerProd(ev, ev, evec_norm, ops);
+    *evec_norm = sqrt(*evec_norm);
+    if (B == NULL) {
+        *bx_norm = *evec_norm;
+    } else {
+        ops->VecInnerProd(tmp2, tmp2, bx_norm, ops);
+        *bx_norm = sqrt(*bx_norm);
+    }
+    ops->RestoreVecForMultiVec(evec, idx, &ev, ops);
+}
+
 void GCGE_CheckConvergence(void *A, void *B, GCGE_DOUBLE *eval, void **evec, GCGE_OPS *ops,
                            GCGE_PARA *para, GCGE_WORKSPACE *ws)
 {
@@ -295,7 +323,7 @@ void GCGE_CheckConvergence(void *A, void *B, GCGE_DOUBLE *eval, void **evec, GCG
         have_block = block_residuals(A, B, eval, evec, ops, para, start, nev, rn, xn, bxn);
     }
 
-    void *tmp1 = NULL, *tmp2 = NULL, *ev;
+    void *tmp1 = NULL, *tmp2 = NULL;
     if (!have_block) {
         ops->GetVecFromMultiVec(ws->V_tmp, 0, &tmp1, ops);
         ops->GetVecFromMultiVec(ws->V_tmp, 1, &tmp2, ops);
@@ -308,25 +336,8 @@ void GCGE_CheckConvergence(void *A, void *B, GCGE_DOUBLE *eval, void **evec, GCG
             evec_norm = xn[idx - start];
             bx_norm = bxn[idx - start];
         } else {
-            ops->GetVecFromMultiVec(evec, idx, &ev, ops);
-            ops->MatDotVec(A, ev, tmp1, ops);
-            if (B == NULL) {
-                ops->VecAxpby(-eval[idx], ev, 1.0, tmp1, ops);
-            } else {
-                ops->MatDotVec(B, ev, tmp2, ops);
-                ops->VecAxpby(-eval[idx], tmp2, 1.0, tmp1, ops);
-            }
-            ops->VecInnerProd(tmp1, tmp1, &res_norm, ops);
-            res_norm = sqrt(res_norm);
-            ops->VecInnerProd(ev, ev, &evec_norm, ops);
-            evec_norm = sqrt(evec_norm);
-            if (B == NULL) {
-                bx_norm = evec_norm;
-            } else {
-                ops->VecInnerProd(tmp2, tmp2, &bx_norm, ops);
-                bx_norm = sqrt(bx_norm);
-            }
-            ops->RestoreVecForMultiVec(evec, idx, &ev, ops);
+            column_residual(A, B, eval[idx], evec, idx, tmp1, tmp2, ops, &res_norm, &evec_norm,
+                            &bx_norm);
         }
         GCGE_DOUBLE residual = res_norm / evec_norm;
         const GCGE_DOUBLE abs_residual = residual;
@@ -411,6 +422,45 @@ void GCGE_CheckConvergence(void *A, void *B, GCGE_DOUBLE *eval, void **evec, GCG
     }
     if (num_conv_continuous < nev && unlock[0] < num_conv_continuous) num_conv_continuous = unlock[0];
     if (num_unlock > para->block_size) num_unlock = para->block_size;
+
+    /* With -gcge_full_rr the converged prefix [0, start) stays in the
+     * Rayleigh-Ritz step, so its vectors keep moving after their check: before
+     * the solve ends, check them again; the first one above the tolerance
+     * and the columns after it go back into the unlock list. */
+    if (para->full_rr && num_conv_continuous >= nev && start > 0) {
+        GCGE_INT first_bad = -1;
+        GCGE_DOUBLE *pr = (GCGE_DOUBLE *)malloc(3 * (size_t)start * sizeof(GCGE_DOUBLE));
+        GCGE_DOUBLE *px = pr + start, *pb = px + start;
+        if (!block_residuals(A, B, eval, evec, ops, para, 0, start, pr, px, pb)) {
+            void *t1 = NULL, *t2 = NULL;
+            ops->GetVecFromMultiVec(ws->V_tmp, 0, &t1, ops);
+            ops->GetVecFromMultiVec(ws->V_tmp, 1, &t2, ops);
+            for (GCGE_INT idx = 0; idx < start; ++idx)
+                column_residual(A, B, eval[idx], evec, idx, t1, t2, ops, &pr[idx], &px[idx], &pb[idx]);
+            ops->RestoreVecForMultiVec(ws->V_tmp, 0, &t1, ops);
+            ops->RestoreVecForMultiVec(ws->V_tmp, 1, &t2, ops);
+        }
+        {
+            for (GCGE_INT idx = 0; idx < start; ++idx) {
+                GCGE_DOUBLE r = pr[idx] / px[idx];
+                if (strcmp(conv_type, "R") == 0)
+                    r /= (1.0 > fabs(eval[idx]) ? 1.0 : fabs(eval[idx]));
+                else if (strcmp(conv_type, "O") == 0)
+                    r /= (para->conv_omega_norm + fabs(eval[idx]));
+                else if (strcmp(conv_type, "RB") == 0)
+                    r = pr[idx] / (fabs(eval[idx]) * pb[idx]);
+                res[idx] = r;
+                if (first_bad < 0 && !(r < ev_tol && pr[idx] / px[idx] < 1e-2)) first_bad = idx;
+            }
+        }
+        free(pr);
+        if (first_bad >= 0) {
+            num_unlock = 0;
+            for (GCGE_INT i = first_bad; i < nev && num_unlock < para->block_size; ++i)
+                unlock[num_unlock++] = i;
+            num_conv_continuous = first_bad;
+        }
+    }
 
     para->num_unlock = num_unlock;
     para->num_conv_continuous = num_conv_continuous;

@@ -1214,11 +1214,11 @@ __global__ __launch_bounds__(64 * NW) void k_gram_mfma(int n, int rows_per_block
                                                      const double *__restrict__ V, long ldv,
                                                      int m, const double *__restrict__ W,
                                                      long ldw, int k,
-                                                     double *__restrict__ partial)
+                                                     double *__restrict__ partial, int bcv, int bcw)
 {
     extern __shared__ double lds_gm[];
-    const int i_base = blockIdx.y * kGmCols, j_base = blockIdx.z * kGmCols;
-    const int mb = min(kGmCols, m - i_base), kb = min(kGmCols, k - j_base);
+    const int i_base = blockIdx.y * bcv, j_base = blockIdx.z * bcw;
+    const int mb = min(bcv, m - i_base), kb = min(bcw, k - j_base);
     const int ti = (mb + 15) / 16, tj = (kb + 15) / 16;
     const int MP = ti * 16, KP = tj * 16, npairs = ti * tj;
     const int ld = MP + KP + 1;
@@ -1289,36 +1289,41 @@ __global__ __launch_bounds__(64 * NW) void k_gram_mfma(int n, int rows_per_block
 
 template <int P, int NW>
 void launch_gram_mfma_e(int ept, dim3 grid, size_t lds, int n, int rows, const double *V, long ldv,
-                        int m, const double *W, long ldw, int k, double *partial)
+                        int m, const double *W, long ldw, int k, double *partial, int bcv, int bcw)
 {
     hipStream_t st = state().stream;
     const dim3 block(64 * NW);
-    if (ept <= 2)
-        hipLaunchKernelGGL((k_gram_mfma<P, 2, NW>), grid, block, lds, st, n, rows, V, ldv, m, W, ldw, k, partial);
-    else if (ept <= 4)
-        hipLaunchKernelGGL((k_gram_mfma<P, 4, NW>), grid, block, lds, st, n, rows, V, ldv, m, W, ldw, k, partial);
-    else if (ept <= 8)
-        hipLaunchKernelGGL((k_gram_mfma<P, 8, NW>), grid, block, lds, st, n, rows, V, ldv, m, W, ldw, k, partial);
-    else
-        hipLaunchKernelGGL((k_gram_mfma<P, 16, NW>), grid, block, lds, st, n, rows, V, ldv, m, W, ldw, k, partial);
+#define GCGE_GM_LAUNCH(E)                                                                             \
+    hipLaunchKernelGGL((k_gram_mfma<P, E, NW>), grid, block, lds, st, n, rows, V, ldv, m, W, ldw, k, \
+                       partial, bcv, bcw)
+    if (ept <= 2) GCGE_GM_LAUNCH(2);
+    else if (ept <= 4) GCGE_GM_LAUNCH(4);
+    else if (ept <= 8) GCGE_GM_LAUNCH(8);
+    else if (P == 17 && ept <= 14) GCGE_GM_LAUNCH(14);
+    else GCGE_GM_LAUNCH(16);
+#undef GCGE_GM_LAUNCH
 }
 
-// tile pairs per wave: 1, 2 or 8 (the 4-pair instance keeps its accumulators
-// in VGPRs and drops to one or two waves per SIMD; the 8-pair one does not);
-// NW = 4 or 8 waves per workgroup
+// tile pairs per wave: 1, 2, 8 or 17 (the 4-pair instance keeps its
+// accumulators in VGPRs and drops to one or two waves per SIMD; the 8- and
+// 17-pair ones do not); NW = 4 or 8 waves per workgroup; output blocks of
+// bcv x bcw
 void launch_gram_mfma(int nw, int pw, int ept, dim3 grid, size_t lds, int n, int rows,
                       const double *V, long ldv, int m, const double *W, long ldw, int k,
-                      double *partial)
+                      double *partial, int bcv, int bcw)
 {
+#define GCGE_GM_ARGS ept, grid, lds, n, rows, V, ldv, m, W, ldw, k, partial, bcv, bcw
     if (nw == 8) {
-        if (pw <= 1) launch_gram_mfma_e<1, 8>(ept, grid, lds, n, rows, V, ldv, m, W, ldw, k, partial);
-        else if (pw <= 2) launch_gram_mfma_e<2, 8>(ept, grid, lds, n, rows, V, ldv, m, W, ldw, k, partial);
-        else launch_gram_mfma_e<8, 8>(ept, grid, lds, n, rows, V, ldv, m, W, ldw, k, partial);
+        if (pw <= 1) launch_gram_mfma_e<1, 8>(GCGE_GM_ARGS);
+        else if (pw <= 2) launch_gram_mfma_e<2, 8>(GCGE_GM_ARGS);
+        else if (pw <= 8) launch_gram_mfma_e<8, 8>(GCGE_GM_ARGS);
+        else launch_gram_mfma_e<17, 8>(GCGE_GM_ARGS);
     } else {
-        if (pw <= 1) launch_gram_mfma_e<1, 4>(ept, grid, lds, n, rows, V, ldv, m, W, ldw, k, partial);
-        else if (pw <= 2) launch_gram_mfma_e<2, 4>(ept, grid, lds, n, rows, V, ldv, m, W, ldw, k, partial);
-        else launch_gram_mfma_e<8, 4>(ept, grid, lds, n, rows, V, ldv, m, W, ldw, k, partial);
+        if (pw <= 1) launch_gram_mfma_e<1, 4>(GCGE_GM_ARGS);
+        else if (pw <= 2) launch_gram_mfma_e<2, 4>(GCGE_GM_ARGS);
+        else launch_gram_mfma_e<8, 4>(GCGE_GM_ARGS);
     }
+#undef GCGE_GM_ARGS
 }
 
 // out[e] = sum_b partial[b * stride + e], e < count.  A block owns OUTS
@@ -1433,78 +1438,6 @@ __global__ __launch_bounds__(kBlock) void k_lincomb_mfma(int n, const double *__
 #pragma unroll
             for (int q = 0; q < RT; ++q)
                 acc[t][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], b[q], acc[t][q], 0, 0, 0);
-    }
-#pragma unroll
-    for (int t = 0; t < TJ; ++t)
-#pragma unroll
-        for (int q = 0; q < RT; ++q)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int j = j0 + t * 16 + kk + 4 * v, r = r0 + q * 16 + li;
-                if (j < k && r < n) {
-                    double *y = Y + (long)j * ldy + r;
-                    *y = (beta == 0.0) ? acc[t][q][v] : beta * (*y) + acc[t][q][v];
-                }
-            }
-}
-
-// The same product for wide output blocks (up to 16 TJ columns per
-// wavefront, TJ = 7: 112 columns, so V is read once for k <= 112), with the
-// operands of the next U steps of 4 input columns loaded before the current
-// U steps' matrix-core products, so the loads' latency overlaps the
-// products.
-template <int TJ, int RT, int U>
-__global__ __launch_bounds__(kBlock) void k_lincomb_mfma_p(int n, const double *__restrict__ V, long ldv,
-                                                          int m, const double *__restrict__ Ct,
-                                                          int ldct, double beta,
-                                                          double *__restrict__ Y, long ldy, int k)
-{
-    const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
-    const int r0 = (blockIdx.x * (kBlock / kWave) + wid) * 16 * RT;
-    if (r0 >= n) return;
-    const int j0 = blockIdx.y * 16 * TJ;
-    const int kk = lane / 16, li = lane % 16;
-    gcge_double4 acc[TJ][RT];
-#pragma unroll
-    for (int t = 0; t < TJ; ++t)
-#pragma unroll
-        for (int q = 0; q < RT; ++q) acc[t][q] = gcge_double4{0.0, 0.0, 0.0, 0.0};
-    double a[U][TJ], b[U][RT];
-    auto load = [&](int m0) {
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int mm = m0 + 4 * u + kk;
-            const bool mok = mm < m;
-#pragma unroll
-            for (int t = 0; t < TJ; ++t) {
-                const int j = j0 + t * 16 + li;
-                a[u][t] = (mok && j < k) ? Ct[(long)mm * ldct + j] : 0.0;
-            }
-#pragma unroll
-            for (int q = 0; q < RT; ++q) {
-                const int r = r0 + q * 16 + li;
-                b[u][q] = (mok && r < n) ? V[(long)mm * ldv + r] : 0.0;
-            }
-        }
-    };
-    load(0);
-    for (int m0 = 0; m0 < m; m0 += 4 * U) {
-        double ca[U][TJ], cb[U][RT];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-#pragma unroll
-            for (int t = 0; t < TJ; ++t) ca[u][t] = a[u][t];
-#pragma unroll
-            for (int q = 0; q < RT; ++q) cb[u][q] = b[u][q];
-        }
-        if (m0 + 4 * U < m) load(m0 + 4 * U);
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-#pragma unroll
-            for (int t = 0; t < TJ; ++t)
-#pragma unroll
-                for (int q = 0; q < RT; ++q)
-                    acc[t][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(ca[u][t], cb[u][q], acc[t][q], 0, 0, 0);
     }
 #pragma unroll
     for (int t = 0; t < TJ; ++t)
@@ -2166,9 +2099,22 @@ int gram_impl(int n, const double *V, long ldv, int m, const double *W, long ldw
         return (e && strcmp(e, "fma") == 0) ? 1 : 0;
     }();
     if (gram_mode == 0) {
-        // matrix cores: 128 x 128 output blocks (grid y, z), tile pairs shared by four waves
-        const int gy = ceil_div(m, kGmCols), gz = ceil_div(k, kGmCols);
-        const int ti = ceil_div(std::min(m, kGmCols), 16), tj = ceil_div(std::min(k, kGmCols), 16);
+        // matrix cores: 128 x 128 output blocks (grid y, z), tile pairs shared
+        // by four or eight waves; up to 304 x 128 outputs in one block (the
+        // Rayleigh-Ritz blocks of a 100-pair solve, 300 x 100), so that W is
+        // read once instead of once per 128 columns of V
+        static const int one_block = [] {
+            const char *e = getenv("GCGE_HIP_GRAM_ONEBLOCK");  // 0: 128 x 128 blocks (measurement)
+            return e ? atoi(e) : 1;
+        }();
+        int bcv = kGmCols, bcw = kGmCols;
+        if (one_block && m > kGmCols && m <= 304 && k <= kGmCols &&
+            ceil_div(m, 16) * ceil_div(k, 16) <= 8 * 17) {
+            bcv = m;
+            bcw = k;
+        }
+        const int gy = ceil_div(m, bcv), gz = ceil_div(k, bcw);
+        const int ti = ceil_div(std::min(m, bcv), 16), tj = ceil_div(std::min(k, bcw), 16);
         // eight waves per workgroup beyond 32 output tiles (four would hold 9..16
         // accumulator tiles each and run at one wave per SIMD)
         const int nw = ti * tj > 32 ? 8 : 4;
@@ -2187,7 +2133,7 @@ int gram_impl(int n, const double *V, long ldv, int m, const double *W, long ldw
         {
             ProfScope prof(pname, gbytes);
             dim3 grid(nbx, gy, gz);
-            launch_gram_mfma(nw, pw, ept, grid, lds, n, rows, V, ldv, m, W, ldw, k, partial);
+            launch_gram_mfma(nw, pw, ept, grid, lds, n, rows, V, ldv, m, W, ldw, k, partial, bcv, bcw);
             HIP_CHECK(hipGetLastError());
         }
         ProfScope prof(rname);
@@ -2285,26 +2231,6 @@ void lincomb_dev(int n, const double *V, long ldv, int m, const double *d_Ct, in
     // m = 70, k = 10: 27.7 us against 34.1 (vector kernel); m = 90, k = 50:
     // 113.7 against 136.3; m = 80, k = 30: 57.5 against 47.9, so the vector
     // kernel keeps 17..32
-    static const int wide = [] {
-        const char *e = getenv("GCGE_HIP_LINCOMB_WIDE");  // variant of the wide kernel (measurement)
-        return e ? atoi(e) : 1;
-    }();
-    if (mode == 0 && k > 64 && wide > 0) {
-        // 112 output columns per wavefront (V read once up to k = 112)
-        const int wpb = kBlock / kWave;
-        const dim3 g(ceil_div(n, 32 * wpb), ceil_div(k, 112));
-        if (wide == 2)
-            hipLaunchKernelGGL((k_lincomb_mfma_p<7, 2, 1>), g, block, 0, st, n, V, ldv, m, d_Ct, ldct, beta,
-                               Y, ldy, k);
-        else if (wide == 3)
-            hipLaunchKernelGGL((k_lincomb_mfma_p<7, 2, 4>), g, block, 0, st, n, V, ldv, m, d_Ct, ldct, beta,
-                               Y, ldy, k);
-        else
-            hipLaunchKernelGGL((k_lincomb_mfma_p<7, 2, 2>), g, block, 0, st, n, V, ldv, m, d_Ct, ldct, beta,
-                               Y, ldy, k);
-        HIP_CHECK(hipGetLastError());
-        return;
-    }
     if (mode == 0 && k > 8 && (k <= 16 || k > 32)) {
         const int wpb = kBlock / kWave;
         if (k <= 16) {
