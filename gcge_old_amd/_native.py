@@ -95,6 +95,9 @@ def _declare(name: str, lib: C.CDLL) -> None:
         lib.GCGE_HIP_RandomFill.argtypes = [C.c_void_p, C.c_int, C.c_int]
         lib.GCGE_HIP_SetLinearSolverParams.argtypes = [C.c_int, C.c_double, C.c_int, C.c_double,
                                                        C.c_void_p]
+        lib.GCGE_HIP_GramNoAlloc.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int,
+                                             C.c_int, dptr, C.c_int]
+        lib.GCGE_HIP_GramNoAlloc.restype = C.c_int
         lib.GCGE_HIP_WorkCounts.argtypes = [dptr, C.c_int, C.c_int]
         lib.GCGE_HIP_WorkCounts.restype = C.c_int
         lib.GCGE_HIP_GetLinearSolverParams.argtypes = [iptr, dptr, iptr, dptr,

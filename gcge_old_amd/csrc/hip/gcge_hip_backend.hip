@@ -63,6 +63,29 @@ State &state()
     return s;
 }
 
+// ---- device memory of the allocation-free entry point ----------------------------------------
+// GCGE_HIP_GramNoAlloc keeps its partial sums and its result in this
+// file-scope array: fixed size, held by every device the library runs on
+// for the life of the process, shared by all calls on that device (calls
+// that overlap on one device race).  Host code never uses the array's name
+// as an address (on the host it is only a stand-in): the device address is
+// taken once per device, with that device current, by hipGetSymbolAddress
+// at initialisation.
+constexpr size_t kNoAllocDoubles = (size_t)1 << 20;  // 8 MiB per device
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+constexpr int kNoAllocMaxDevices = 64;
+static double *g_scratch_ptr[kNoAllocMaxDevices];
+
+static void noalloc_setup(int device)
+{
+    if (device < 0 || device >= kNoAllocMaxDevices) fail_msg("device index beyond the scratch table");
+    if (g_scratch_ptr[device]) return;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) fail("hipGetSymbolAddress(scratch)", e, __FILE__, __LINE__);
+    g_scratch_ptr[device] = (double *)p;
+}
+
 int ensure_init()
 {
     State &s = state();
@@ -86,6 +109,7 @@ int ensure_init()
             fprintf(stderr, "gcge_hip: GCGE_HIP_SYNC=%s not applied (device already active)\n", mode);
     }
     HIP_CHECK(hipSetDevice(s.device));
+    noalloc_setup(s.device);
     HIP_CHECK(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
     for (int i = 0; i < State::kRing; ++i) HIP_CHECK(hipEventCreateWithFlags(&s.ring_event[i], hipEventDisableTiming));
     return 0;
@@ -1867,6 +1891,42 @@ void GCGE_HIP_Gram(void **V, int vs, int m, void **W, int ws, int k, double *G, 
     GCGE_INT s[2] = {vs, ws}, e[2] = {vs + m, ws + k};
     char ns[] = "ns";
     op_mv_inner(V, W, G, ns, s, e, ldg, 0, nullptr);
+}
+
+int GCGE_HIP_GramNoAlloc(void **V, int vs, int m, void **W, int ws, int k, double *G, int ldg)
+{
+    require_init();
+    if (m <= 0 || k <= 0) return 0;
+    MultiVec *Vm = as_mv(V), *Wm = as_mv(W);
+    check_range(Vm, vs, m, "GramNoAlloc");
+    check_range(Wm, ws, k, "GramNoAlloc");
+    if (Vm->n != Wm->n) fail_msg("GramNoAlloc: blocks of different lengths");
+    if (ldg < m) fail_msg("GramNoAlloc: ldg < m");
+    const long mk = (long)m * k;
+    if ((size_t)mk > kNoAllocDoubles / 2) {
+        char buf[160];
+        snprintf(buf, sizeof(buf), "GramNoAlloc: m*k = %ld exceeds the %zu outputs its fixed device "
+                 "memory holds", mk, kNoAllocDoubles / 2);
+        g_last_error = buf;
+        return 1;
+    }
+    const int n = Vm->n;
+    if (n <= 0) {
+        for (int j = 0; j < k; ++j)
+            for (int i = 0; i < m; ++i) G[(long)j * ldg + i] = 0.0;
+        return 0;
+    }
+    State &s = state();
+    double *base = g_scratch_ptr[s.device];
+    if (!base) fail_msg("GramNoAlloc: device scratch not set up");
+    // [result m x k | partial sums of the row ranges]
+    double *d_out = base, *partial = base + mk;
+    gram_impl(n, col_ptr(Vm, vs), Vm->ld, m, col_ptr(Wm, ws), Wm->ld, k, d_out, partial,
+              kNoAllocDoubles - (size_t)mk, false);
+    HIP_CHECK(hipMemcpy2DAsync(G, (size_t)ldg * sizeof(double), d_out, (size_t)m * sizeof(double),
+                               (size_t)m * sizeof(double), k, hipMemcpyDeviceToHost, s.stream));
+    HIP_CHECK(hipStreamSynchronize(s.stream));
+    return 0;
 }
 
 void GCGE_HIP_LinearComb(void **V, int vs, int m, const double *C, int ldc, double beta, void **Y,

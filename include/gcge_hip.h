@@ -126,6 +126,18 @@ void GCGE_HIP_SpMM(void *mat, void **X, int xs, void **Y, int ys, int k);
 /* G(i, j) = V(:, vs+i)^T W(:, ws+j), host G with leading dimension ldg
  * (reference src/gcge_ops.c:226-309, "ns" form) */
 void GCGE_HIP_Gram(void **V, int vs, int m, void **W, int ws, int k, double *G, int ldg);
+/* GCGE_HIP_Gram without allocating: no counterpart in the reference; for
+ * callers that cannot allocate device or pinned memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * partial sums and the result live in a fixed 8 MiB device array that each
+ * device holds for the life of the process; m * k may be at most 524288
+ * (else returns 1 and sets GCGE_HIP_LastError, computing nothing).  Same
+ * kernels and, while the partial sums fit, the same row ranges as
+ * GCGE_HIP_Gram, so the same result bit for bit; beyond that fewer ranges
+ * (the same products summed in larger groups).  G is written through a
+ * plain device-to-host copy.  Every call on one device shares its array:
+ * calls that overlap on one device (two host threads) race.  Returns 0. */
+int GCGE_HIP_GramNoAlloc(void **V, int vs, int m, void **W, int ws, int k, double *G, int ldg);
 /* Y(:, ys+j) = beta Y(:, ys+j) + sum_i V(:, vs+i) C(i, j), host C with leading
  * dimension ldc (reference src/gcge_ops.c:190-221) */
 void GCGE_HIP_LinearComb(void **V, int vs, int m, const double *C, int ldc, double beta,
