@@ -79,16 +79,16 @@ def parse():
                         "q1-64: Q1 64^3, kappa = 1, nev 50, value = float64 solve; "
                         "q1-64-adjoint: Q1 64^3 lognormal kappa, nev 20, gradient of "
                         "sum w_k lambda_k + ||P x_1||^2 with respect to all kappa_e, "
-                        "checked by central differences on --fd elements; "
+                        "checked by central differences on --fd elements at each of --fd-steps; "
                         "q1-192-permuted: Q1 192^3 lognormal kappa, rows and columns "
                         "symmetrically permuted by a seeded random permutation, nev 200, block "
                         "400 requested, a bounded number of GCG iterations (--max-it, default "
                         "3 here) in float64 and mixed precision, then the eigenvalue adjoint on "
                         "all elements")
-    p.add_argument("--fd", type=int, default=8, help="elements checked by finite differences "
-                                                     "(q1-64-adjoint)")
-    p.add_argument("--fd-step", type=float, default=None,
-                   help="relative step of the central differences (q1-64-adjoint)")
+    p.add_argument("--fd", type=int, default=64, help="elements checked by central differences "
+                                                      "(q1-64-adjoint)")
+    p.add_argument("--fd-steps", type=float, nargs="+", default=[1e-6, 1e-4, 1e-3, 1e-2],
+                   help="relative steps of the central differences (q1-64-adjoint)")
     p.add_argument("--reorder", choices=("none", "rcm"), default="none",
                    help="q1-192-permuted: reorder the permuted pencil by reverse Cuthill-McKee "
                         "(computed from the matrix graph, a preprocessing step) before the solve")
@@ -693,43 +693,54 @@ def adjoint_workload(args):
     """Gradient of L = sum_k w_k lambda_k + ||P x_1||^2 with respect to every
     kappa_e of the lognormal Q1 64^3 problem (nev 20): one solve, the
     eigenvalue adjoint (element kernel), the eigenvector adjoint (shifted
-    block CG + element kernel); central differences on a few elements."""
+    block CG + element kernel); then central differences of both terms on
+    --fd random elements at each relative step of --fd-steps (default the
+    specified 1e-6, and 1e-4, 1e-3, 1e-2), each difference quotient from two
+    solves of the perturbed pencil (assembled on the device)."""
     import gcge_old_amd as gcge
-    from gcge_old_amd import adjoint, problems
+    from gcge_old_amd import _native, adjoint, problems
 
+    lib = _native.hip()
     E = args.mesh or 64
     nev = args.nev or 20
+    n = (E - 1) ** 3
     kappa = problems.lognormal_kappa(E, args.kappa_seed)
     rng = np.random.default_rng(args.kappa_seed + 1)
     w = rng.uniform(size=nev)
+    mask = (np.random.default_rng(args.kappa_seed + 2).uniform(size=n) < 0.5)
     # tolerance 1e-9: the lognormal pencil does not reach 1e-11 in 800
     # iterations
     flags = ["-gcge_conv_type", "RB", "-gcge_ev_tol", str(min(args.tol, 1e-9)),
              "-gcge_ev_max_it", "800", "-gcge_full_rr", "1", "-gcge_w_orth_type", "svqb"]
+    dM = gcge.DeviceMatrix.q1(E, None, "mass")
+
+    def products(dA, X):
+        dX = gcge.DeviceMultiVec.from_host(X)
+        dY = gcge.DeviceMultiVec(X.shape[0], X.shape[1])
+        lib.GCGE_HIP_SpMM(dA.handle, dX.handle, 0, dY.handle, 0, X.shape[1])
+        return dY.get()
 
     def evaluate(kv, grad):
-        K, M = problems.q1_fem_3d(E, kv)
-        dK, dM = gcge.DeviceMatrix(K), gcge.DeviceMatrix(M)
+        dK = gcge.DeviceMatrix.q1(E, kv, "stiffness")
         ts = time.perf_counter()
         out = gcge.solve(dK, dM, nev, flags, seed=1)
         if out["num_conv"] < nev:
             raise RuntimeError(f"solve converged {out['num_conv']} of {nev} pairs")
         x1 = out["evec"][:, 0]
-        mask = (np.random.default_rng(args.kappa_seed + 2).uniform(size=K.nrows) < 0.5)
         if not grad:
             # difference quotients need both terms beyond the rounding of the
             # solver's Ritz values (~1e-13 relative, from its Gram sums):
-            # eigenvalues as Rayleigh quotients of the computed vectors
-            # (error quadratic in the vector error), and x_1 refined by one
-            # Newton (Jacobi-Davidson) step, (K - lam M) d = -r with d
-            # M-orthogonal to x_1, through the same shifted block CG
-            Ks, Ms = K.to_scipy(), M.to_scipy()
+            # eigenvalues as Rayleigh quotients of the computed vectors (error
+            # quadratic in the vector error), and x_1 refined by one Newton
+            # (Jacobi-Davidson) step, (K - lam M) d = -r with d M-orthogonal
+            # to x_1, through the same shifted block CG
             X = out["evec"]
-            rq = np.einsum("ij,ij->j", X, Ks @ X) / np.einsum("ij,ij->j", X, Ms @ X)
+            KX, MX = products(dK, X), products(dM, X)
+            rq = np.einsum("ij,ij->j", X, KX) / np.einsum("ij,ij->j", X, MX)
             lam1 = float(rq[0])
-            r = Ks @ x1 - lam1 * (Ms @ x1)
+            r = KX[:, 0] - lam1 * MX[:, 0]
             x1 = x1 + adjoint.eigvec_adjoint_solve(dK, dM, x1, lam1, -r, rate=1e-12)
-            x1 = x1 / np.sqrt(float(x1 @ (Ms @ x1)))
+            x1 = x1 / np.sqrt(float(x1 @ products(dM, x1[:, None])[:, 0]))
             return (float(w @ rq), float(np.sum(x1[mask] ** 2))), out, None
         terms = (float(w @ out["eval"]), float(np.sum(x1[mask] ** 2)))
         t0 = time.perf_counter()
@@ -739,34 +750,37 @@ def adjoint_workload(args):
         t2 = time.perf_counter()
         return terms, out, (g, gv, t1 - t0, t2 - t1, t2 - ts)
 
-    # the timed span starts with the matrices resident on the device (host
-    # assembly and upload excluded) and ends with both gradients on the host
+    # the timed span starts with the matrices resident on the device (the
+    # device assembly is excluded) and ends with both gradients on the host
     _, out, (g_val, g_vec, t_eval, t_vec, total) = evaluate(kappa, True)
     print(f"# adjoint: solve {out['time']:.3f}s iters={out['num_iter']} conv={out['num_conv']}, "
           f"eigenvalue adjoint {t_eval*1e3:.1f} ms, eigenvector adjoint {t_vec:.3f}s",
           file=sys.stderr, flush=True)
-    # fourth-order central differences with the relative step 1e-2: one
-    # element moves L by ~1e-9 of its size per unit step, and x_1 is only
-    # determined to ~1e-11 in float64 (the residual K x - lam M x cannot be
-    # evaluated more accurately), so smaller steps drown in that noise at
-    # elements with small gradients; the truncation error of the
-    # fourth-order formula at this step is ~1e-8
-    errs_val, errs_vec, errs_val_max, errs_vec_max = [], [], [], []
-    for e in rng.choice(E ** 3, args.fd, replace=False):
-        d = 1e-2 * kappa[e]
-        t = {}
-        for j in (-2, -1, 1, 2):
-            kv = kappa.copy()
-            kv[e] += j * d
-            t[j] = evaluate(kv, False)[0]
-        fds = [(t[-2][i] - 8 * t[-1][i] + 8 * t[1][i] - t[2][i]) / (12 * d) for i in range(2)]
-        errs_val.append(abs(g_val[e] - fds[0]) / max(abs(fds[0]), 1e-300))
-        errs_vec.append(abs(g_vec[e] - fds[1]) / max(abs(fds[1]), 1e-300))
-        errs_val_max.append(abs(g_val[e] - fds[0]) / np.abs(g_val).max())
-        errs_vec_max.append(abs(g_vec[e] - fds[1]) / np.abs(g_vec).max())
-        print(f"# element {e}: eigenvalue term adjoint {g_val[e]:.10e} fd {fds[0]:.10e} "
-              f"rel {errs_val[-1]:.2e}; eigenvector term adjoint {g_vec[e]:.10e} fd {fds[1]:.10e} "
-              f"rel {errs_vec[-1]:.2e}", file=sys.stderr, flush=True)
+    elements = rng.choice(E ** 3, args.fd, replace=False)
+    checks = {}
+    for step in args.fd_steps:
+        ev, evv, gm_val, gm_vec = [], [], [], []
+        for e in elements:
+            d = step * kappa[e]
+            t = {}
+            for j in (-1, 1):
+                kv = kappa.copy()
+                kv[e] += j * d
+                t[j] = evaluate(kv, False)[0]
+            fds = [(t[1][i] - t[-1][i]) / (2 * d) for i in range(2)]
+            ev.append(abs(g_val[e] - fds[0]) / max(abs(fds[0]), 1e-300))
+            evv.append(abs(g_vec[e] - fds[1]) / max(abs(fds[1]), 1e-300))
+            gm_val.append(abs(g_val[e] - fds[0]) / np.abs(g_val).max())
+            gm_vec.append(abs(g_vec[e] - fds[1]) / np.abs(g_vec).max())
+        ev, evv = np.array(ev), np.array(evv)
+        checks[f"{step:g}"] = {
+            "eigenvalue_term_rel_err": {"max": float(ev.max()), "median": float(np.median(ev)),
+                                        "within_1e-5": int((ev <= 1e-5).sum())},
+            "eigenvector_term_rel_err": {"max": float(evv.max()), "median": float(np.median(evv)),
+                                         "within_1e-5": int((evv <= 1e-5).sum())},
+            "max_err_over_max_gradient": [float(max(gm_val)), float(max(gm_vec))]}
+        print(f"# central differences, relative step {step:g}: {checks[f'{step:g}']}",
+              file=sys.stderr, flush=True)
     homog = float(kappa @ g_val - w @ out["eval"]) / float(w @ out["eval"])
     line = {"metric": "s for the eigenpair gradient (solve + adjoints), all kappa_e",
             "value": round(total, 4), "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 0,
@@ -774,13 +788,10 @@ def adjoint_workload(args):
             "vs_baseline": None, "dtype": "f64", "data": "synthetic",
             "config": {"workload": f"Q1 FEM {E}^3 lognormal kappa (seed {args.kappa_seed}), nev={nev}, "
                                    "L = sum w_k lambda_k + ||P x_1||^2, dL/dkappa_e for all elements",
-                       "n": (E - 1) ** 3, "elements": E ** 3, "nev": nev},
+                       "n": n, "elements": E ** 3, "nev": nev},
             "solve_s": round(out["time"], 4), "eigenvalue_adjoint_ms": round(t_eval * 1e3, 3),
             "eigenvector_adjoint_s": round(t_vec, 4),
-            "fd_elements": args.fd,
-            "max_rel_err_vs_fd_eigenvalue_term": max(errs_val) if errs_val else None,
-            "max_rel_err_vs_fd_eigenvector_term": max(errs_vec) if errs_vec else None,
-            "max_err_vs_fd_over_max_gradient": [max(errs_val_max), max(errs_vec_max)] if errs_val else None,
+            "fd_elements": args.fd, "fd_checks_by_relative_step": checks,
             "homogeneity_rel_err": homog}
     print(json.dumps(line), flush=True)
 

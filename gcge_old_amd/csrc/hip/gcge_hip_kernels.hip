@@ -22,6 +22,7 @@
 //   axpby/swap/random/coldots/residual/bcg_*: streaming kernels.
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cstdio>
@@ -1299,15 +1300,13 @@ void launch_gram_mfma_e(int ept, dim3 grid, size_t lds, int n, int rows, const d
     if (ept <= 2) GCGE_GM_LAUNCH(2);
     else if (ept <= 4) GCGE_GM_LAUNCH(4);
     else if (ept <= 8) GCGE_GM_LAUNCH(8);
-    else if (P == 17 && ept <= 14) GCGE_GM_LAUNCH(14);
     else GCGE_GM_LAUNCH(16);
 #undef GCGE_GM_LAUNCH
 }
 
-// tile pairs per wave: 1, 2, 8 or 17 (the 4-pair instance keeps its
-// accumulators in VGPRs and drops to one or two waves per SIMD; the 8- and
-// 17-pair ones do not); NW = 4 or 8 waves per workgroup; output blocks of
-// bcv x bcw
+// tile pairs per wave: 1, 2 or 8 (the 4-pair instance keeps its
+// accumulators in VGPRs and drops to one or two waves per SIMD; the 8-pair
+// one does not); NW = 4 or 8 waves per workgroup; output blocks of bcv x bcw
 void launch_gram_mfma(int nw, int pw, int ept, dim3 grid, size_t lds, int n, int rows,
                       const double *V, long ldv, int m, const double *W, long ldw, int k,
                       double *partial, int bcv, int bcw)
@@ -1316,8 +1315,7 @@ void launch_gram_mfma(int nw, int pw, int ept, dim3 grid, size_t lds, int n, int
     if (nw == 8) {
         if (pw <= 1) launch_gram_mfma_e<1, 8>(GCGE_GM_ARGS);
         else if (pw <= 2) launch_gram_mfma_e<2, 8>(GCGE_GM_ARGS);
-        else if (pw <= 8) launch_gram_mfma_e<8, 8>(GCGE_GM_ARGS);
-        else launch_gram_mfma_e<17, 8>(GCGE_GM_ARGS);
+        else launch_gram_mfma_e<8, 8>(GCGE_GM_ARGS);
     } else {
         if (pw <= 1) launch_gram_mfma_e<1, 4>(GCGE_GM_ARGS);
         else if (pw <= 2) launch_gram_mfma_e<2, 4>(GCGE_GM_ARGS);
@@ -1642,18 +1640,33 @@ __global__ __launch_bounds__(kBlock) void k_bcg_p(int n, PT *__restrict__ P, lon
                                                  const double *__restrict__ beta,
                                                  const double *__restrict__ s_old,
                                                  const double *__restrict__ s_new, int first,
-                                                 const int *__restrict__ count)
+                                                 const int *__restrict__ count, int rows_per_block)
 {
-    const int row = blockIdx.x * kBlock + threadIdx.x;
+    // a block sweeps rows_per_block rows of one column, four rows per
+    // thread at a time (four loads of each operand in flight)
     const int c = blockIdx.y;
-    if (row >= n || (count && c >= *count)) return;
-    PT *p = P + (long)pmap[c] * ldp + row;
-    const double r = (double)R[(long)rmap[c] * ldr + row];
-    if (!s_new) {
-        *p = (PT)(first ? r : r + beta[c] * (double)(*p));
-    } else {
-        const double v = first ? r : r + (beta[c] * s_old[c]) * (double)(*p);
-        *p = (PT)(v / s_new[c]);
+    if (count && c >= *count) return;
+    PT *p = P + (long)pmap[c] * ldp;
+    const RT *rr = R + (long)rmap[c] * ldr;
+    const double b = s_new ? beta[c] * s_old[c] : beta[c];
+    const int r0 = blockIdx.x * rows_per_block, r1 = min(n, r0 + rows_per_block);
+    for (int i0 = r0 + threadIdx.x; i0 < r1; i0 += 4 * kBlock) {
+        double r[4], pv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = i0 + u * kBlock;
+            r[u] = i < r1 ? (double)rr[i] : 0.0;
+            pv[u] = (!first && i < r1) ? (double)p[i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = i0 + u * kBlock;
+            if (i >= r1) continue;
+            if (!s_new)
+                p[i] = (PT)(first ? r[u] : r[u] + b * pv[u]);
+            else
+                p[i] = (PT)((first ? r[u] : r[u] + b * pv[u]) / s_new[c]);
+        }
     }
 }
 
@@ -2048,6 +2061,27 @@ void to_double(const float *src, double *dst, size_t len)
     HIP_CHECK(hipGetLastError());
 }
 
+// rocBLAS dgemm for the large Gram blocks and linear combinations instead of
+// the kernels here (GCGE_HIP_BLAS=1: a comparison for measurement; the
+// kernels here are the default)
+static bool use_blas()
+{
+    static const bool v = [] {
+        const char *e = getenv("GCGE_HIP_BLAS");
+        return e && atoi(e) == 1;
+    }();
+    return v;
+}
+static rocblas_handle blas_handle()
+{
+    static rocblas_handle h = nullptr;
+    if (!h) {
+        if (rocblas_create_handle(&h) != rocblas_status_success) fail_msg("rocblas_create_handle failed");
+        if (rocblas_set_stream(h, state().stream) != rocblas_status_success) fail_msg("rocblas_set_stream failed");
+    }
+    return h;
+}
+
 // The row ranges of a Gram launch: as many as `want`, or fewer when the
 // caller's partial buffer (cap doubles, 0: the backend's grown buffer)
 // cannot hold that many m x k blocks; rows per range rounded up to `gran`.
@@ -2067,6 +2101,14 @@ int gram_impl(int n, const double *V, long ldv, int m, const double *W, long ldw
     State &s = state();
     const long mk = (long)m * k;
     if (partial && (size_t)mk > partial_cap) return -1;
+    if (!partial && use_blas() && m > 16 && k > 16) {
+        ProfScope prof(profile ? "gram" : nullptr, 8.0 * (double)n * (m + k));
+        const double one = 1.0, zero = 0.0;
+        if (rocblas_dgemm(blas_handle(), rocblas_operation_transpose, rocblas_operation_none, m, k, n, &one,
+                          V, ldv, W, ldw, &zero, d_out, m) != rocblas_status_success)
+            fail_msg("rocblas_dgemm failed");
+        return 1;
+    }
     const char *pname = profile ? "gram" : nullptr;
     const char *rname = profile ? "gram_reduce" : nullptr;
     const double gbytes = 8.0 * (double)n * (m + k);
@@ -2100,19 +2142,9 @@ int gram_impl(int n, const double *V, long ldv, int m, const double *W, long ldw
     }();
     if (gram_mode == 0) {
         // matrix cores: 128 x 128 output blocks (grid y, z), tile pairs shared
-        // by four or eight waves; up to 304 x 128 outputs in one block (the
-        // Rayleigh-Ritz blocks of a 100-pair solve, 300 x 100), so that W is
-        // read once instead of once per 128 columns of V
-        static const int one_block = [] {
-            const char *e = getenv("GCGE_HIP_GRAM_ONEBLOCK");  // 0: 128 x 128 blocks (measurement)
-            return e ? atoi(e) : 1;
-        }();
-        int bcv = kGmCols, bcw = kGmCols;
-        if (one_block && m > kGmCols && m <= 304 && k <= kGmCols &&
-            ceil_div(m, 16) * ceil_div(k, 16) <= 8 * 17) {
-            bcv = m;
-            bcw = k;
-        }
+        // by four or eight waves (one block of up to 304 x 128 outputs, W read
+        // once, measured slower at 300 x 100: 5.85 against 4.77 ms)
+        const int bcv = kGmCols, bcw = kGmCols;
         const int gy = ceil_div(m, bcv), gz = ceil_div(k, bcw);
         const int ti = ceil_div(std::min(m, bcv), 16), tj = ceil_div(std::min(k, bcw), 16);
         // eight waves per workgroup beyond 32 output tiles (four would hold 9..16
@@ -2220,6 +2252,14 @@ void lincomb_dev(int n, const double *V, long ldv, int m, const double *d_Ct, in
         return;
     }
     ProfScope prof("lincomb", 8.0 * (double)n * (m + k) + (beta == 0.0 ? 0.0 : 8.0 * (double)n * k));
+    if (use_blas() && m > 16 && k > 16) {
+        // Y = beta Y + V C, C^T stored column-major in d_Ct (leading dimension ldct)
+        const double one = 1.0;
+        if (rocblas_dgemm(blas_handle(), rocblas_operation_none, rocblas_operation_transpose, n, k, m, &one, V,
+                          ldv, d_Ct, ldct, &beta, Y, ldy) != rocblas_status_success)
+            fail_msg("rocblas_dgemm failed");
+        return;
+    }
     const dim3 grid(ceil_div(n, kBlock)), block(kBlock);
     hipStream_t st = state().stream;
     static const int mode = [] {
@@ -2332,9 +2372,13 @@ static void bcg_p_t(int n, PT *P, long ldp, const int *d_pmap, const RT *R, long
 {
     if (k <= 0) return;
     ProfScope prof("bcg_p");
-    hipLaunchKernelGGL((k_bcg_p<PT, RT>), dim3(ceil_div(n, kBlock), k), dim3(kBlock), 0,
+    // about 2048 blocks over all columns, at least 4 x 256 rows each
+    const long per_col = std::max<long>(1, 2048 / std::max(1, k));
+    int rows = (int)std::max<long>(4L * kBlock, (n + per_col - 1) / per_col);
+    rows = (rows + 4 * kBlock - 1) / (4 * kBlock) * (4 * kBlock);
+    hipLaunchKernelGGL((k_bcg_p<PT, RT>), dim3(ceil_div(n, rows), k), dim3(kBlock), 0,
                        state().stream, n, P, ldp, d_pmap, R, ldr, d_rmap, d_beta, d_sold, d_snew,
-                       first, d_count);
+                       first, d_count, rows);
     HIP_CHECK(hipGetLastError());
 }
 
