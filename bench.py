@@ -306,7 +306,7 @@ def work_counts(lib, reset):
     return dict(zip(WORK_NAMES, w.tolist()))
 
 
-def cpu_model(K, M, work, reps=10):
+def cpu_model(K, M, work, reps=60):
     """Seconds the reference's CSR path needs for `work` (counts per solve,
     work_counts) on one host core: every count priced by the oracle port's
     own single-vector call (app/csr/gcge_app_csr.c:297-378 restated, serial
@@ -339,22 +339,22 @@ def cpu_model(K, M, work, reps=10):
     one = np.ones(1)
     L.ORACLE_Gram(n, 1, 1, x.ctypes.data_as(dp), n, y.ctypes.data_as(dp), n, g.ctypes.data_as(dp), 1)
     t0 = time.perf_counter()
-    for _ in range(5 * reps):
+    for _ in range(10 * reps):
         L.ORACLE_Gram(n, 1, 1, x.ctypes.data_as(dp), n, y.ctypes.data_as(dp), n,
                       g.ctypes.data_as(dp), 1)
-    t_dot = (time.perf_counter() - t0) / (5 * reps)
+    t_dot = (time.perf_counter() - t0) / (10 * reps)
     t0 = time.perf_counter()
-    for _ in range(5 * reps):
+    for _ in range(10 * reps):
         L.ORACLE_LinearComb(n, 1, 1, x.ctypes.data_as(dp), n, one.ctypes.data_as(dp), 1, 1.0,
                             y.ctypes.data_as(dp), n)
-    t_axpby = (time.perf_counter() - t0) / (5 * reps)
+    t_axpby = (time.perf_counter() - t0) / (10 * reps)
     # dense eigensolve at the mean order of the solve's calls
     n_eig = int(round((work["dense_eig_n3"] / work["dense_eig"]) ** (1.0 / 3.0))) if work["dense_eig"] else 0
     t_eig, eig_kind = 0.0, "none"
     if n_eig > 0:
         a = rng.standard_normal((n_eig, n_eig))
         a = np.asfortranarray(a + a.T)
-        t_eig, eig_kind = _time_dense_eig(a)
+        t_eig, eig_kind = _time_dense_eig(a, reps=10)
     secs = (work["spmv_nnz"] * per_nnz + work["dot"] * t_dot + work["axpby"] * t_axpby
             + work["dense_eig"] * t_eig)
     sample_s = time.perf_counter() - t_start

@@ -52,12 +52,14 @@ void GCGE_ComputeW(void *A, void *B, void **V, GCGE_DOUBLE *eval, GCGE_OPS *ops,
         GCGE_INT *xcols = cols, *rcols = cols + w_length, *wcols = cols + 2 * w_length;
         GCGE_DOUBLE *ra = coef, *rb = coef + w_length, *wa = coef + 2 * w_length,
                     *wb = coef + 3 * w_length;
+        int any_small = 0;
         for (GCGE_INT idx = 0; idx < w_length; ++idx) {
             const GCGE_INT xi = unlock[idx];
             xcols[idx] = xi;
             rcols[idx] = idx;
             wcols[idx] = w_start + idx;
             const int small = fabs(eval[xi]) <= 1.0;
+            any_small |= small;
             ra[idx] = 0.0;
             rb[idx] = small ? eval[xi] + para->shift : 1.0;
             wa[idx] = small ? 1.0 : 1.0 / (eval[xi] + para->shift);
@@ -72,7 +74,9 @@ void GCGE_ComputeW(void *A, void *B, void **V, GCGE_DOUBLE *eval, GCGE_OPS *ops,
         } else {
             ext->MatDotMultiVecCols(B, V, xcols, V_tmp, rcols, w_length, ops);
         }
-        ext->MultiVecAxpbyCols(ra, V_tmp, rcols, rb, V_tmp, rcols, w_length, ops);
+        /* the scaling pass only where some |lambda| <= 1 (the column loop
+         * below scales those columns only; y = 0 x + 1 y elsewhere) */
+        if (any_small) ext->MultiVecAxpbyCols(ra, V_tmp, rcols, rb, V_tmp, rcols, w_length, ops);
         ext->MultiVecAxpbyCols(wa, V, xcols, wb, V, wcols, w_length, ops);
         free(cols);
         free(coef);

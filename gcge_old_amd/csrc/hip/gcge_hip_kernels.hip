@@ -1451,6 +1451,89 @@ __global__ __launch_bounds__(kBlock) void k_lincomb_mfma(int n, const double *__
             }
 }
 
+// The same product with V staged through LDS: a workgroup owns 16 RT rows
+// and up to 128 output columns (4 waves x TJW column tiles), so each V value
+// is read from HBM once for all the columns (the wavefront kernel above
+// reads V once per 64 columns, and each wave waits for its own loads).
+// Stages of 16 input columns go through two LDS buffers: the next stage's
+// global loads are issued before the current stage's matrix-core products
+// and written to the other buffer after them, one barrier per stage.
+constexpr int kLcStage = 16;
+template <int TJW, int RT>
+__global__ __launch_bounds__(kBlock, 2) void k_lincomb_lds(int n, const double *__restrict__ V, long ldv,
+                                                          int m, const double *__restrict__ Ct,
+                                                          int ldct, double beta,
+                                                          double *__restrict__ Y, long ldy, int k)
+{
+    constexpr int R = 16 * RT;                       // rows per workgroup
+    constexpr int PER = kLcStage * R / kBlock;       // staged values per thread
+    __shared__ double lds[2][kLcStage][R];
+    const int t = threadIdx.x, lane = t % kWave, wid = t / kWave;
+    const int r0 = blockIdx.x * R;
+    const int jg = blockIdx.y * 128;
+    const int kk = lane / 16, li = lane % 16;
+    gcge_double4 acc[TJW][RT];
+#pragma unroll
+    for (int u = 0; u < TJW; ++u)
+#pragma unroll
+        for (int q = 0; q < RT; ++q) acc[u][q] = gcge_double4{0.0, 0.0, 0.0, 0.0};
+    double pv[PER];
+    auto load = [&](int m0) {
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int e = t + kBlock * u, c = e / R, r = e % R;
+            pv[u] = (m0 + c < m && r0 + r < n) ? V[(long)(m0 + c) * ldv + r0 + r] : 0.0;
+        }
+    };
+    auto stash = [&](int b) {
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int e = t + kBlock * u;
+            lds[b][e / R][e % R] = pv[u];
+        }
+    };
+    load(0);
+    stash(0);
+    __syncthreads();
+    int buf = 0;
+    for (int m0 = 0; m0 < m; m0 += kLcStage) {
+        const bool more = m0 + kLcStage < m;
+        if (more) load(m0 + kLcStage);
+#pragma unroll
+        for (int st = 0; st < kLcStage / 4; ++st) {
+            const int mm = m0 + 4 * st + kk;
+            double a[TJW], b[RT];
+#pragma unroll
+            for (int u = 0; u < TJW; ++u) {
+                const int j = jg + 16 * (wid + 4 * u) + li;
+                a[u] = (mm < m && j < k) ? Ct[(long)mm * ldct + j] : 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < RT; ++q) b[q] = lds[buf][4 * st + kk][16 * q + li];
+#pragma unroll
+            for (int u = 0; u < TJW; ++u)
+#pragma unroll
+                for (int q = 0; q < RT; ++q)
+                    acc[u][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[q], acc[u][q], 0, 0, 0);
+        }
+        if (more) stash(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+#pragma unroll
+    for (int u = 0; u < TJW; ++u)
+#pragma unroll
+        for (int q = 0; q < RT; ++q)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int j = jg + 16 * (wid + 4 * u) + kk + 4 * v, r = r0 + 16 * q + li;
+                if (j < k && r < n) {
+                    double *y = Y + (long)j * ldy + r;
+                    *y = (beta == 0.0) ? acc[u][q][v] : beta * (*y) + acc[u][q][v];
+                }
+            }
+}
+
 // ---- streaming kernels ------------------------------------------------------------------
 // y = a x + b y with the reference special cases (a == 0: x not read;
 // b == 0: y not read)
@@ -2271,6 +2354,23 @@ void lincomb_dev(int n, const double *V, long ldv, int m, const double *d_Ct, in
     // m = 70, k = 10: 27.7 us against 34.1 (vector kernel); m = 90, k = 50:
     // 113.7 against 136.3; m = 80, k = 30: 57.5 against 47.9, so the vector
     // kernel keeps 17..32
+    static const int lds_mode = [] {
+        const char *e = getenv("GCGE_HIP_LINCOMB_LDS");  // 0: the wavefront kernels (measurement)
+        return e ? atoi(e) : 1;
+    }();
+    if (mode == 0 && lds_mode && k > 32 && m >= 32) {
+        // V staged through LDS: 128 rows x up to 128 columns per workgroup
+        const int tjw = std::min(k, 128) > 64 ? 2 : 1;
+        const dim3 g(ceil_div(n, 128), ceil_div(k, 128));
+        if (tjw == 2)
+            hipLaunchKernelGGL((k_lincomb_lds<2, 8>), g, block, 0, st, n, V, ldv, m, d_Ct, ldct, beta, Y,
+                               ldy, k);
+        else
+            hipLaunchKernelGGL((k_lincomb_lds<1, 8>), g, block, 0, st, n, V, ldv, m, d_Ct, ldct, beta, Y,
+                               ldy, k);
+        HIP_CHECK(hipGetLastError());
+        return;
+    }
     if (mode == 0 && k > 8 && (k <= 16 || k > 32)) {
         const int wpb = kBlock / kWave;
         if (k <= 16) {

@@ -165,11 +165,15 @@ def test_gram_matches_oracle(n, m, k):
 
 
 @pytest.mark.parametrize("m,k", [(1, 1), (5, 7), (90, 8), (90, 9), (300, 50), (70, 16), (3, 17),
-                                 (20, 24), (21, 70), (6, 130), (300, 100), (41, 200)])
+                                 (20, 24), (21, 70), (6, 130), (300, 100), (41, 200), (37, 45),
+                                 (64, 33), (33, 129)])
 @pytest.mark.parametrize("beta", [0.0, 1.0, -0.5])
 def test_linear_comb_matches_oracle(m, k, beta):
-    """Vector kernel (k <= 8, 17..32) and matrix-core kernel (9..16, > 32:
-    one to four 64-column groups, m not a multiple of the 4-column step)."""
+    """Vector kernel (k <= 8, 17..32), matrix-core wavefront kernel (9..16,
+    and > 32 with m < 32: one to four 64-column groups, m not a multiple of
+    the 4-column step) and the LDS-staged matrix-core kernel (k > 32, m >= 32:
+    one or two 128-column groups, partial 16-column stages, a partial last
+    block of 128 rows)."""
     n = 3001
     rng = _rng(m * k)
     V = rng.standard_normal((n, m + 2))
