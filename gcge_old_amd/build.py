@@ -58,7 +58,7 @@ def build_core() -> str:
     with cf.ThreadPoolExecutor(max_workers=8) as ex:
         list(ex.map(_compile, jobs))
     out = os.path.join(LIB, "libgcge_amd.so")
-    _run([CC, "-shared", "-o", out] + objs + ["-lm"])
+    _run([CC, "-shared", "-o", out] + objs + ["-lm", "-pthread"])
     return out
 
 

@@ -23,7 +23,9 @@
  * when lda != n); otherwise eigenpairs il..iu are written to evec with
  * leading dimension lde.
  */
+#define _GNU_SOURCE /* sched_getaffinity */
 #include <math.h>
+#include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -219,9 +221,26 @@ void GCGE_Default_ArrayScale(GCGE_DOUBLE alpha, GCGE_DOUBLE *a, GCGE_INT length)
  * matrix-vector product, the rank-2 update, the reflector applications and
  * the plane rotations).  Backward stable; eigenvalues agree with LAPACK
  * dsyev to rounding, eigenvectors up to sign and the choice of basis inside
- * multiple eigenvalues. */
+ * multiple eigenvalues.
+ *
+ * From order GCGE_DENSE_MIN_N (default 80) the work is shared by a small
+ * pool of host threads (GCGE_DENSE_THREADS, default min(8, CPUs available);
+ * 1 turns it off).  Every element still sees the same operations in the same
+ * order, so the eigenpairs are bit-identical for every thread count:
+ *   - reduction: the columns of the trailing block are split between the
+ *     threads; A stays exactly symmetric (both triangles get the same
+ *     rank-2 terms, x_i w_c + w_i x_c), so the matrix-vector product entry
+ *     p_i is the dot product of column i with tau v, summed in the same
+ *     order as the column sweep; three barriers per reflector;
+ *   - QL: the calling thread runs the scalar recurrence (it needs only the
+ *     tridiagonal) and publishes each sweep's rotations in a log;
+ *   - meanwhile the other threads form Q (columns are independent, each gets
+ *     the reflectors in descending order; dealt out cyclically), then apply
+ *     the published sweeps in order to their own row ranges of it. */
 
-
+#include <pthread.h>
+#include <sched.h>
+#include <stdatomic.h>
 
 GCGE_INLINE double hypot_safe(double a, double b)
 {
@@ -256,96 +275,265 @@ GCGE_INLINE double norm2(const double *x, int m)
     return r;
 }
 
-/* A (n x n, column-major, full symmetric) -> tridiagonal (d, e) with
- * A = Q T Q^T; e[i] couples i and i+1.  The Householder vectors are left in
- * the columns of A below the subdiagonal (unit leading entry implied),
- * their factors in tau. */
-GCGE_INLINE void sym_tridiag(int n, double *A, double *d, double *e, double *tau, double *p)
+/* one step of a spin wait; yields the CPU now and then, so that a machine
+ * with fewer free cores than threads still makes progress */
+static inline void spin_step(unsigned *k)
 {
-#define AC(i, j) A[(size_t)(j) * n + (i)]
-    for (int j = 0; j < n - 2; ++j) {
-        const int m = n - j - 1; /* length of the column below the diagonal */
-        double *x = &AC(j + 1, j);
-        double alpha = x[0];
-        const double xnorm = m > 1 ? norm2(x + 1, m - 1) : 0.0;
-        d[j] = AC(j, j);
-        if (xnorm == 0.0) {
-            tau[j] = 0.0;
-            e[j] = alpha;
-            continue;
-        }
-        double beta = -copysign(hypot_safe(alpha, xnorm), alpha);
-        tau[j] = (beta - alpha) / beta;
-        const double scal = 1.0 / (alpha - beta);
-        for (int i = 1; i < m; ++i) x[i] *= scal;
-        x[0] = 1.0;
-        e[j] = beta;
-        /* p = tau A22 v, A22 = A(j+1:, j+1:) */
-        for (int i = 0; i < m; ++i) p[i] = 0.0;
-        for (int c = 0; c < m; ++c) {
-            const double vc = tau[j] * x[c];
-            const double *col = &AC(j + 1, j + 1 + c);
-            for (int i = 0; i < m; ++i) p[i] += col[i] * vc;
-        }
-        /* w = p - (tau/2)(p.v) v, stored in p */
-        double pv = 0.0;
-        for (int i = 0; i < m; ++i) pv += p[i] * x[i];
-        const double h = 0.5 * tau[j] * pv;
-        for (int i = 0; i < m; ++i) p[i] -= h * x[i];
-        /* A22 -= v w^T + w v^T */
-        for (int c = 0; c < m; ++c) {
-            double *col = &AC(j + 1, j + 1 + c);
-            const double wc = p[c], vc = x[c];
-            for (int i = 0; i < m; ++i) col[i] -= x[i] * wc + p[i] * vc;
-        }
+    if ((++*k & 1023u) == 0) {
+        sched_yield();
+        return;
     }
-    if (n >= 2) {
-        d[n - 2] = AC(n - 2, n - 2);
-        e[n - 2] = AC(n - 1, n - 2);
-        tau[n - 2] = 0.0;
-    }
-    d[n - 1] = AC(n - 1, n - 1);
-    e[n - 1] = 0.0;
-#undef AC
+#if defined(__x86_64__) || defined(__i386__)
+    __builtin_ia32_pause();
+#endif
 }
 
-/* Q = H_0 H_1 ... H_{n-3} into Z (column-major, ldz), reflectors applied
- * from the last to the first so each touches only the trailing block. */
-GCGE_INLINE void form_q(int n, const double *A, const double *tau, double *Z, int ldz)
+/* ---- worker pool -------------------------------------------------------- */
+enum { EIG_MAX_THREADS = 64, EIG_SPINS = 20000 };
+
+typedef struct {
+    pthread_mutex_t mu;
+    pthread_cond_t cv;
+    atomic_uint gen;  /* job generation; workers run a job when it changes */
+    atomic_int done;  /* workers finished with the current job */
+    atomic_int bar_count, bar_sense;   /* all threads */
+    atomic_int wbar_count, wbar_sense; /* the workers (threads 1 ..) */
+    int nthreads;     /* including the calling thread; 0 = not started */
+    unsigned gen0;    /* generation when the workers were started */
+    void (*fn)(void *, int);
+    void *arg;
+    /* sweep log of the QL phase, kept between calls */
+    double *lcs, *lsn;
+    int *llo, *lhi, *loff;
+    int lcap, lscap;
+} eig_pool;
+
+static eig_pool g_pool = {.mu = PTHREAD_MUTEX_INITIALIZER, .cv = PTHREAD_COND_INITIALIZER};
+/* one solve uses the pool at a time; a concurrent caller runs single-threaded */
+static pthread_mutex_t g_pool_busy = PTHREAD_MUTEX_INITIALIZER;
+static int g_pool_atfork;
+
+static void *pool_worker(void *arg)
 {
-    for (int c = 0; c < n; ++c)
-        for (int i = 0; i < n; ++i) Z[(size_t)c * ldz + i] = (i == c) ? 1.0 : 0.0;
-    for (int j = n - 3; j >= 0; --j) {
+    const int tid = (int)(intptr_t)arg;
+    unsigned seen = g_pool.gen0;
+    for (;;) {
+        unsigned g, spins = 0;
+        while ((g = atomic_load_explicit(&g_pool.gen, memory_order_acquire)) == seen) {
+            if (spins < EIG_SPINS) {
+                spin_step(&spins);
+                continue;
+            }
+            pthread_mutex_lock(&g_pool.mu);
+            while ((g = atomic_load_explicit(&g_pool.gen, memory_order_acquire)) == seen)
+                pthread_cond_wait(&g_pool.cv, &g_pool.mu);
+            pthread_mutex_unlock(&g_pool.mu);
+            break;
+        }
+        seen = g;
+        g_pool.fn(g_pool.arg, tid);
+        atomic_fetch_add_explicit(&g_pool.done, 1, memory_order_release);
+    }
+    return NULL;
+}
+
+/* a forked child has none of the parent's threads */
+static void pool_child_reset(void)
+{
+    pthread_mutex_init(&g_pool.mu, NULL);
+    pthread_cond_init(&g_pool.cv, NULL);
+    pthread_mutex_init(&g_pool_busy, NULL);
+    g_pool.nthreads = 0;
+}
+
+static int env_int(const char *name, int dflt)
+{
+    const char *s = getenv(name);
+    if (!s || !*s) return dflt;
+    return atoi(s);
+}
+
+/* threads to use (>= 1); starts the workers the first time */
+static int pool_threads(void)
+{
+    if (g_pool.nthreads > 0) return g_pool.nthreads;
+    int want = env_int("GCGE_DENSE_THREADS", 0);
+    if (want <= 0) {
+        cpu_set_t set;
+        int ncpu = 1;
+        if (sched_getaffinity(0, sizeof set, &set) == 0) ncpu = CPU_COUNT(&set);
+        want = ncpu < 8 ? ncpu : 8;
+    }
+    if (want > EIG_MAX_THREADS) want = EIG_MAX_THREADS;
+    if (want < 1) want = 1;
+    if (!g_pool_atfork) {
+        pthread_atfork(NULL, NULL, pool_child_reset);
+        g_pool_atfork = 1;
+    }
+    g_pool.gen0 = atomic_load(&g_pool.gen);
+    int started = 1;
+    for (int t = 1; t < want; ++t) {
+        pthread_t th;
+        pthread_attr_t at;
+        pthread_attr_init(&at);
+        pthread_attr_setdetachstate(&at, PTHREAD_CREATE_DETACHED);
+        if (pthread_create(&th, &at, pool_worker, (void *)(intptr_t)t) == 0) ++started;
+        pthread_attr_destroy(&at);
+        if (started != t + 1) break;
+    }
+    g_pool.nthreads = started;
+    return started;
+}
+
+static void pool_run(void (*fn)(void *, int), void *arg)
+{
+    g_pool.fn = fn;
+    g_pool.arg = arg;
+    atomic_store_explicit(&g_pool.done, 0, memory_order_relaxed);
+    pthread_mutex_lock(&g_pool.mu);
+    atomic_fetch_add_explicit(&g_pool.gen, 1, memory_order_release);
+    pthread_cond_broadcast(&g_pool.cv);
+    pthread_mutex_unlock(&g_pool.mu);
+    fn(arg, 0);
+    unsigned k = 0;
+    while (atomic_load_explicit(&g_pool.done, memory_order_acquire) < g_pool.nthreads - 1)
+        spin_step(&k);
+}
+
+/* sense-reversing barrier of T threads on (count, flag) */
+static void barrier_on(atomic_int *count, atomic_int *flag, int T, int *sense)
+{
+    if (T <= 1) return;
+    const int s = !*sense;
+    *sense = s;
+    if (atomic_fetch_add_explicit(count, 1, memory_order_acq_rel) == T - 1) {
+        atomic_store_explicit(count, 0, memory_order_relaxed);
+        atomic_store_explicit(flag, s, memory_order_release);
+    } else {
+        unsigned k = 0;
+        while (atomic_load_explicit(flag, memory_order_acquire) != s) spin_step(&k);
+    }
+}
+
+static void pool_barrier(int T, int *sense)
+{
+    barrier_on(&g_pool.bar_count, &g_pool.bar_sense, T, sense);
+}
+
+/* ---- the solve ---------------------------------------------------------- */
+typedef struct {
+    int n, T;
+    double *A, *d, *e, *tau, *p, *sn; /* caller's work array */
+    double *vv, *wb;                  /* tau v; one w per thread */
+    double *Z;
+    int ldz;
+    double *panel; /* per worker: its rows of Z, columns 64-byte aligned */
+    int ldp;
+    int ql_status;
+    /* one cache line each: the workers poll `published` while the others
+     * store their progress */
+    _Alignas(64) atomic_int published;
+    atomic_int finished;
+    struct {
+        _Alignas(64) atomic_int v;
+    } consumed[EIG_MAX_THREADS];
+} eig_job;
+
+#define AC(i, j) A[(size_t)(j) * n + (i)]
+
+/* Householder reflector of column j: d[j], e[j], tau[j]; the column below
+ * the diagonal becomes v (unit leading entry stored), vv = tau v.  Returns 0
+ * when the column is already reduced (tau = 0). */
+GCGE_INLINE int reflector(int n, double *A, int j, double *d, double *e, double *tau, double *vv)
+{
+    const int m = n - j - 1;
+    double *x = &AC(j + 1, j);
+    const double alpha = x[0];
+    const double xnorm = m > 1 ? norm2(x + 1, m - 1) : 0.0;
+    d[j] = AC(j, j);
+    if (xnorm == 0.0) {
+        tau[j] = 0.0;
+        e[j] = alpha;
+        return 0;
+    }
+    const double beta = -copysign(hypot_safe(alpha, xnorm), alpha);
+    tau[j] = (beta - alpha) / beta;
+    const double scal = 1.0 / (alpha - beta);
+    for (int i = 1; i < m; ++i) x[i] *= scal;
+    x[0] = 1.0;
+    e[j] = beta;
+    for (int c = 0; c < m; ++c) vv[c] = tau[j] * x[c];
+    return 1;
+}
+
+/* p_i = (A22 tau v)_i for i in [c0, c1): column i of A22 dotted with vv */
+GCGE_INLINE void reduce_matvec(int n, const double *A, int j, const double *vv, double *p, int c0,
+                               int c1)
+{
+    const int m = n - j - 1;
+    for (int i = c0; i < c1; ++i) {
+        const double *col = &AC(j + 1, j + 1 + i);
+        double s = 0.0;
+        for (int c = 0; c < m; ++c) s += col[c] * vv[c];
+        p[i] = s;
+    }
+}
+
+/* w = p - (tau/2)(p.v) v (the whole vector, into this thread's w), then
+ * A22(:, c) -= v w_c + w v_c for c in [c0, c1) */
+GCGE_INLINE void reduce_update(int n, double *A, int j, double tau_j, const double *p, double *w,
+                               int c0, int c1)
+{
+    const int m = n - j - 1;
+    const double *x = &AC(j + 1, j);
+    double pv = 0.0;
+    for (int i = 0; i < m; ++i) pv += p[i] * x[i];
+    const double h = 0.5 * tau_j * pv;
+    for (int i = 0; i < m; ++i) w[i] = p[i] - h * x[i];
+    for (int c = c0; c < c1; ++c) {
+        double *col = &AC(j + 1, j + 1 + c);
+        const double wc = w[c], vc = x[c];
+        for (int i = 0; i < m; ++i) col[i] -= x[i] * wc + w[i] * vc;
+    }
+}
+
+/* Q = H_0 H_1 ... H_{n-3} applied to column c of the identity: the
+ * reflectors j = min(c-1, n-3) .. 0, each touching rows j+1 .. n-1 */
+GCGE_INLINE void form_q_column(int n, const double *A, const double *tau, double *Z, int ldz, int c)
+{
+    double *zc = Z + (size_t)c * ldz;
+    for (int i = 0; i < n; ++i) zc[i] = (i == c) ? 1.0 : 0.0;
+    for (int j = (c - 1 < n - 3 ? c - 1 : n - 3); j >= 0; --j) {
         if (tau[j] == 0.0) continue;
         const int m = n - j - 1;
         const double *v = A + (size_t)j * n + j + 1;
-        for (int c = j + 1; c < n; ++c) {
-            double *z = Z + (size_t)c * ldz + j + 1;
-            double s = z[0];
-            for (int i = 1; i < m; ++i) s += v[i] * z[i];
-            s *= tau[j];
-            z[0] -= s;
-            for (int i = 1; i < m; ++i) z[i] -= s * v[i];
-        }
+        double *z = zc + j + 1;
+        double s = z[0];
+        for (int i = 1; i < m; ++i) s += v[i] * z[i];
+        s *= tau[j];
+        z[0] -= s;
+        for (int i = 1; i < m; ++i) z[i] -= s * v[i];
     }
 }
+#undef AC
 
-/* Apply the plane rotations i = hi-1 .. lo of one QL sweep to columns
- * lo .. hi of Z, as the sweep would one at a time:
+/* Apply the plane rotations i = hi-1 .. lo of one QL sweep (c_i = cs[i-lo],
+ * s_i = sn[i-lo]) to rows 0 .. nrows-1 of columns lo .. hi of Z, as the
+ * sweep would one at a time:
  *     (z_i, z_{i+1}) <- (c_i z_i - s_i z_{i+1}, s_i z_i + c_i z_{i+1}).
  * Rows are taken in chunks whose running column stays in registers, so every
  * element is loaded and stored once per sweep. */
-GCGE_INLINE void apply_sweep(double *Z, int ldz, int n, int lo, int hi, const double *cs,
-                        const double *sn)
+GCGE_INLINE void apply_sweep(double *Z, int ldz, int nrows, int lo, int hi, const double *cs,
+                             const double *sn)
 {
     enum { CH = 16 };
-    for (int k0 = 0; k0 < n; k0 += CH) {
-        const int kn = (n - k0 < CH) ? n - k0 : CH;
+    for (int k0 = 0; k0 < nrows; k0 += CH) {
+        const int kn = (nrows - k0 < CH) ? nrows - k0 : CH;
         double t[CH];
         const double *zh = Z + (size_t)hi * ldz + k0;
         for (int k = 0; k < kn; ++k) t[k] = zh[k];
         for (int i = hi - 1; i >= lo; --i) {
-            const double c = cs[i], s = sn[i];
+            const double c = cs[i - lo], s = sn[i - lo];
             const double *zi = Z + (size_t)i * ldz + k0;
             double *zn = Z + (size_t)(i + 1) * ldz + k0;
             for (int k = 0; k < kn; ++k) {
@@ -359,13 +547,45 @@ GCGE_INLINE void apply_sweep(double *Z, int ldz, int n, int lo, int hi, const do
     }
 }
 
-/* Implicit-shift QL on the tridiagonal (d, e), rotating columns of Z; the
- * rotations of a sweep are recorded in cs/sn (n entries each) and applied
- * together.  Returns 0 on success, 1 when an eigenvalue needs more than 60
- * sweeps. */
-GCGE_INLINE int tridiag_ql(int n, double *d, double *e, double *Z, int ldz, double *cs, double *sn)
+static int min_consumed(eig_job *J)
 {
-    for (int l = 0; l < n; ++l) {
+    int m = atomic_load_explicit(&J->consumed[1].v, memory_order_acquire);
+    for (int t = 2; t < J->T; ++t) {
+        const int c = atomic_load_explicit(&J->consumed[t].v, memory_order_acquire);
+        if (c < m) m = c;
+    }
+    return m;
+}
+
+/* hand sweep k (rotations lo .. hi-1) to the workers; the log is reused from
+ * the start once every worker has applied everything published */
+static void publish_sweep(eig_job *J, int *pos, int k, int lo, int hi, const double *cs,
+                          const double *sn)
+{
+    const int L = hi - lo;
+    if (*pos + L > g_pool.lcap || k - min_consumed(J) >= g_pool.lscap) {
+        unsigned spins = 0;
+        while (min_consumed(J) < k) spin_step(&spins);
+        *pos = 0;
+    }
+    const int slot = k % g_pool.lscap;
+    memcpy(g_pool.lcs + *pos, cs + lo, (size_t)L * sizeof(double));
+    memcpy(g_pool.lsn + *pos, sn + lo, (size_t)L * sizeof(double));
+    g_pool.llo[slot] = lo;
+    g_pool.lhi[slot] = hi;
+    g_pool.loff[slot] = *pos;
+    *pos += L;
+    atomic_store_explicit(&J->published, k + 1, memory_order_release);
+}
+
+/* Implicit-shift QL on the tridiagonal (d, e); the rotations of a sweep are
+ * recorded in cs/sn (n entries each) and applied to the columns of Z
+ * together, here (single thread) or by the workers.  Returns 0 on success,
+ * 1 when an eigenvalue needs more than 60 sweeps. */
+GCGE_INLINE int tridiag_ql(eig_job *J, int n, double *d, double *e, double *cs, double *sn)
+{
+    int k = 0, pos = 0, status = 0;
+    for (int l = 0; l < n && !status; ++l) {
         int iter = 0, m;
         do {
             for (m = l; m < n - 1; ++m) {
@@ -373,7 +593,10 @@ GCGE_INLINE int tridiag_ql(int n, double *d, double *e, double *Z, int ldz, doub
                 if (fabs(e[m]) <= 1.1102230246251565e-16 * dd) break;
             }
             if (m != l) {
-                if (iter++ == 60) return 1;
+                if (iter++ == 60) {
+                    status = 1;
+                    break;
+                }
                 double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
                 double r = hypot_safe(g, 1.0);
                 g = d[m] - d[l] + e[l] / (g + (g >= 0.0 ? fabs(r) : -fabs(r)));
@@ -401,7 +624,12 @@ GCGE_INLINE int tridiag_ql(int n, double *d, double *e, double *Z, int ldz, doub
                     sn[i] = s;
                 }
                 /* rotations m-1 .. i+1 were taken (all of them when i < l) */
-                if (i + 1 < m) apply_sweep(Z, ldz, n, i + 1, m, cs, sn);
+                if (i + 1 < m) {
+                    if (J->T > 1)
+                        publish_sweep(J, &pos, k++, i + 1, m, cs, sn);
+                    else
+                        apply_sweep(J->Z, J->ldz, n, i + 1, m, cs + i + 1, sn + i + 1);
+                }
                 if (r == 0.0 && i >= l) continue;
                 d[l] -= p;
                 e[l] = g;
@@ -409,19 +637,166 @@ GCGE_INLINE int tridiag_ql(int n, double *d, double *e, double *Z, int ldz, doub
             }
         } while (m != l);
     }
-    return 0;
+    if (J->T > 1) atomic_store_explicit(&J->finished, 1, memory_order_release);
+    return status;
+}
+
+/* rows of Z per QL worker */
+GCGE_INLINE int ql_rows(int n, int T) { return (n + T - 2) / (T - 1); }
+
+/* worker t >= 1 of the QL phase: rows [r0, r1) of Z, copied to the worker's
+ * own panel (no cache line shared with another worker), every published
+ * sweep applied in order, then copied back */
+GCGE_INLINE void ql_worker(eig_job *J, int t)
+{
+    const int n = J->n, chunk = ql_rows(n, J->T);
+    const int r0 = (t - 1) * chunk < n ? (t - 1) * chunk : n;
+    const int r1 = r0 + chunk < n ? r0 + chunk : n, nr = r1 - r0, ldp = J->ldp;
+    double *Pn = J->panel + (size_t)(t - 1) * ldp * n;
+    for (int c = 0; c < n && nr > 0; ++c)
+        memcpy(Pn + (size_t)c * ldp, J->Z + (size_t)c * J->ldz + r0, (size_t)nr * sizeof(double));
+    int k = 0;
+    unsigned spins = 0;
+    for (;;) {
+        int pub = atomic_load_explicit(&J->published, memory_order_acquire);
+        if (pub == k) {
+            if (!atomic_load_explicit(&J->finished, memory_order_acquire)) {
+                spin_step(&spins);
+                continue;
+            }
+            pub = atomic_load_explicit(&J->published, memory_order_acquire);
+            if (pub == k) break;
+        }
+        for (; k < pub; ++k) {
+            const int slot = k % g_pool.lscap;
+            const int off = g_pool.loff[slot];
+            if (nr > 0)
+                apply_sweep(Pn, ldp, nr, g_pool.llo[slot], g_pool.lhi[slot], g_pool.lcs + off,
+                            g_pool.lsn + off);
+            atomic_store_explicit(&J->consumed[t].v, k + 1, memory_order_release);
+        }
+    }
+    for (int c = 0; c < n && nr > 0; ++c)
+        memcpy(J->Z + (size_t)c * J->ldz + r0, Pn + (size_t)c * ldp, (size_t)nr * sizeof(double));
+}
+
+/* reflectors with fewer trailing rows than this are done by thread 0 alone */
+enum { EIG_PAR_MIN_M = 48 };
+
+GCGE_VECTOR_CLONES
+static void eig_job_run(void *arg, int t)
+{
+    eig_job *J = (eig_job *)arg;
+    const int n = J->n, T = J->T;
+    double *A = J->A, *d = J->d, *e = J->e, *tau = J->tau;
+    int sense = atomic_load_explicit(&g_pool.bar_sense, memory_order_acquire);
+    int wsense = atomic_load_explicit(&g_pool.wbar_sense, memory_order_acquire);
+    double *w = J->wb + (size_t)t * n;
+
+    /* tridiagonal reduction.  Thread 0 owns the leading columns of every
+     * trailing block, so after its share of update j it already holds the
+     * final column j+1 and computes reflector j+1 before the barrier. */
+    int j = 0;
+    /* m >= T keeps column 0 of every trailing block with thread 0 */
+    const int minm = EIG_PAR_MIN_M > T ? EIG_PAR_MIN_M : T;
+    if (T > 1 && n - 1 >= minm) {
+        if (t == 0) reflector(n, A, 0, d, e, tau, J->vv);
+        pool_barrier(T, &sense);
+        for (; j < n - 2; ++j) {
+            const int m = n - j - 1;
+            if (tau[j] != 0.0) {
+                const int c0 = (int)((long)m * t / T), c1 = (int)((long)m * (t + 1) / T);
+                reduce_matvec(n, A, j, J->vv, J->p, c0, c1);
+                pool_barrier(T, &sense);
+                reduce_update(n, A, j, tau[j], J->p, w, c0, c1);
+            }
+            const int next = j + 1 < n - 2 && m - 1 >= minm;
+            if (t == 0 && next) reflector(n, A, j + 1, d, e, tau, J->vv);
+            pool_barrier(T, &sense);
+            if (!next) {
+                ++j;
+                break;
+            }
+        }
+    }
+    if (t == 0) {
+        for (; j < n - 2; ++j)
+            if (reflector(n, A, j, d, e, tau, J->vv)) {
+                const int m = n - j - 1;
+                reduce_matvec(n, A, j, J->vv, J->p, 0, m);
+                reduce_update(n, A, j, tau[j], J->p, w, 0, m);
+            }
+        if (n >= 2) {
+            d[n - 2] = A[(size_t)(n - 2) * n + n - 2];
+            e[n - 2] = A[(size_t)(n - 2) * n + n - 1];
+            tau[n - 2] = 0.0;
+        }
+        d[n - 1] = A[(size_t)(n - 1) * n + n - 1];
+        e[n - 1] = 0.0;
+    }
+    pool_barrier(T, &sense);
+
+    if (T == 1) {
+        for (int c = 0; c < n; ++c) form_q_column(n, A, tau, J->Z, J->ldz, c);
+        J->ql_status = tridiag_ql(J, n, d, e, J->p, J->sn);
+        return;
+    }
+    /* thread 0 runs the QL recurrence (it needs only d and e) while the
+     * workers form the orthogonal factor, columns dealt out cyclically; they
+     * then apply the published sweeps to their rows */
+    if (t == 0) {
+        J->ql_status = tridiag_ql(J, n, d, e, J->p, J->sn);
+    } else {
+        for (int c = t - 1; c < n; c += T - 1) form_q_column(n, A, tau, J->Z, J->ldz, c);
+        barrier_on(&g_pool.wbar_count, &g_pool.wbar_sense, T - 1, &wsense);
+        ql_worker(J, t);
+    }
+    pool_barrier(T, &sense);
+}
+
+/* grow the pool's sweep log and per-thread buffers for order n */
+static int pool_reserve(int n, int T)
+{
+    const int lcap = 2 * n * n + 4 * n, lscap = 4 * n + 64;
+    if (lcap > g_pool.lcap) {
+        free(g_pool.lcs);
+        free(g_pool.lsn);
+        g_pool.lcs = (double *)malloc((size_t)lcap * sizeof(double));
+        g_pool.lsn = (double *)malloc((size_t)lcap * sizeof(double));
+        g_pool.lcap = (g_pool.lcs && g_pool.lsn) ? lcap : 0;
+    }
+    if (lscap > g_pool.lscap) {
+        free(g_pool.llo);
+        free(g_pool.lhi);
+        free(g_pool.loff);
+        g_pool.llo = (int *)malloc((size_t)lscap * sizeof(int));
+        g_pool.lhi = (int *)malloc((size_t)lscap * sizeof(int));
+        g_pool.loff = (int *)malloc((size_t)lscap * sizeof(int));
+        g_pool.lscap = (g_pool.llo && g_pool.lhi && g_pool.loff) ? lscap : 0;
+    }
+    (void)T;
+    return g_pool.lcap >= lcap && g_pool.lscap >= lscap;
 }
 
 /* All eigenpairs of the symmetric n x n matrix whose upper triangle is in
  * a (column-major, lda).  Eigenvalues ascending in eval; eigenvector j in
  * evec[j*lde .. j*lde+n-1] (lde >= n).  work needs n*n + 5n doubles. */
-GCGE_VECTOR_CLONES
 void GCGE_Dense_SymEig(GCGE_DOUBLE *a, GCGE_INT lda, GCGE_INT n, GCGE_DOUBLE *eval,
                        GCGE_DOUBLE *evec, GCGE_INT lde, GCGE_DOUBLE *work)
 {
     if (n <= 0) return;
-    double *A = work, *d = work + (size_t)n * n, *e = d + n, *tau = e + n, *p = tau + n,
-           *sn = p + n;
+    eig_job J;
+    memset(&J, 0, sizeof J);
+    J.n = n;
+    J.A = work;
+    J.d = work + (size_t)n * n;
+    J.e = J.d + n;
+    J.tau = J.e + n;
+    J.p = J.tau + n;
+    J.sn = J.p + n;
+    J.Z = evec;
+    J.ldz = lde;
+    double *A = J.A, *d = J.d;
     /* full symmetric copy from the upper triangle */
     for (int j = 0; j < n; ++j)
         for (int i = 0; i <= j; ++i) {
@@ -429,10 +804,40 @@ void GCGE_Dense_SymEig(GCGE_DOUBLE *a, GCGE_INT lda, GCGE_INT n, GCGE_DOUBLE *ev
             A[(size_t)j * n + i] = v;
             A[(size_t)i * n + j] = v;
         }
-    sym_tridiag(n, A, d, e, tau, p);
-    form_q(n, A, tau, evec, lde);
-    /* tau and p are free again: they hold the rotations of a sweep */
-    tridiag_ql(n, d, e, evec, lde, tau, sn);
+    int T = 1, locked = 0;
+    if (n >= env_int("GCGE_DENSE_MIN_N", 80) && pthread_mutex_trylock(&g_pool_busy) == 0) {
+        locked = 1;
+        T = pool_threads();
+        if (T > 1 && !pool_reserve(n, T)) T = 1;
+    }
+    J.T = T;
+    /* tau v and the per-thread w: the caller's p is one of them when T = 1 */
+    double *extra = NULL;
+    if (T > 1) {
+        J.ldp = (ql_rows(n, T) + 7) & ~7;
+        const size_t nv = (((size_t)T + 1) * n + 7) & ~(size_t)7;
+        const size_t bytes = (nv + (size_t)(T - 1) * J.ldp * n) * sizeof(double);
+        extra = (double *)aligned_alloc(64, (bytes + 63) & ~(size_t)63);
+        if (!extra) T = J.T = 1;
+    }
+    if (T > 1) {
+        J.vv = extra;
+        J.wb = extra + n;
+        J.panel = extra + ((((size_t)T + 1) * n + 7) & ~(size_t)7);
+        pool_run(eig_job_run, &J);
+    } else {
+        double *buf = (double *)malloc((size_t)2 * n * sizeof(double));
+        if (!buf) {
+            fprintf(stderr, "GCGE_Dense_SymEig: out of memory\n");
+            abort();
+        }
+        J.vv = buf;
+        J.wb = buf + n;
+        eig_job_run(&J, 0);
+        free(buf);
+    }
+    free(extra);
+    if (locked) pthread_mutex_unlock(&g_pool_busy);
     /* selection sort of the eigenvalues, swapping vector columns */
     for (int i = 0; i < n - 1; ++i) {
         int k = i;

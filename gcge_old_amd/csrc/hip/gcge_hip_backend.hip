@@ -1476,9 +1476,17 @@ static void ext_axpby_cols(const GCGE_DOUBLE *a, void **x, const GCGE_INT *xcols
 }
 
 // ---- dense symmetric eigensolve of the Rayleigh-Ritz matrix on the device ---------------
-// rocSOLVER syevd (tridiagonal reduction + divide and conquer) from n = 128 on;
-// below that the host solver is as fast and the call is declined.
-constexpr int kDeviceEigMinN = 128;
+// rocSOLVER syevd (tridiagonal reduction + divide and conquer) from order
+// GCGE_HIP_DEVICE_EIG_MIN (default 128) on; below that the call is declined
+// and the host solver (threaded from order 80) runs.
+static int device_eig_min_n()
+{
+    static const int v = [] {
+        const char *e = getenv("GCGE_HIP_DEVICE_EIG_MIN");
+        return e && *e ? atoi(e) : 128;
+    }();
+    return v;
+}
 
 struct DenseEig {
     rocblas_handle handle = nullptr;
@@ -1493,7 +1501,7 @@ static int ext_dense_eig(GCGE_DOUBLE *a, GCGE_INT lda, GCGE_INT n, GCGE_DOUBLE *
 {
     g_work[kWorkEig] += 1;
     g_work[kWorkEigN3] += (double)n * n * n;
-    if (n < kDeviceEigMinN) return 1;
+    if (n < device_eig_min_n()) return 1;
     State &s = state();
     DenseEig &e = g_eig;
     if (!e.handle) {

@@ -1457,7 +1457,9 @@ __global__ __launch_bounds__(kBlock) void k_lincomb_mfma(int n, const double *__
 // reads V once per 64 columns, and each wave waits for its own loads).
 // Stages of 16 input columns go through two LDS buffers: the next stage's
 // global loads are issued before the current stage's matrix-core products
-// and written to the other buffer after them, one barrier per stage.
+// and written to the other buffer after them, one barrier per stage.  The
+// stage's coefficients C(m0 .. m0+15, jg .. jg+127) go through LDS too: one
+// coalesced load per workgroup instead of one per wavefront and step.
 constexpr int kLcStage = 16;
 template <int TJW, int RT>
 __global__ __launch_bounds__(kBlock, 2) void k_lincomb_lds(int n, const double *__restrict__ V, long ldv,
@@ -1467,7 +1469,9 @@ __global__ __launch_bounds__(kBlock, 2) void k_lincomb_lds(int n, const double *
 {
     constexpr int R = 16 * RT;                       // rows per workgroup
     constexpr int PER = kLcStage * R / kBlock;       // staged values per thread
+    constexpr int CPER = kLcStage * 128 / kBlock;    // staged coefficients per thread
     __shared__ double lds[2][kLcStage][R];
+    __shared__ double ldc[2][kLcStage][128];
     const int t = threadIdx.x, lane = t % kWave, wid = t / kWave;
     const int r0 = blockIdx.x * R;
     const int jg = blockIdx.y * 128;
@@ -1477,12 +1481,17 @@ __global__ __launch_bounds__(kBlock, 2) void k_lincomb_lds(int n, const double *
     for (int u = 0; u < TJW; ++u)
 #pragma unroll
         for (int q = 0; q < RT; ++q) acc[u][q] = gcge_double4{0.0, 0.0, 0.0, 0.0};
-    double pv[PER];
+    double pv[PER], pc[CPER];
     auto load = [&](int m0) {
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
             const int e = t + kBlock * u, c = e / R, r = e % R;
             pv[u] = (m0 + c < m && r0 + r < n) ? V[(long)(m0 + c) * ldv + r0 + r] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < CPER; ++u) {
+            const int e = t + kBlock * u, i = e / 128, j = jg + e % 128;
+            pc[u] = (m0 + i < m && j < k) ? Ct[(long)(m0 + i) * ldct + j] : 0.0;
         }
     };
     auto stash = [&](int b) {
@@ -1490,6 +1499,11 @@ __global__ __launch_bounds__(kBlock, 2) void k_lincomb_lds(int n, const double *
         for (int u = 0; u < PER; ++u) {
             const int e = t + kBlock * u;
             lds[b][e / R][e % R] = pv[u];
+        }
+#pragma unroll
+        for (int u = 0; u < CPER; ++u) {
+            const int e = t + kBlock * u;
+            ldc[b][e / 128][e % 128] = pc[u];
         }
     };
     load(0);
@@ -1501,13 +1515,9 @@ __global__ __launch_bounds__(kBlock, 2) void k_lincomb_lds(int n, const double *
         if (more) load(m0 + kLcStage);
 #pragma unroll
         for (int st = 0; st < kLcStage / 4; ++st) {
-            const int mm = m0 + 4 * st + kk;
             double a[TJW], b[RT];
 #pragma unroll
-            for (int u = 0; u < TJW; ++u) {
-                const int j = jg + 16 * (wid + 4 * u) + li;
-                a[u] = (mm < m && j < k) ? Ct[(long)mm * ldct + j] : 0.0;
-            }
+            for (int u = 0; u < TJW; ++u) a[u] = ldc[buf][4 * st + kk][16 * (wid + 4 * u) + li];
 #pragma unroll
             for (int q = 0; q < RT; ++q) b[q] = lds[buf][4 * st + kk][16 * q + li];
 #pragma unroll
