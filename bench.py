@@ -153,8 +153,9 @@ def pmc_probe(dK, k_cg):
     """Isolated launches for the counter passes, three of each: the block-CG
     float32 product on K with k_cg columns (through the block CG with a
     fixed step count: one direction update, product and x/r update per
-    step), the float64 products with k_cg and 2 k_cg columns, and a plain
-    copy of n x 64 doubles (the calibration)."""
+    step), the float64 products with 10 (the float64 block-CG shape at 64^3),
+    k_cg and 2 k_cg columns, and a plain copy of n x 64 doubles (the
+    calibration)."""
     import gcge_old_amd as gcge
     from gcge_old_amd import _native
 
@@ -171,7 +172,8 @@ def pmc_probe(dK, k_cg):
     lib.GCGE_HIP_MultiLinearSolve(dK.handle, Y.handle, 0, X.handle, 0, k_cg)
     lib.GCGE_HIP_SetLinearSolverParams(20, 1e-2, 0, 0.0, None)
     dK.set_precision(False)
-    for k in (k_cg, 2 * k_cg):
+    widths = sorted({10, k_cg, 2 * k_cg})
+    for k in widths:
         for _ in range(3):
             lib.GCGE_HIP_SpMM(dK.handle, X.handle, 0, Y.handle, 0, k)
     for _ in range(3):
@@ -181,8 +183,7 @@ def pmc_probe(dK, k_cg):
     fixed32 = dK.nnz * 8 + 4 * (n + 1)
     print(json.dumps({"pmc_probe": True, "n": n, "nnz": dK.nnz,
                       "spmm32_cg_algorithmic_bytes": fixed32 + 8 * k_cg * n,
-                      "spmm64_algorithmic_bytes": {str(k): fixed64 + 16 * k * n
-                                                   for k in (k_cg, 2 * k_cg)},
+                      "spmm64_algorithmic_bytes": {str(k): fixed64 + 16 * k * n for k in widths},
                       "copy_bytes_read": 8 * n * 64, "copy_bytes_written": 8 * n * 64}),
           flush=True)
 
@@ -604,8 +605,11 @@ def solve_workload(args):
                                     + ("the block-CG shape (float32, nev columns)" if dom == "spmm32"
                                        else "10 columns (the block-CG shape)")
                                     + ", next to that launch's algorithmic bytes",
-                    "kernel": ("k_dia_spmm float32 block-CG products (direction block times K, "
-                               "with p.w)" if dom == "spmm32" else "k_dia_spmm float64 products"),
+                    "kernel": ("float32 block-CG products (direction block times K with p.w; "
+                               "k_dia_tile LDS plane tiles from 8 columns, k_dia_spmm below)"
+                               if dom == "spmm32" else
+                               "float64 products (k_dia_tile LDS plane tiles from 8 columns, "
+                               "k_dia_spmm below)"),
                     "launches": sp["launches"],
                     "avg_launch_us": round(sp["ms"] * 1e3 / sp["launches"], 2),
                     "algorithmic_bytes_per_launch": round(sp["bytes"] / sp["launches"]),

@@ -1187,8 +1187,9 @@ static void bcg_iterate_dev(Mat *A, int n, int len, PT *Rd, long ldr, int r0, XT
     const int steps = std::max(0, max_it - 1);
     bcg_control(c, 1, 0, steps > 0 ? 1 : 0);
     // fixed and per-column algorithmic bytes of each product, for the profile
+    // (P read, W written; p.w and p.p reuse the P values the product reads)
     const double fixed = (double)A->nnz * (scaled ? 8 : 12) + 4.0 * (A->nrows + 1);
-    const double per_col = (scaled ? 4.0 : 8.0) * (A->ncols + 2.0 * A->nrows);
+    const double per_col = (scaled ? 4.0 : 8.0) * ((double)A->ncols + A->nrows);
     std::vector<int> slots(steps, -1);
     for (int it = 1; it <= steps; ++it) {
         if constexpr (scaled)

@@ -46,6 +46,7 @@ struct State {
     int ring_next = 0;
     uint64_t seed = 0x5eed;
     uint64_t draw = 0;
+    int cus = 0;  // compute units of the device (0: not queried yet)
     bool profiling = false;
     int profile_mode = 0;  // 1: every launch, 2: SpMM launches only
 };

@@ -537,10 +537,12 @@ void launch_dia_spmm(const Mat *A, const VT *vals, const XT *X, long ldx, XT *Y,
 // (b, then a, then d), as the CSR rows list them and as k_dia_spmm adds
 // them; diagonals the matrix lacks contribute zero products (finite X), so
 // both kernels give the same sums.
-// Workgroup b runs on XCD b % 8; XCD x takes plane groups x, x + 8, ...,
-// each with all its in-plane tiles in order: the in-plane halo of a tile is
-// read by its neighbours on the same XCD, the planes shared by two plane
-// groups are read by neighbouring XCDs at the same time.
+// Workgroup b runs on XCD b % 8; XCD x takes a contiguous range of plane
+// groups, in-plane tile by in-plane tile, so that the workgroups resident on
+// an XCD at one time cover neighbouring groups and tiles: the planes two
+// groups share and the in-plane halos come from the XCD's L2 (at 128^3 the
+// L2 misses of a 100-column float64 product fell from 1.69 to 1.17 times the
+// algorithmic read bytes).
 constexpr int kTileRows = 256;
 struct TileDesc {
     int S, N, planes, tq, groups, nparts;  // strides, planes, in-plane tiles, plane groups,
@@ -566,8 +568,14 @@ __global__ __launch_bounds__(kTileRows, 2) void k_dia_tile(TileDesc td, const VT
     extern __shared__ __attribute__((aligned(16))) unsigned char tile_smem[];
     Pack *lds = reinterpret_cast<Pack *>(tile_smem);
     const int xcd = blockIdx.x % kXcds, loc = blockIdx.x / kXcds;
-    const int zg = kXcds * (loc / td.tq) + xcd, qt = loc % td.tq;
-    if (zg >= td.groups) return;
+    // XCD x: the contiguous plane groups [g0, g0 + gx), tile-major, so that
+    // adjacent groups of one in-plane tile run side by side and share their
+    // halo planes through the XCD's L2
+    const int gq = td.groups / kXcds, gr = td.groups % kXcds;
+    const int gx = gq + (xcd < gr ? 1 : 0), g0 = xcd * gq + min(xcd, gr);
+    if (gx == 0) return;
+    const int qt = loc / gx, zg = g0 + loc % gx;
+    if (qt >= td.tq) return;
     const int W = kTileRows + 2 * td.N + 2;
     const int t = threadIdx.x;
     const int q0 = qt * kTileRows, z0 = zg * PZ;
@@ -1324,6 +1332,149 @@ void launch_gram_mfma(int nw, int pw, int ept, dim3 grid, size_t lds, int n, int
 #undef GCGE_GM_ARGS
 }
 
+// V^T W on the matrix cores for the wide projection blocks (64 <= m <= 128
+// MI, 64 <= k <= 16 NJ): one workgroup of eight waves holds the whole m x k
+// result of its row range, so V and W are read from HBM once.  Wave w owns
+// the output row tiles w, w + 8, w + 16 (up to MI of them) against every
+// column tile; per step of 4 rows it reads its column operands once and its
+// row operands once from LDS for up to MI x NJ matrix-core products.  Stages
+// of 16 rows go through two LDS buffers (row stride kGlLd, V columns first):
+// the next stage's global loads are issued before the current stage's
+// products and written to the other buffer after them, one barrier per
+// stage.  MFMA operand and accumulator layout as in k_gram_mfma; partials as
+// k_gram (one m x k block per row range).
+constexpr int kGlStage = 16;
+
+template <int MI, int NJ>
+__global__ __launch_bounds__(512) void k_gram_lds(int n, int rows_per_block,
+                                                  const double *__restrict__ V, long ldv, int m,
+                                                  const double *__restrict__ W, long ldw, int k,
+                                                  double *__restrict__ partial)
+{
+    constexpr int kLd = MI * 128 + NJ * 16 + 1;  // odd: rows start in different banks
+    constexpr int kPass = 512 / kGlStage;        // columns staged per pass
+    constexpr int EPT = (MI * 128 + NJ * 16 + kPass - 1) / kPass;
+    __shared__ double lds[2][kGlStage][kLd];
+    const int ti = (m + 15) / 16, tj = (k + 15) / 16;
+    const int MP = ti * 16, ncols = MP + tj * 16;
+    const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+    const int li = lane % 16, lq = lane / 16;
+    const int r0 = blockIdx.x * rows_per_block, r1 = min(n, r0 + rows_per_block);
+    const int q = threadIdx.x % kGlStage, c0 = threadIdx.x / kGlStage;
+    double buf[EPT];
+    auto load = [&](int rs) {
+        const int r = rs + q;
+#pragma unroll
+        for (int u = 0; u < EPT; ++u) {
+            const int c = c0 + kPass * u;
+            double v = 0.0;
+            if (r < r1) {
+                if (c < MP) {
+                    if (c < m) v = V[(long)c * ldv + r];
+                } else if (c - MP < k) {
+                    v = W[(long)(c - MP) * ldw + r];
+                }
+            }
+            buf[u] = v;
+        }
+    };
+    auto stash = [&](int b) {
+#pragma unroll
+        for (int u = 0; u < EPT; ++u) {
+            const int c = c0 + kPass * u;
+            if (c < ncols) lds[b][q][c] = buf[u];
+        }
+    };
+    gcge_double4 acc[MI][NJ];
+#pragma unroll
+    for (int i = 0; i < MI; ++i)
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) acc[i][j] = gcge_double4{0.0, 0.0, 0.0, 0.0};
+    if (r0 < r1) {
+        load(r0);
+        stash(0);
+    }
+    __syncthreads();
+    int b = 0;
+    for (int rs = r0; rs < r1; rs += kGlStage) {
+        const bool more = rs + kGlStage < r1;
+        if (more) load(rs + kGlStage);
+#pragma unroll
+        for (int kk = 0; kk < kGlStage / 4; ++kk) {
+            const int qa = 4 * kk + lq;
+            double bw[NJ];
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) bw[j] = j < tj ? lds[b][qa][MP + 16 * j + li] : 0.0;
+#pragma unroll
+            for (int i = 0; i < MI; ++i) {
+                const int it = wid + 8 * i;
+                if (it < ti) {
+                    const double a = lds[b][qa][16 * it + li];
+#pragma unroll
+                    for (int j = 0; j < NJ; ++j)
+                        if (j < tj) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bw[j], acc[i][j], 0, 0, 0);
+                }
+            }
+        }
+        if (more) stash(b ^ 1);
+        __syncthreads();
+        b ^= 1;
+    }
+    double *out = partial + (long)blockIdx.x * m * k;
+#pragma unroll
+    for (int i = 0; i < MI; ++i) {
+        const int it = wid + 8 * i;
+        if (it >= ti) continue;
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+            const int col = 16 * j + li;
+            if (j >= tj || col >= k) continue;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = 16 * it + lq + 4 * v;
+                if (row < m) out[(long)col * m + row] = acc[i][j][v];
+            }
+        }
+    }
+}
+
+static int num_cus()
+{
+    State &s = state();
+    if (!s.cus) {
+        int v = 0;
+        HIP_CHECK(hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, s.device));
+        s.cus = std::max(1, v);
+    }
+    return s.cus;
+}
+
+// the k_gram_lds instance for an m x k block, false when none covers it
+static bool launch_gram_lds(int nbx, int n, int rows, const double *V, long ldv, int m, const double *W,
+                     long ldw, int k, double *partial)
+{
+    const int ti = (m + 15) / 16, tj = (k + 15) / 16;
+    const int mi = (ti + 7) / 8;
+    hipStream_t st = state().stream;
+#define GCGE_GL(MI, NJ)                                                                                  \
+    hipLaunchKernelGGL((k_gram_lds<MI, NJ>), dim3(nbx), dim3(512), 0, st, n, rows, V, ldv, m, W, ldw, k, \
+                       partial)
+    if (tj <= 7) {
+        if (mi == 1) GCGE_GL(1, 7);
+        else if (mi == 2) GCGE_GL(2, 7);
+        else if (mi == 3) GCGE_GL(3, 7);
+        else return false;
+    } else if (tj == 8) {
+        if (mi == 1) GCGE_GL(1, 8);
+        else if (mi == 2) GCGE_GL(2, 8);
+        else return false;
+    } else {
+        return false;
+    }
+#undef GCGE_GL
+    return true;
+}
+
 // out[e] = sum_b partial[b * stride + e], e < count.  A block owns OUTS
 // consecutive outputs; its 256 / OUTS threads per output split the partials
 // (coalesced over e), each with 4 independent running sums, and combine them
@@ -1763,6 +1914,91 @@ __global__ __launch_bounds__(kBlock) void k_bcg_p(int n, PT *__restrict__ P, lon
     }
 }
 
+// Four consecutive elements as one 16-byte access (float) or two (double),
+// widened to double; the address is 16-byte aligned.
+template <typename T>
+struct Quad;
+template <>
+struct Quad<float> {
+    static __device__ __forceinline__ void load(const float *p, double (&v)[4])
+    {
+        const float4 t = *reinterpret_cast<const float4 *>(p);
+        v[0] = t.x;
+        v[1] = t.y;
+        v[2] = t.z;
+        v[3] = t.w;
+    }
+    static __device__ __forceinline__ void store(float *p, const double (&v)[4])
+    {
+        *reinterpret_cast<float4 *>(p) = make_float4((float)v[0], (float)v[1], (float)v[2], (float)v[3]);
+    }
+};
+template <>
+struct Quad<double> {
+    static __device__ __forceinline__ void load(const double *p, double (&v)[4])
+    {
+        const double2 a = *reinterpret_cast<const double2 *>(p);
+        const double2 b = *reinterpret_cast<const double2 *>(p + 2);
+        v[0] = a.x;
+        v[1] = a.y;
+        v[2] = b.x;
+        v[3] = b.y;
+    }
+    static __device__ __forceinline__ void store(double *p, const double (&v)[4])
+    {
+        *reinterpret_cast<double2 *>(p) = make_double2(v[0], v[1]);
+        *reinterpret_cast<double2 *>(p + 2) = make_double2(v[2], v[3]);
+    }
+};
+
+// k_bcg_p with 16-byte accesses: a thread takes four consecutive rows per
+// access, two such groups at a time (rows_per_block is a multiple of 4; the
+// columns are 16-byte aligned).  Element arithmetic as in k_bcg_p.
+template <typename PT, typename RT>
+__global__ __launch_bounds__(kBlock) void k_bcg_p4(int n, PT *__restrict__ P, long ldp,
+                                                  const int *__restrict__ pmap,
+                                                  const RT *__restrict__ R, long ldr,
+                                                  const int *__restrict__ rmap,
+                                                  const double *__restrict__ beta,
+                                                  const double *__restrict__ s_old,
+                                                  const double *__restrict__ s_new, int first,
+                                                  const int *__restrict__ count, int rows_per_block)
+{
+    const int c = blockIdx.y;
+    if (count && c >= *count) return;
+    PT *p = P + (long)pmap[c] * ldp;
+    const RT *rr = R + (long)rmap[c] * ldr;
+    const double b = s_new ? beta[c] * s_old[c] : beta[c];
+    const double sn = s_new ? s_new[c] : 1.0;
+    const int r0 = blockIdx.x * rows_per_block, r1 = min(n, r0 + rows_per_block);
+    const int r1q = r0 + ((r1 - r0) & ~3);
+    auto one = [&](double r, double pv) -> double {
+        const double t = first ? r : r + b * pv;
+        return s_new ? t / sn : t;
+    };
+    for (int i0 = r0 + 4 * threadIdx.x; i0 < r1q; i0 += 8 * kBlock) {
+        const int i1 = i0 + 4 * kBlock;
+        const bool two = i1 < r1q;
+        double ra[4], pa[4] = {0.0, 0.0, 0.0, 0.0}, rb[4], pb[4] = {0.0, 0.0, 0.0, 0.0};
+        Quad<RT>::load(rr + i0, ra);
+        if (!first) Quad<PT>::load(p + i0, pa);
+        if (two) {
+            Quad<RT>::load(rr + i1, rb);
+            if (!first) Quad<PT>::load(p + i1, pb);
+        }
+        double oa[4], ob[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            oa[u] = one(ra[u], pa[u]);
+            ob[u] = two ? one(rb[u], pb[u]) : 0.0;
+        }
+        Quad<PT>::store(p + i0, oa);
+        if (two) Quad<PT>::store(p + i1, ob);
+    }
+    for (int i = r1q + threadIdx.x; i < r1; i += kBlock)
+        p[i] = (PT)one((double)rr[i], first ? 0.0 : (double)p[i]);
+}
+
 // p.w and p.p per column (the float32 directions of the block CG),
 // partials[c * nbx + bx] and partials[(k + c) * nbx + bx]
 template <typename PT>
@@ -1823,6 +2059,62 @@ __global__ __launch_bounds__(kBlock) void k_bcg_xr(int n, int rows_per_block,
     const int r0 = bx * rows_per_block, r1 = min(n, r0 + rows_per_block);
     double s = 0.0;
     for (int i = r0 + threadIdx.x; i < r1; i += kBlock) {
+        x[i] = (XT)(alpha * (double)p[i] + (double)x[i]);
+        const RT rn = (RT)(-alpha * (double)w[i] + (double)r[i]);
+        r[i] = rn;
+        s += (double)rn * (double)rn;
+    }
+    s = block_sum(s, lds);
+    if (threadIdx.x == 0) partial[(long)c * gridDim.x + bx] = s;
+}
+
+// k_bcg_xr with 16-byte accesses (four consecutive rows per access;
+// rows_per_block a multiple of 4, columns 16-byte aligned).  Element
+// arithmetic as in k_bcg_xr; the r.r partial of a block sums the same rows in
+// another order.
+template <typename PT, typename RT, typename XT>
+__global__ __launch_bounds__(kBlock) void k_bcg_xr4(int n, int rows_per_block,
+                                                   XT *__restrict__ X, long ldx,
+                                                   const int *__restrict__ xmap,
+                                                   RT *__restrict__ R, long ldr,
+                                                   const int *__restrict__ rmap,
+                                                   const PT *__restrict__ P, long ldp,
+                                                   const int *__restrict__ pmap,
+                                                   const PT *__restrict__ W, long ldw,
+                                                   const int *__restrict__ wmap,
+                                                   const double *__restrict__ rho2,
+                                                   const double *__restrict__ ptw,
+                                                   const double *__restrict__ scale,
+                                                   double *__restrict__ partial,
+                                                   const int *__restrict__ count)
+{
+    __shared__ double lds[kBlock / kWave];
+    const int bx = blockIdx.x, c = blockIdx.y;
+    if (count && c >= *count) return;
+    const double alpha = scale ? rho2[c] / (scale[c] * ptw[c]) : rho2[c] / ptw[c];
+    XT *x = X + (long)xmap[c] * ldx;
+    RT *r = R + (long)rmap[c] * ldr;
+    const PT *p = P + (long)pmap[c] * ldp;
+    const PT *w = W + (long)wmap[c] * ldw;
+    const int r0 = bx * rows_per_block, r1 = min(n, r0 + rows_per_block);
+    const int r1q = r0 + ((r1 - r0) & ~3);
+    double s = 0.0;
+    for (int i0 = r0 + 4 * threadIdx.x; i0 < r1q; i0 += 4 * kBlock) {
+        double pa[4], xa[4], wa[4], ra[4], xo[4], ro[4];
+        Quad<PT>::load(p + i0, pa);
+        Quad<XT>::load(x + i0, xa);
+        Quad<PT>::load(w + i0, wa);
+        Quad<RT>::load(r + i0, ra);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            xo[u] = (double)(XT)(alpha * pa[u] + xa[u]);
+            ro[u] = (double)(RT)(-alpha * wa[u] + ra[u]);
+            s += ro[u] * ro[u];
+        }
+        Quad<XT>::store(x + i0, xo);
+        Quad<RT>::store(r + i0, ro);
+    }
+    for (int i = r1q + threadIdx.x; i < r1; i += kBlock) {
         x[i] = (XT)(alpha * (double)p[i] + (double)x[i]);
         const RT rn = (RT)(-alpha * (double)w[i] + (double)r[i]);
         r[i] = rn;
@@ -2051,8 +2343,9 @@ void spmm_dot(const Mat *A, const double *P, long ldp, double *W, long ldw, int 
               int *prof_slot)
 {
     if (k <= 0 || A->nrows <= 0) return;
+    // P read and W written (the p.w epilogue reuses the P values the product reads)
     const double bytes = (double)A->nnz * (8 + 4) + 4.0 * (A->nrows + 1) +
-                         8.0 * (double)k * (A->ncols + 2.0 * A->nrows);
+                         8.0 * (double)k * ((double)A->ncols + A->nrows);
     const bool rows = !A->use_dia && A->use_rows && k >= kRowsMinCols;
     const bool tiles = A->use_dia && use_tiles<double>(A, k);
     const int nparts = tiles ? tile_parts<double>(A)
@@ -2085,7 +2378,7 @@ void spmm_dot(const Mat *A, const float *P, long ldp, float *W, long ldw, int k,
     if (k <= 0 || A->nrows <= 0) return;
     if (!A->d_val32) fail_msg("float32 SpMM on a matrix without float32 values");
     const double bytes = (double)A->nnz * (4 + 4) + 4.0 * (A->nrows + 1) +
-                         4.0 * (double)k * (A->ncols + 2.0 * A->nrows);
+                         4.0 * (double)k * ((double)A->ncols + A->nrows);
     const bool dia = A->use_dia && A->d_dia32;
     const bool rows = !dia && A->use_rows && k >= kRowsMinCols;
     const bool tiles = dia && use_tiles<float>(A, k);
@@ -2233,6 +2526,29 @@ int gram_impl(int n, const double *V, long ldv, int m, const double *W, long ldw
         const char *e = getenv("GCGE_HIP_GRAM");  // "fma": the vector-FMA kernels (measurement)
         return (e && strcmp(e, "fma") == 0) ? 1 : 0;
     }();
+    // smallest m and k for the whole-block kernel (GCGE_HIP_GRAM_LDS; 0: the
+    // k_gram_mfma blocks only, for measurement)
+    static const int gram_lds_min = [] {
+        const char *e = getenv("GCGE_HIP_GRAM_LDS");
+        return e ? atoi(e) : 64;
+    }();
+    const int gti = ceil_div(m, 16), gtj = ceil_div(k, 16);
+    if (gram_mode == 0 && gram_lds_min > 0 && m >= gram_lds_min && k >= gram_lds_min &&
+        ((gtj <= 7 && gti <= 24) || (gtj == 8 && gti <= 16))) {
+        // one workgroup per CU (eight waves, up to 127 KiB of LDS), the whole
+        // m x k block each, about 500 stages of 16 rows per workgroup at 128^3
+        nbx = gram_ranges(n, std::min(ceil_div(n, 4 * kGlStage), num_cus()), mk, partial_cap, kGlStage,
+                          &rows);
+        if (!partial) partial = partial_buffer((size_t)nbx * mk);
+        {
+            ProfScope prof(pname, gbytes);
+            if (!launch_gram_lds(nbx, n, rows, V, ldv, m, W, ldw, k, partial)) fail_msg("k_gram_lds shape");
+            HIP_CHECK(hipGetLastError());
+        }
+        ProfScope prof(rname);
+        reduce_partials(partial, mk, nbx, mk, d_out);
+        return nbx;
+    }
     if (gram_mode == 0) {
         // matrix cores: 128 x 128 output blocks (grid y, z), tile pairs shared
         // by four or eight waves (one block of up to 304 x 128 outputs, W read
@@ -2475,6 +2791,13 @@ void residual_sq(int n, const double *AX, long ldax, const double *BX, long ldbx
     HIP_CHECK(hipGetLastError());
 }
 
+// every column of a block with leading dimension ld starts 16-byte aligned
+template <typename T>
+static bool quad_ok(const T *base, long ld)
+{
+    return ((uintptr_t)base % 16) == 0 && ((size_t)ld * sizeof(T)) % 16 == 0;
+}
+
 template <typename PT, typename RT>
 static void bcg_p_t(int n, PT *P, long ldp, const int *d_pmap, const RT *R, long ldr,
                     const int *d_rmap, const double *d_beta, const double *d_sold,
@@ -2486,9 +2809,14 @@ static void bcg_p_t(int n, PT *P, long ldp, const int *d_pmap, const RT *R, long
     const long per_col = std::max<long>(1, 2048 / std::max(1, k));
     int rows = (int)std::max<long>(4L * kBlock, (n + per_col - 1) / per_col);
     rows = (rows + 4 * kBlock - 1) / (4 * kBlock) * (4 * kBlock);
-    hipLaunchKernelGGL((k_bcg_p<PT, RT>), dim3(ceil_div(n, rows), k), dim3(kBlock), 0,
-                       state().stream, n, P, ldp, d_pmap, R, ldr, d_rmap, d_beta, d_sold, d_snew,
-                       first, d_count, rows);
+    if (quad_ok(P, ldp) && quad_ok(R, ldr))
+        hipLaunchKernelGGL((k_bcg_p4<PT, RT>), dim3(ceil_div(n, rows), k), dim3(kBlock), 0,
+                           state().stream, n, P, ldp, d_pmap, R, ldr, d_rmap, d_beta, d_sold, d_snew,
+                           first, d_count, rows);
+    else
+        hipLaunchKernelGGL((k_bcg_p<PT, RT>), dim3(ceil_div(n, rows), k), dim3(kBlock), 0,
+                           state().stream, n, P, ldp, d_pmap, R, ldr, d_rmap, d_beta, d_sold, d_snew,
+                           first, d_count, rows);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -2533,9 +2861,15 @@ static void bcg_xr_t(int n, XT *X, long ldx, const int *d_xmap, RT *R, long ldr,
     const int nbx = std::max(1, ceil_div(n, rows));
     double *partial = partial_buffer((size_t)nbx * k);
     ProfScope prof("bcg_xr");
-    hipLaunchKernelGGL((k_bcg_xr<PT, RT, XT>), dim3(nbx, k), dim3(kBlock), 0, state().stream, n, rows, X,
-                       ldx, d_xmap, R, ldr, d_rmap, P, ldp, d_pmap, W, ldw, d_wmap, d_rho2, d_ptw,
-                       d_scale, partial, d_count);
+    // rows is a multiple of kBlock, so of 4
+    if (quad_ok(X, ldx) && quad_ok(R, ldr) && quad_ok(P, ldp) && quad_ok(W, ldw))
+        hipLaunchKernelGGL((k_bcg_xr4<PT, RT, XT>), dim3(nbx, k), dim3(kBlock), 0, state().stream, n,
+                           rows, X, ldx, d_xmap, R, ldr, d_rmap, P, ldp, d_pmap, W, ldw, d_wmap, d_rho2,
+                           d_ptw, d_scale, partial, d_count);
+    else
+        hipLaunchKernelGGL((k_bcg_xr<PT, RT, XT>), dim3(nbx, k), dim3(kBlock), 0, state().stream, n,
+                           rows, X, ldx, d_xmap, R, ldr, d_rmap, P, ldp, d_pmap, W, ldw, d_wmap, d_rho2,
+                           d_ptw, d_scale, partial, d_count);
     if (rr_part) {
         *rr_part = partial;
         *rr_nbx = nbx;
