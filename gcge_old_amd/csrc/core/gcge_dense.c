@@ -50,122 +50,6 @@
 
 static inline int lsame(const char *c, char u) { return (*c == u) || (*c == (char)(u + 32)); }
 
-/* c = alpha op(a) op(b) + beta c, all column-major (Fortran dgemm contract:
- * with beta == 0, c is not read). */
-GCGE_VECTOR_CLONES
-void GCGE_Default_DenseMatDotDenseMat(char *transa, char *transb, GCGE_INT *nrows,
-                                      GCGE_INT *ncols, GCGE_INT *mid, GCGE_DOUBLE *alpha,
-                                      GCGE_DOUBLE *a, GCGE_INT *lda, GCGE_DOUBLE *b,
-                                      GCGE_INT *ldb, GCGE_DOUBLE *beta, GCGE_DOUBLE *c,
-                                      GCGE_INT *ldc)
-{
-    const int M = *nrows, N = *ncols, K = *mid;
-    const int ta = lsame(transa, 'T') || lsame(transa, 'C');
-    const int tb = lsame(transb, 'T') || lsame(transb, 'C');
-    const double al = *alpha, be = *beta;
-    const int LA = *lda, LB = *ldb, LC = *ldc;
-    if (M <= 0 || N <= 0) return;
-    for (int j = 0; j < N; ++j) {
-        double *cj = c + (size_t)j * LC;
-        if (be == 0.0) {
-            for (int i = 0; i < M; ++i) cj[i] = 0.0;
-        } else if (be != 1.0) {
-            for (int i = 0; i < M; ++i) cj[i] *= be;
-        }
-        if (al == 0.0 || K <= 0) continue;
-        if (!ta) {
-            /* column sweep: c(:,j) += alpha * a(:,l) * op(b)(l,j) */
-            for (int l = 0; l < K; ++l) {
-                double blj = tb ? b[j + (size_t)l * LB] : b[l + (size_t)j * LB];
-                double t = al * blj;
-                if (t == 0.0) continue;
-                const double *al_col = a + (size_t)l * LA;
-                for (int i = 0; i < M; ++i) cj[i] += t * al_col[i];
-            }
-        } else {
-            /* dot form: c(i,j) += alpha * a(:,i) . op(b)(:,j); four rows i at a
-             * time (independent sums, each over l in order) */
-            int i = 0;
-            if (!tb) {
-                const double *bj = b + (size_t)j * LB;
-                for (; i + 4 <= M; i += 4) {
-                    const double *a0 = a + (size_t)i * LA, *a1 = a0 + LA, *a2 = a1 + LA, *a3 = a2 + LA;
-                    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-                    for (int l = 0; l < K; ++l) {
-                        const double bl = bj[l];
-                        s0 += a0[l] * bl;
-                        s1 += a1[l] * bl;
-                        s2 += a2[l] * bl;
-                        s3 += a3[l] * bl;
-                    }
-                    cj[i] += al * s0;
-                    cj[i + 1] += al * s1;
-                    cj[i + 2] += al * s2;
-                    cj[i + 3] += al * s3;
-                }
-            }
-            for (; i < M; ++i) {
-                const double *ai = a + (size_t)i * LA;
-                double s = 0.0;
-                if (!tb) {
-                    const double *bj = b + (size_t)j * LB;
-                    for (int l = 0; l < K; ++l) s += ai[l] * bj[l];
-                } else {
-                    for (int l = 0; l < K; ++l) s += ai[l] * b[j + (size_t)l * LB];
-                }
-                cj[i] += al * s;
-            }
-        }
-    }
-}
-
-/* c = alpha a b + beta c (side 'L') or alpha b a + beta c (side 'R'), with
- * a symmetric and only its `uplo` triangle referenced.  The triangle is first
- * expanded to a full square in scratch memory; every c(i,j) is then the sum
- * over k in increasing order of a(i,k) b(k,j) (side 'L') or b(i,k) a(k,j)
- * (side 'R'), formed for a whole column of c at a time. */
-GCGE_VECTOR_CLONES
-void GCGE_Default_DenseSymMatDotDenseMat(char *side, char *uplo, GCGE_INT *nrows, GCGE_INT *ncols,
-                                         GCGE_DOUBLE *alpha, GCGE_DOUBLE *a, GCGE_INT *lda,
-                                         GCGE_DOUBLE *b, GCGE_INT *ldb, GCGE_DOUBLE *beta,
-                                         GCGE_DOUBLE *c, GCGE_INT *ldc)
-{
-    const int M = *nrows, N = *ncols, LA = *lda, LB = *ldb, LC = *ldc;
-    const int left = lsame(side, 'L'), upper = lsame(uplo, 'U');
-    const double al = *alpha, be = *beta;
-    if (M <= 0 || N <= 0) return;
-    const int S = left ? M : N; /* order of a */
-    double *full = (double *)malloc(((size_t)S * S + (size_t)M) * sizeof(double));
-    if (!full) {
-        fprintf(stderr, "GCGE_Default_DenseSymMatDotDenseMat: out of memory\n");
-        abort();
-    }
-    double *s = full + (size_t)S * S; /* one column of sums */
-    for (int k = 0; k < S; ++k)
-        for (int i = 0; i < S; ++i)
-            full[i + (size_t)k * S] = (upper ? (i <= k) : (i >= k)) ? a[i + (size_t)k * LA]
-                                                                  : a[k + (size_t)i * LA];
-    for (int j = 0; j < N; ++j) {
-        for (int i = 0; i < M; ++i) s[i] = 0.0;
-        if (left) {
-            for (int k = 0; k < M; ++k) {
-                const double bkj = b[k + (size_t)j * LB];
-                const double *ak = full + (size_t)k * S;
-                for (int i = 0; i < M; ++i) s[i] += ak[i] * bkj;
-            }
-        } else {
-            for (int k = 0; k < N; ++k) {
-                const double akj = full[k + (size_t)j * S];
-                const double *bk = b + (size_t)k * LB;
-                for (int i = 0; i < M; ++i) s[i] += bk[i] * akj;
-            }
-        }
-        double *cj = c + (size_t)j * LC;
-        for (int i = 0; i < M; ++i) cj[i] = (be == 0.0 ? 0.0 : be * cj[i]) + al * s[i];
-    }
-    free(full);
-}
-
 GCGE_DOUBLE GCGE_Default_ArrayDotArray(GCGE_DOUBLE *x, GCGE_DOUBLE *y, GCGE_INT length)
 {
     double s = 0.0;
@@ -418,6 +302,185 @@ static void barrier_on(atomic_int *count, atomic_int *flag, int T, int *sense)
 static void pool_barrier(int T, int *sense)
 {
     barrier_on(&g_pool.bar_count, &g_pool.bar_sense, T, sense);
+}
+
+/* ---- dense products ------------------------------------------------------ */
+/* c = alpha op(a) op(b) + beta c, all column-major (Fortran dgemm contract:
+ * with beta == 0, c is not read).  Columns j0 .. j1-1 of c. */
+GCGE_INLINE void gemm_cols(int ta, int tb, int M, int K, double al, const double *a, int LA,
+                           const double *b, int LB, double be, double *c, int LC, int j0, int j1)
+{
+    for (int j = j0; j < j1; ++j) {
+        double *cj = c + (size_t)j * LC;
+        if (be == 0.0) {
+            for (int i = 0; i < M; ++i) cj[i] = 0.0;
+        } else if (be != 1.0) {
+            for (int i = 0; i < M; ++i) cj[i] *= be;
+        }
+        if (al == 0.0 || K <= 0) continue;
+        if (!ta) {
+            /* column sweep: c(:,j) += alpha * a(:,l) * op(b)(l,j) */
+            for (int l = 0; l < K; ++l) {
+                double blj = tb ? b[j + (size_t)l * LB] : b[l + (size_t)j * LB];
+                double t = al * blj;
+                if (t == 0.0) continue;
+                const double *al_col = a + (size_t)l * LA;
+                for (int i = 0; i < M; ++i) cj[i] += t * al_col[i];
+            }
+        } else {
+            /* dot form: c(i,j) += alpha * a(:,i) . op(b)(:,j); four rows i at a
+             * time (independent sums, each over l in order) */
+            int i = 0;
+            if (!tb) {
+                const double *bj = b + (size_t)j * LB;
+                for (; i + 4 <= M; i += 4) {
+                    const double *a0 = a + (size_t)i * LA, *a1 = a0 + LA, *a2 = a1 + LA, *a3 = a2 + LA;
+                    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+                    for (int l = 0; l < K; ++l) {
+                        const double bl = bj[l];
+                        s0 += a0[l] * bl;
+                        s1 += a1[l] * bl;
+                        s2 += a2[l] * bl;
+                        s3 += a3[l] * bl;
+                    }
+                    cj[i] += al * s0;
+                    cj[i + 1] += al * s1;
+                    cj[i + 2] += al * s2;
+                    cj[i + 3] += al * s3;
+                }
+            }
+            for (; i < M; ++i) {
+                const double *ai = a + (size_t)i * LA;
+                double s = 0.0;
+                if (!tb) {
+                    const double *bj = b + (size_t)j * LB;
+                    for (int l = 0; l < K; ++l) s += ai[l] * bj[l];
+                } else {
+                    for (int l = 0; l < K; ++l) s += ai[l] * b[j + (size_t)l * LB];
+                }
+                cj[i] += al * s;
+            }
+        }
+    }
+}
+
+/* columns j0 .. j1-1 of the symmetric product: `full` is the expanded S x S
+ * matrix, s one column of M sums */
+GCGE_INLINE void symm_cols(int left, int M, int S, double al, const double *full, const double *b,
+                           int LB, double be, double *c, int LC, double *s, int j0, int j1)
+{
+    for (int j = j0; j < j1; ++j) {
+        for (int i = 0; i < M; ++i) s[i] = 0.0;
+        if (left) {
+            for (int k = 0; k < M; ++k) {
+                const double bkj = b[k + (size_t)j * LB];
+                const double *ak = full + (size_t)k * S;
+                for (int i = 0; i < M; ++i) s[i] += ak[i] * bkj;
+            }
+        } else {
+            for (int k = 0; k < S; ++k) {
+                const double akj = full[k + (size_t)j * S];
+                const double *bk = b + (size_t)k * LB;
+                for (int i = 0; i < M; ++i) s[i] += bk[i] * akj;
+            }
+        }
+        double *cj = c + (size_t)j * LC;
+        for (int i = 0; i < M; ++i) cj[i] = (be == 0.0 ? 0.0 : be * cj[i]) + al * s[i];
+    }
+}
+
+typedef struct {
+    int symm, ta, tb, left, M, N, K, S, LA, LB, LC, T;
+    double al, be;
+    const double *a, *b;
+    double *c, *s;
+} prod_job;
+
+GCGE_VECTOR_CLONES
+static void prod_job_run(void *arg, int t)
+{
+    const prod_job *P = (const prod_job *)arg;
+    const int j0 = (int)((long)P->N * t / P->T), j1 = (int)((long)P->N * (t + 1) / P->T);
+    if (P->symm)
+        symm_cols(P->left, P->M, P->S, P->al, P->a, P->b, P->LB, P->be, P->c, P->LC,
+                  P->s + (size_t)t * P->M, j0, j1);
+    else
+        gemm_cols(P->ta, P->tb, P->M, P->K, P->al, P->a, P->LA, P->b, P->LB, P->be, P->c, P->LC, j0,
+                  j1);
+}
+
+/* the product on the pool when it is large enough and the pool is free
+ * (then the pool is held until prod_run): the threads' count, else 1.  The
+ * columns of c are dealt out in contiguous ranges, each computed exactly as
+ * the single-threaded loop computes it. */
+static int prod_threads(long flops, int N)
+{
+    if (flops < env_int("GCGE_DENSE_MIN_FLOPS", 2000000) || N < 2) return 1;
+    if (pthread_mutex_trylock(&g_pool_busy) != 0) return 1;
+    const int T = pool_threads();
+    if (T <= 1) pthread_mutex_unlock(&g_pool_busy);
+    return T;
+}
+
+static void prod_run(prod_job *P)
+{
+    pool_run(prod_job_run, P);
+    pthread_mutex_unlock(&g_pool_busy);
+}
+
+GCGE_VECTOR_CLONES
+void GCGE_Default_DenseMatDotDenseMat(char *transa, char *transb, GCGE_INT *nrows,
+                                      GCGE_INT *ncols, GCGE_INT *mid, GCGE_DOUBLE *alpha,
+                                      GCGE_DOUBLE *a, GCGE_INT *lda, GCGE_DOUBLE *b,
+                                      GCGE_INT *ldb, GCGE_DOUBLE *beta, GCGE_DOUBLE *c,
+                                      GCGE_INT *ldc)
+{
+    const int M = *nrows, N = *ncols, K = *mid;
+    const int ta = lsame(transa, 'T') || lsame(transa, 'C');
+    const int tb = lsame(transb, 'T') || lsame(transb, 'C');
+    if (M <= 0 || N <= 0) return;
+    const int T = prod_threads(2L * M * N * (K > 0 ? K : 0), N);
+    if (T > 1) {
+        prod_job P = {0, ta, tb, 0, M, N, K, 0, *lda, *ldb, *ldc, T, *alpha, *beta, a, b, c, NULL};
+        prod_run(&P);
+        return;
+    }
+    gemm_cols(ta, tb, M, K, *alpha, a, *lda, b, *ldb, *beta, c, *ldc, 0, N);
+}
+
+/* c = alpha a b + beta c (side 'L') or alpha b a + beta c (side 'R'), with
+ * a symmetric and only its `uplo` triangle referenced.  The triangle is first
+ * expanded to a full square in scratch memory; every c(i,j) is then the sum
+ * over k in increasing order of a(i,k) b(k,j) (side 'L') or b(i,k) a(k,j)
+ * (side 'R'), formed for a whole column of c at a time. */
+GCGE_VECTOR_CLONES
+void GCGE_Default_DenseSymMatDotDenseMat(char *side, char *uplo, GCGE_INT *nrows, GCGE_INT *ncols,
+                                         GCGE_DOUBLE *alpha, GCGE_DOUBLE *a, GCGE_INT *lda,
+                                         GCGE_DOUBLE *b, GCGE_INT *ldb, GCGE_DOUBLE *beta,
+                                         GCGE_DOUBLE *c, GCGE_INT *ldc)
+{
+    const int M = *nrows, N = *ncols, LA = *lda;
+    const int left = lsame(side, 'L'), upper = lsame(uplo, 'U');
+    if (M <= 0 || N <= 0) return;
+    const int S = left ? M : N; /* order of a */
+    const int T = prod_threads(2L * M * N * S, N);
+    double *full = (double *)malloc(((size_t)S * S + (size_t)M * T) * sizeof(double));
+    if (!full) {
+        fprintf(stderr, "GCGE_Default_DenseSymMatDotDenseMat: out of memory\n");
+        abort();
+    }
+    double *s = full + (size_t)S * S; /* one column of sums per thread */
+    for (int k = 0; k < S; ++k)
+        for (int i = 0; i < S; ++i)
+            full[i + (size_t)k * S] = (upper ? (i <= k) : (i >= k)) ? a[i + (size_t)k * LA]
+                                                                  : a[k + (size_t)i * LA];
+    if (T > 1) {
+        prod_job P = {1, 0, 0, left, M, N, 0, S, LA, *ldb, *ldc, T, *alpha, *beta, full, b, c, s};
+        prod_run(&P);
+    } else {
+        symm_cols(left, M, S, *alpha, full, b, *ldb, *beta, c, *ldc, s, 0, N);
+    }
+    free(full);
 }
 
 /* ---- the solve ---------------------------------------------------------- */
