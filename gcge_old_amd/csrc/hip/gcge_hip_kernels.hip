@@ -555,7 +555,17 @@ struct alignas(sizeof(XT) * CK) TilePack {
     XT v[CK];
 };
 
-template <typename VT, typename XT, int PZ, int CK, int J, int DOT>
+// NF: the six face diagonals (offsets +-1, +-N, +-S) are absent (the
+// stiffness matrix of trilinear elements: its face couplings vanish for any
+// element coefficients); their coefficients take no registers and their
+// products are skipped (a zero coefficient adds an exact zero to a sum that
+// starts at +0, so the sums are unchanged).
+__host__ __device__ constexpr bool tile_face(int j)
+{
+    return (j / 9 != 1) + ((j / 3) % 3 != 1) + (j % 3 != 1) == 1;
+}
+
+template <typename VT, typename XT, int PZ, int CK, int J, int DOT, bool NF>
 __global__ __launch_bounds__(kTileRows, 2) void k_dia_tile(TileDesc td, const VT *__restrict__ dval,
                                                           long ldd, const XT *__restrict__ X, long ldx,
                                                           XT *__restrict__ Y, long ldy, int k,
@@ -589,6 +599,7 @@ __global__ __launch_bounds__(kTileRows, 2) void k_dia_tile(TileDesc td, const VT
         const long row = (z0 + p) * S + q;
 #pragma unroll
         for (int j = 0; j < 27; ++j) {
+            if (NF && tile_face(j)) continue;
             const int s = td.slot[j];
             v[p][j] = (s >= 0 && own[p]) ? (XT)dval[(long)s * ldd + row] : XT(0);
         }
@@ -654,6 +665,7 @@ __global__ __launch_bounds__(kTileRows, 2) void k_dia_tile(TileDesc td, const VT
                         const int p = pl - 1 - b;
                         if (p < 0 || p >= PZ) continue;
                         const int j = 9 * (b + 1) + 3 * (a + 1) + d + 1;
+                        if (NF && tile_face(j)) continue;
 #pragma unroll
                         for (int cc = 0; cc < CK; ++cc) acc[p][cc] += v[p][j] * xv[a + 1][d + 1].v[cc];
                     }
@@ -761,10 +773,20 @@ void launch_dia_tile(const Mat *A, const VT *vals, const XT *X, long ldx, XT *Y,
     const dim3 grid(td.tq * rounds * kXcds);
     const size_t lds = (size_t)(pz + 2) * W * 16;
     const long ldd = (long)A->nslices * kWave;
+    bool nf = true;
+    for (int j = 0; j < 27; ++j)
+        if (tile_face(j) && td.slot[j] >= 0) nf = false;
 #define GCGE_TILE_LAUNCH(PZ, JJ)                                                                    \
-    hipExtLaunchKernelGGL((k_dia_tile<VT, XT, PZ, CK, JJ, DOT>), grid, dim3(kTileRows), lds,         \
-                          state().stream, ev0, ev1, 0, td, vals, ldd, X, ldx, Y, ldy, k, d_xmap,    \
-                          d_ymap, dpart, d_count)
+    do {                                                                                            \
+        if (nf)                                                                                     \
+            hipExtLaunchKernelGGL((k_dia_tile<VT, XT, PZ, CK, JJ, DOT, true>), grid, dim3(kTileRows), \
+                                  lds, state().stream, ev0, ev1, 0, td, vals, ldd, X, ldx, Y, ldy, k, \
+                                  d_xmap, d_ymap, dpart, d_count);                                  \
+        else                                                                                        \
+            hipExtLaunchKernelGGL((k_dia_tile<VT, XT, PZ, CK, JJ, DOT, false>), grid,               \
+                                  dim3(kTileRows), lds, state().stream, ev0, ev1, 0, td, vals, ldd, X, \
+                                  ldx, Y, ldy, k, d_xmap, d_ymap, dpart, d_count);                  \
+    } while (0)
     if constexpr (sizeof(XT) == 4) {
         if (pz == 4) {
             if (J <= 2) GCGE_TILE_LAUNCH(4, 2);
