@@ -555,6 +555,19 @@ struct alignas(sizeof(XT) * CK) TilePack {
     XT v[CK];
 };
 
+// Workgroup barrier for LDS hand-offs only: the wave's LDS accesses are
+// complete (lgkmcnt 0) before s_barrier, and the empty asm statements keep the
+// compiler from moving memory accesses across it.  Unlike __syncthreads() it
+// does not wait for the wave's outstanding global loads and stores, so a
+// chunk's output stores and the next chunk's loads stay in flight.
+__device__ __forceinline__ void lds_barrier()
+{
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // vmcnt and expcnt unconstrained, lgkmcnt(0)
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
 // NF: the six face diagonals (offsets +-1, +-N, +-S) are absent (the
 // stiffness matrix of trilinear elements: its face couplings vanish for any
 // element coefficients); their coefficients take no registers and their
@@ -632,7 +645,7 @@ __global__ __launch_bounds__(kTileRows, 2) void k_dia_tile(TileDesc td, const VT
     };
     for (int c0 = 0; c0 < kc; c0 += CK) {
         fetch(c0);
-        __syncthreads();  // the previous chunk's products are done with the LDS
+        lds_barrier();  // the previous chunk's products are done with the LDS
 #pragma unroll
         for (int pl = 0; pl < PZ + 2; ++pl)
 #pragma unroll
@@ -640,7 +653,7 @@ __global__ __launch_bounds__(kTileRows, 2) void k_dia_tile(TileDesc td, const VT
                 const int o = t + kTileRows * jj;
                 if (o < W) lds[pl * W + o] = stage[pl][jj];
             }
-        __syncthreads();
+        lds_barrier();
         XT acc[PZ][CK];
 #pragma unroll
         for (int p = 0; p < PZ; ++p)
@@ -688,13 +701,17 @@ __global__ __launch_bounds__(kTileRows, 2) void k_dia_tile(TileDesc td, const VT
         }
         if constexpr (DOT > 0) {
             const int part = (zg * td.tq + qt) * (kTileRows / kWave) + t / kWave;
+            // this thread's own staged values, one 16-byte read per plane
+            Pack own_x[PZ];
+#pragma unroll
+            for (int p = 0; p < PZ; ++p) own_x[p] = lds[(p + 1) * W + base];
 #pragma unroll
             for (int cc = 0; cc < CK; ++cc) {
                 if (cc >= cn) continue;
                 double pw = 0.0, pp = 0.0;
 #pragma unroll
                 for (int p = 0; p < PZ; ++p) {
-                    const double pv = own[p] ? (double)lds[(p + 1) * W + base].v[cc] : 0.0;
+                    const double pv = own[p] ? (double)own_x[p].v[cc] : 0.0;
                     pw += pv * (own[p] ? (double)acc[p][cc] : 0.0);
                     if constexpr (DOT > 1) pp += pv * pv;
                 }
