@@ -173,7 +173,7 @@ static inline void spin_step(unsigned *k)
 }
 
 /* ---- worker pool -------------------------------------------------------- */
-enum { EIG_MAX_THREADS = 64, EIG_SPINS = 20000 };
+enum { EIG_MAX_THREADS = 64 };
 
 typedef struct {
     pthread_mutex_t mu;
@@ -184,6 +184,7 @@ typedef struct {
     atomic_int wbar_count, wbar_sense; /* the workers (threads 1 ..) */
     int nthreads;     /* including the calling thread; 0 = not started */
     unsigned gen0;    /* generation when the workers were started */
+    unsigned spins;   /* spin steps of an idle worker before it sleeps */
     void (*fn)(void *, int);
     void *arg;
     /* sweep log of the QL phase, kept between calls */
@@ -204,7 +205,7 @@ static void *pool_worker(void *arg)
     for (;;) {
         unsigned g, spins = 0;
         while ((g = atomic_load_explicit(&g_pool.gen, memory_order_acquire)) == seen) {
-            if (spins < EIG_SPINS) {
+            if (spins < g_pool.spins) {
                 spin_step(&spins);
                 continue;
             }
@@ -255,6 +256,7 @@ static int pool_threads(void)
         g_pool_atfork = 1;
     }
     g_pool.gen0 = atomic_load(&g_pool.gen);
+    g_pool.spins = (unsigned)env_int("GCGE_DENSE_SPIN", 20000);
     int started = 1;
     for (int t = 1; t < want; ++t) {
         pthread_t th;

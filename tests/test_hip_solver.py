@@ -151,10 +151,11 @@ def test_full_rr_on_device():
 
 
 def test_large_rayleigh_ritz_on_device_eigensolver():
-    """nev = 100 makes the projected matrix larger than 128, so the
-    Rayleigh-Ritz eigensolve runs on rocSOLVER through the backend's dense
-    eigensolver extension; the spectrum must still be the closed form."""
-    E, nev = 12, 100
+    """nev = 150 makes the projected matrix (nev + 2 x block) larger than 400,
+    so the Rayleigh-Ritz eigensolve runs on rocSOLVER through the backend's
+    dense eigensolver extension (smaller orders run on the host threads); the
+    spectrum must still be the closed form."""
+    E, nev = 12, 150
     K, M = problems.q1_fem_3d(E)
     out = gcge.solve(K, M, nev, ["-gcge_conv_type", "RB", "-gcge_ev_tol", "1e-8",
                                  "-gcge_ev_max_it", "200", "-gcge_full_rr", "1"], seed=7)
