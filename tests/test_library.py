@@ -3,7 +3,9 @@
 import ctypes as C
 import os
 import re
+import sys
 
+import numpy as np
 import pytest
 
 from gcge_old_amd import _native
@@ -128,3 +130,17 @@ def test_bench_refuses_more_than_one_gpu():
     assert p.returncode == 2
     assert "one GPU" in p.stderr and "--gpus 2" in p.stderr
     assert p.stdout == ""
+
+
+def test_bench_cluster_cut_for_subspace_angles():
+    """bench.py compares the two precisions' eigenvector subspaces only below
+    the last eigenvalue cluster (a cluster that may continue beyond nev has
+    no unique basis): the kappa = 1 cube's 50 smallest eigenvalues end in a
+    pair, so the cut is at 48; a simple spectrum loses its last value only."""
+    sys.path.insert(0, ROOT)
+    import bench
+    from gcge_old_amd import problems
+
+    assert bench.leading_complete_clusters(problems.q1_eigs(64, 50)) == 48
+    assert bench.leading_complete_clusters(np.array([1.0, 2.0, 3.0, 4.0])) == 3
+    assert bench.leading_complete_clusters(np.array([1.0, 2.0, 3.0, 3.0, 3.0 + 1e-9])) == 2
