@@ -708,7 +708,8 @@ def solve_workload(args):
                                f"||r||/(|lambda| ||Mx||) <= {args.tol:g}",
                    "precision": value_prec,
                    "rayleigh_ritz": "reference soft locking" if args.reference_locking else "full (-gcge_full_rr 1)",
-                   "w_orth": "multi (reference)" if args.reference_orth else "svqb (-gcge_w_orth_type svqb)",
+                   "w_orth": "multi (reference)" if args.reference_orth else
+                             "svqb (-gcge_w_orth_type svqb; one pass below Gram condition 1e4, else two)",
                    "n": n, "nnz_K": dK.nnz, "nnz_M": dM.nnz, "nev": nev,
                    "block_size": last["block_size"], "assembly": args.assembly,
                    "parallelism": "1 GPU"},

@@ -200,13 +200,15 @@ void GCGE_StableMultiOrthonormalizationPartly(void **V, GCGE_INT start, GCGE_INT
         GCGE_MultiOrthonormalization(V, start, end, B, ops, para, ws);
 }
 
-/* B-orthonormalise columns start .. *end-1 among themselves by two SVQB
+/* B-orthonormalise columns start .. *end-1 among themselves by SVQB
  * passes: G = W^T B W, D = diag(G)^(-1/2) (columns with sqrt(G_jj) <=
  * orth_zero_tol are zero columns, as in the recursive leaves, and dropped),
  * D G D = U diag(l) U^T, W <- W D U diag(l)^(-1/2) over the directions with
  * l > 1e-14 max(l), largest first.  One Gram and one linear combination per
  * pass instead of one Gram per bisection level and leaf; the span of the
- * surviving columns is the same as with the recursive algorithm. */
+ * surviving columns is the same as with the recursive algorithm.  A second
+ * pass runs unless the first kept every column with a scaled-Gram condition
+ * number below para->svqb_one_pass_cond. */
 void GCGE_SVQBOrthonormalization(void **V, GCGE_INT start, GCGE_INT *end, void *B, GCGE_OPS *ops,
                                  GCGE_PARA *para, GCGE_WORKSPACE *ws)
 {
@@ -274,8 +276,14 @@ void GCGE_SVQBOrthonormalization(void **V, GCGE_INT start, GCGE_INT *end, void *
         }
         if (r < k && para->orth_para->print_orth_zero == 1)
             GCGE_Printf("In GCGE_SVQBOrthonormalization, %d of %d columns dropped\n", k - r, k);
+        /* one pass is enough when it kept every column and the scaled Gram
+         * is well conditioned: the B-orthonormality error of an SVQB pass is
+         * about u * lambda_max / lambda_min */
+        const int kept_all = (r == k && r == m);
+        const GCGE_DOUBLE cond = r > 0 ? lam[m - 1] / lam[m - r] : 0.0;
         *end = start + r;
         k = r;
+        if (pass == 0 && kept_all && cond < para->svqb_one_pass_cond) break;
     }
     free(G);
     free(S);

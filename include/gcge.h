@@ -184,6 +184,11 @@ typedef struct GCGE_PARA_ {
     GCGE_DOUBLE *history;
     GCGE_INT history_cap;
     GCGE_INT history_len;
+    /* added, w_orth_type "svqb" only: the second SVQB pass is skipped when the
+     * first kept every column and the scaled Gram's condition number
+     * lambda_max / lambda_min is below this (B-orthonormality error of one
+     * pass ~ u times that number); 0 always runs two passes */
+    GCGE_DOUBLE svqb_one_pass_cond;
 } GCGE_PARA;
 
 /* ---- workspace (reference src/gcge_workspace.h:23-61) ---- */

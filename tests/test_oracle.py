@@ -469,7 +469,7 @@ def test_live_reference_soft_locking_stall():
 
 
 def test_svqb_w_orth_same_spectrum_and_iteration_count():
-    """-gcge_w_orth_type svqb (two SVQB passes inside W after the same
+    """-gcge_w_orth_type svqb (SVQB passes inside W after the same
     projection against [X, P]) spans the same space as the recursive "multi"
     orthonormalisation, so the iteration converges to the same pairs in about
     as many steps; the eigenvectors come out B-orthonormal."""
@@ -485,3 +485,30 @@ def test_svqb_w_orth_same_spectrum_and_iteration_count():
     np.testing.assert_allclose(np.sort(svqb["eval"]), problems.q1_eigs(E, nev), rtol=1e-10)
     X = svqb["evec"]
     assert np.abs(X.T @ (M.to_scipy() @ X) - np.eye(nev)).max() < 1e-10
+
+
+def test_svqb_one_pass_matches_two_passes():
+    """-gcge_svqb_one_pass_cond (default 1e4) skips the second SVQB pass when
+    the first kept every column of a well-conditioned block; 0 always runs
+    two.  Both reach the closed-form spectrum in about as many iterations
+    with B-orthonormal eigenvectors, and a threshold of 1e30 (one pass
+    whenever no column is dropped) still gives B-orthonormal vectors on this
+    pencil."""
+    E, nev = 14, 30
+    K, M = problems.q1_fem_3d(E)
+    Ms = M.to_scipy()
+    x0 = oracle.random_fill(5, 0, 0, nev, K.nrows)
+    base = ["-gcge_conv_type", "RB", "-gcge_ev_tol", "1e-8", "-gcge_ev_max_it", "100",
+            "-gcge_full_rr", "1", "-gcge_w_orth_type", "svqb"]
+    runs = {c: oracle.solve(K, M, nev, base + ["-gcge_svqb_one_pass_cond", c], x0=x0)
+            for c in ("0", "1e4", "1e30")}
+    default = oracle.solve(K, M, nev, base, x0=x0)
+    np.testing.assert_array_equal(default["eval"], runs["1e4"]["eval"])
+    assert default["num_iter"] == runs["1e4"]["num_iter"]
+    want = problems.q1_eigs(E, nev)
+    for c, out in runs.items():
+        assert out["num_conv"] == nev, c
+        assert abs(out["num_iter"] - runs["0"]["num_iter"]) <= 3, c
+        np.testing.assert_allclose(np.sort(out["eval"]), want, rtol=1e-10)
+        X = out["evec"]
+        assert np.abs(X.T @ (Ms @ X) - np.eye(nev)).max() < 1e-10, c
