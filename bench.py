@@ -26,9 +26,11 @@ Rayleigh-Ritz step with its computed coupling.  The reference's soft locking
 zeroes that coupling, which at this tolerance stalls some starts short of nev
 pairs (the reference build does the same); --reference-locking restores it.
 The W block is B-orthonormalised with -gcge_w_orth_type svqb (the reference's
-projection against [X, P], then two SVQB passes instead of the recursive
-single-column leaves: 5 host round trips instead of about 40, same span);
---reference-orth restores "multi".
+projection against [X, P], then one or two SVQB passes instead of the
+recursive single-column leaves: a few host round trips instead of about 40,
+same span), and the projection runs two passes (-gcge_max_reorth_time 2;
+the reference's 3rd pass removes components of relative size 1e-16 and
+below); --reference-orth restores "multi" and three passes.
 
 The JSON line carries:
   value          mean seconds per solve over the timed steps (lower is better)
@@ -116,6 +118,10 @@ def parse():
     p.add_argument("--reference-orth", action="store_true",
                    help="recursive \"multi\" W orthonormalisation as in the reference "
                         "(default here: -gcge_w_orth_type svqb)")
+    p.add_argument("--max-reorth-time", type=int, default=None,
+                   help="-gcge_max_reorth_time: projection passes of W against [X, P] "
+                        "(default 2 with the svqb W orthonormalisation, the reference's 3 "
+                        "with --reference-orth)")
     p.add_argument("--no-profile", action="store_true",
                    help="time without per-kernel events (no roofline numbers)")
     p.add_argument("--no-breakdown", action="store_true",
@@ -523,6 +529,11 @@ def solve_workload(args):
         flags += ["-gcge_full_rr", "1"]
     if not args.reference_orth:
         flags += ["-gcge_w_orth_type", "svqb"]
+    reorth = args.max_reorth_time
+    if reorth is None and not args.reference_orth:
+        reorth = 2
+    if reorth is not None:
+        flags += ["-gcge_max_reorth_time", str(reorth)]
     print(f"# workload: Q1 {E}^3 {'lognormal kappa ' if lognormal else ''}n={n} nnz(K)={dK.nnz} "
           f"nnz(M)={dM.nnz} nev={nev} ({args.assembly} assembly {setup_s:.2f}s, storage "
           f"{dK.storage_info()})", file=sys.stderr, flush=True)
@@ -710,6 +721,7 @@ def solve_workload(args):
                    "rayleigh_ritz": "reference soft locking" if args.reference_locking else "full (-gcge_full_rr 1)",
                    "w_orth": "multi (reference)" if args.reference_orth else
                              "svqb (-gcge_w_orth_type svqb; one pass below Gram condition 1e4, else two)",
+                   "projection_passes": reorth if reorth is not None else 3,
                    "n": n, "nnz_K": dK.nnz, "nnz_M": dM.nnz, "nev": nev,
                    "block_size": last["block_size"], "assembly": args.assembly,
                    "parallelism": "1 GPU"},
@@ -757,7 +769,8 @@ def adjoint_workload(args):
     # tolerance 1e-9: the lognormal pencil does not reach 1e-11 in 800
     # iterations
     flags = ["-gcge_conv_type", "RB", "-gcge_ev_tol", str(min(args.tol, 1e-9)),
-             "-gcge_ev_max_it", "800", "-gcge_full_rr", "1", "-gcge_w_orth_type", "svqb"]
+             "-gcge_ev_max_it", "800", "-gcge_full_rr", "1", "-gcge_w_orth_type", "svqb",
+             "-gcge_max_reorth_time", "2"]
     dM = gcge.DeviceMatrix.q1(E, None, "mass")
 
     def products(dA, X):
@@ -916,7 +929,7 @@ def permuted_workload(args):
     def flags_for(max_it):
         return ["-gcge_conv_type", "RB", "-gcge_ev_tol", str(args.tol), "-gcge_ev_max_it",
                 str(max_it), "-gcge_block_size", str(2 * nev), "-gcge_full_rr", "1",
-                "-gcge_w_orth_type", "svqb"]
+                "-gcge_w_orth_type", "svqb", "-gcge_max_reorth_time", "2"]
 
     # seconds per iteration as the difference of a 1-iteration and a
     # (1 + iters)-iteration run from the same start: the set-up (the initial
@@ -974,7 +987,8 @@ def permuted_workload(args):
                        "reorder": args.reorder, "reorder_s": t_rcm,
                        "assembly": args.assembly, "setup_s": round(t_prep, 3),
                        "n": n, "nnz_K": dK.nnz, "nnz_M": dM.nnz, "nev": nev,
-                       "rayleigh_ritz": "full (-gcge_full_rr 1)", "w_orth": "svqb"},
+                       "rayleigh_ritz": "full (-gcge_full_rr 1)", "w_orth": "svqb",
+                       "projection_passes": 2},
             "fp64": legs["fp64"], "mixed": legs["mixed"],
             "mixed_speedup_per_iteration": round(fp64 / mixed, 3),
             "eigenvalue_adjoint_all_elements_ms": round(t_adj * 1e3, 2),

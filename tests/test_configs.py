@@ -34,7 +34,7 @@ from gcge_old_amd import DeviceMatrix, DeviceMultiVec, _native, adjoint, problem
 pytestmark = pytest.mark.gpu
 
 BENCH_FLAGS = ["-gcge_conv_type", "RB", "-gcge_ev_tol", "1e-8", "-gcge_full_rr", "1",
-               "-gcge_w_orth_type", "svqb"]
+               "-gcge_w_orth_type", "svqb", "-gcge_max_reorth_time", "2"]
 
 
 def _products(dA, X):
@@ -139,7 +139,7 @@ def test_c4_q1_64_lognormal_adjoint():
     kappa = problems.lognormal_kappa(E, 1)
     w = np.random.default_rng(2).uniform(size=nev)
     flags = ["-gcge_conv_type", "RB", "-gcge_ev_tol", "1e-9", "-gcge_ev_max_it", "800",
-             "-gcge_full_rr", "1", "-gcge_w_orth_type", "svqb"]
+             "-gcge_full_rr", "1", "-gcge_w_orth_type", "svqb", "-gcge_max_reorth_time", "2"]
 
     def solve(kv):
         dK = DeviceMatrix.q1(E, kv, "stiffness")
@@ -225,7 +225,8 @@ def test_c5_q1_192_permuted_products_solve_adjoint():
     # a bounded mixed-precision solve: the block comes back M-orthonormal
     print("C5 bounded mixed-precision solve", flush=True)
     flags = ["-gcge_conv_type", "RB", "-gcge_ev_tol", "1e-8", "-gcge_ev_max_it", "1",
-             "-gcge_block_size", str(2 * nev), "-gcge_full_rr", "1", "-gcge_w_orth_type", "svqb"]
+             "-gcge_block_size", str(2 * nev), "-gcge_full_rr", "1", "-gcge_w_orth_type", "svqb",
+             "-gcge_max_reorth_time", "2"]
     out = gcge.solve(dK, dM, nev, flags, seed=1, precision="mixed")
     dK.set_precision(False)
     dM.set_precision(False)

@@ -512,3 +512,30 @@ def test_svqb_one_pass_matches_two_passes():
         np.testing.assert_allclose(np.sort(out["eval"]), want, rtol=1e-10)
         X = out["evec"]
         assert np.abs(X.T @ (Ms @ X) - np.eye(nev)).max() < 1e-10, c
+
+
+def test_two_projection_passes_with_svqb():
+    """-gcge_max_reorth_time 2 (the benchmark's setting with svqb): the
+    projection of W against [X, P] stops after two classical Gram-Schmidt
+    passes, where the reference default runs a third whose coefficients are
+    of relative size 1e-16 and below.  Same closed-form spectrum, about as
+    many iterations, B-orthonormal eigenvectors; on a lognormal pencil the
+    eigenvalues agree with the three-pass solve to 1e-12."""
+    E, nev = 14, 30
+    K, M = problems.q1_fem_3d(E)
+    x0 = oracle.random_fill(4, 0, 0, nev, K.nrows)
+    base = ["-gcge_conv_type", "RB", "-gcge_ev_tol", "1e-8", "-gcge_ev_max_it", "100",
+            "-gcge_full_rr", "1", "-gcge_w_orth_type", "svqb"]
+    three = oracle.solve(K, M, nev, base, x0=x0)
+    two = oracle.solve(K, M, nev, base + ["-gcge_max_reorth_time", "2"], x0=x0)
+    assert three["num_conv"] == nev and two["num_conv"] == nev
+    assert abs(three["num_iter"] - two["num_iter"]) <= 2
+    np.testing.assert_allclose(np.sort(two["eval"]), problems.q1_eigs(E, nev), rtol=1e-10)
+    X = two["evec"]
+    assert np.abs(X.T @ (M.to_scipy() @ X) - np.eye(nev)).max() < 1e-10
+    Kl, Ml = problems.q1_fem_3d(12, problems.lognormal_kappa(12, 3))
+    x0 = oracle.random_fill(6, 0, 0, 20, Kl.nrows)
+    a = oracle.solve(Kl, Ml, 20, base, x0=x0)
+    b = oracle.solve(Kl, Ml, 20, base + ["-gcge_max_reorth_time", "2"], x0=x0)
+    assert a["num_conv"] == 20 and b["num_conv"] == 20
+    np.testing.assert_allclose(b["eval"], a["eval"], rtol=1e-12)
