@@ -178,6 +178,33 @@ static GCGE_DOUBLE frobenius(const GCGE_DOUBLE *d, GCGE_INT len)
     return sqrt(s);
 }
 
+/* One projection pass of the svqb path: GCGE_SubOrthonormalization with the
+ * subtraction folded into the linear combination, V2 <- V2 + V1 (-d) with
+ * beta = 1 (one pass over V2 instead of a temporary block plus an axpby).
+ * subspace_dtmp holds -d afterwards (the same Frobenius norm).  A backend
+ * that forms the combination first and adds it once gives the bits of the
+ * two-step form; the default column loops accumulate into V2 term by term
+ * and round differently. */
+static void sub_orth_fused(void **V, GCGE_INT *start, GCGE_INT *end, void *B, void **V_tmp,
+                           GCGE_DOUBLE *subspace_dtmp, GCGE_OPS *ops)
+{
+    const GCGE_INT len1 = end[0] - start[0], len2 = end[1] - start[1];
+    GCGE_INT mv_s[2], mv_e[2];
+    mv_s[0] = start[1]; mv_e[0] = end[1];
+    mv_s[1] = 0; mv_e[1] = len2;
+    if (B == NULL)
+        ops->MultiVecAxpby(1.0, V, 0.0, V_tmp, mv_s, mv_e, ops);
+    else
+        ops->MatDotMultiVec(B, V, V_tmp, mv_s, mv_e, ops);
+    mv_s[0] = start[0]; mv_e[0] = end[0];
+    mv_s[1] = 0; mv_e[1] = len2;
+    ops->MultiVecInnerProd(V, V_tmp, subspace_dtmp, "ns", mv_s, mv_e, len1, 0, ops);
+    for (GCGE_INT i = 0; i < len1 * len2; ++i) subspace_dtmp[i] = -subspace_dtmp[i];
+    mv_s[0] = start[0]; mv_e[0] = end[0];
+    mv_s[1] = start[1]; mv_e[1] = end[1];
+    ops->MultiVecLinearComb(V, V, mv_s, mv_e, subspace_dtmp, len1, 0, 1.0, 1.0, ops);
+}
+
 void GCGE_StableMultiOrthonormalizationPartly(void **V, GCGE_INT start, GCGE_INT *end, void *B,
                                               GCGE_OPS *ops, GCGE_PARA *para,
                                               GCGE_WORKSPACE *ws)
@@ -187,9 +214,13 @@ void GCGE_StableMultiOrthonormalizationPartly(void **V, GCGE_INT start, GCGE_INT
         GCGE_DOUBLE value_inner = 1.0;
         GCGE_INT iter = 0;
         const GCGE_DOUBLE tol = para->orth_para->scbgs_reorth_tol;
+        const int fused = strcmp(para->w_orth_type, "svqb") == 0;
         while (value_inner > tol) {
             iter++;
-            GCGE_SubOrthonormalization(V, mv_s, mv_e, B, ws->V_tmp, ws->subspace_dtmp, ops);
+            if (fused)
+                sub_orth_fused(V, mv_s, mv_e, B, ws->V_tmp, ws->subspace_dtmp, ops);
+            else
+                GCGE_SubOrthonormalization(V, mv_s, mv_e, B, ws->V_tmp, ws->subspace_dtmp, ops);
             value_inner = frobenius(ws->subspace_dtmp, start * (*end - start));
             if (iter >= para->orth_para->max_reorth_time) break;
         }

@@ -199,6 +199,28 @@ def test_linear_comb_matches_oracle(m, k, beta):
     np.testing.assert_array_equal(dY.get(0, 1), Y0[:, :1])  # column outside the range untouched
 
 
+@pytest.mark.parametrize("m,k", [(5, 7), (90, 9), (3, 17), (21, 70), (300, 100), (64, 33)])
+def test_fused_projection_subtraction_same_bits(m, k):
+    """The projection pass of the svqb path folds W -= V d into one linear
+    combination (coefficients -d, beta = 1, V and W columns of one block).
+    Every combination kernel forms V(-d) = -(V d) exactly and adds it to W in
+    one rounding, so W comes out bit for bit as from the two-step form (V d
+    into a temporary, then W = -1 tmp + 1 W)."""
+    n = 3001
+    rng = _rng(31 * m + k)
+    A = rng.standard_normal((n, m + k))
+    d = np.asfortranarray(rng.standard_normal((m, k)))
+    nd = np.asfortranarray(-d)
+    dA = DeviceMultiVec.from_host(A)
+    _native.hip().GCGE_HIP_LinearComb(dA.handle, 0, m, _native.as_dptr(nd), m, 1.0, dA.handle, m, k)
+    fused = dA.get(m, k)
+    dB, dT = DeviceMultiVec.from_host(A), DeviceMultiVec(n, k)
+    _native.hip().GCGE_HIP_LinearComb(dB.handle, 0, m, _native.as_dptr(d), m, 0.0, dT.handle, 0, k)
+    _native.hip().GCGE_HIP_Axpby(-1.0, dT.handle, 0, 1.0, dB.handle, m, k)
+    np.testing.assert_array_equal(fused, dB.get(m, k))
+    np.testing.assert_array_equal(dA.get(0, m), A[:, :m])
+
+
 @pytest.mark.parametrize("a,b", [(0.0, 2.0), (1.0, 0.0), (-3.0, 0.0), (0.5, -2.0), (0.0, 0.0)])
 def test_axpby_special_cases(a, b):
     n, k = 777, 4
