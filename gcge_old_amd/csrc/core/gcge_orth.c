@@ -205,10 +205,14 @@ static void sub_orth_fused(void **V, GCGE_INT *start, GCGE_INT *end, void *B, vo
     ops->MultiVecLinearComb(V, V, mv_s, mv_e, subspace_dtmp, len1, 0, 1.0, 1.0, ops);
 }
 
+static void svqb_orth(void **V, GCGE_INT start, GCGE_INT *end, void *B, GCGE_OPS *ops,
+                      GCGE_PARA *para, GCGE_WORKSPACE *ws, int bw_ready);
+
 void GCGE_StableMultiOrthonormalizationPartly(void **V, GCGE_INT start, GCGE_INT *end, void *B,
                                               GCGE_OPS *ops, GCGE_PARA *para,
                                               GCGE_WORKSPACE *ws)
 {
+    int bw_ready = 0;
     if (start > 0) {
         GCGE_INT mv_s[2] = {0, start}, mv_e[2] = {start, *end};
         GCGE_DOUBLE value_inner = 1.0;
@@ -224,9 +228,15 @@ void GCGE_StableMultiOrthonormalizationPartly(void **V, GCGE_INT start, GCGE_INT
             value_inner = frobenius(ws->subspace_dtmp, start * (*end - start));
             if (iter >= para->orth_para->max_reorth_time) break;
         }
+        /* after a second pass (or a first one below the tolerance) V_tmp holds
+         * B W' for the W' before the last correction W = W' - V d, and the
+         * SVQB Gram W^T (B W') = W^T B W + (W^T B V) d differs from W^T B W
+         * only by the product of two rounding-level factors (W is B-orthogonal
+         * to V up to rounding, d at most the size of W) */
+        bw_ready = fused && (iter >= 2 || value_inner <= tol);
     }
     if (strcmp(para->w_orth_type, "svqb") == 0)
-        GCGE_SVQBOrthonormalization(V, start, end, B, ops, para, ws);
+        svqb_orth(V, start, end, B, ops, para, ws, bw_ready);
     else
         GCGE_MultiOrthonormalization(V, start, end, B, ops, para, ws);
 }
@@ -239,9 +249,16 @@ void GCGE_StableMultiOrthonormalizationPartly(void **V, GCGE_INT start, GCGE_INT
  * pass instead of one Gram per bisection level and leaf; the span of the
  * surviving columns is the same as with the recursive algorithm.  A second
  * pass runs unless the first kept every column with a scaled-Gram condition
- * number below para->svqb_one_pass_cond. */
+ * number below para->svqb_one_pass_cond.  bw_ready: V_tmp(:, 0:k) already
+ * holds B W for the first pass (from the projection). */
 void GCGE_SVQBOrthonormalization(void **V, GCGE_INT start, GCGE_INT *end, void *B, GCGE_OPS *ops,
                                  GCGE_PARA *para, GCGE_WORKSPACE *ws)
+{
+    svqb_orth(V, start, end, B, ops, para, ws, 0);
+}
+
+static void svqb_orth(void **V, GCGE_INT start, GCGE_INT *end, void *B, GCGE_OPS *ops,
+                      GCGE_PARA *para, GCGE_WORKSPACE *ws, int bw_ready)
 {
     const GCGE_DOUBLE zero_tol = para->orth_para->orth_zero_tol;
     GCGE_INT k = *end - start;
@@ -259,10 +276,12 @@ void GCGE_SVQBOrthonormalization(void **V, GCGE_INT start, GCGE_INT *end, void *
         /* V_tmp = B W; G = W^T V_tmp */
         mv_s[0] = start; mv_e[0] = *end;
         mv_s[1] = 0; mv_e[1] = k;
-        if (B == NULL)
-            ops->MultiVecAxpby(1.0, V, 0.0, ws->V_tmp, mv_s, mv_e, ops);
-        else
-            ops->MatDotMultiVec(B, V, ws->V_tmp, mv_s, mv_e, ops);
+        if (pass > 0 || !bw_ready) {
+            if (B == NULL)
+                ops->MultiVecAxpby(1.0, V, 0.0, ws->V_tmp, mv_s, mv_e, ops);
+            else
+                ops->MatDotMultiVec(B, V, ws->V_tmp, mv_s, mv_e, ops);
+        }
         ops->MultiVecInnerProd(V, ws->V_tmp, G, "ns", mv_s, mv_e, k, 0, ops);
         GCGE_INT m = 0;
         for (GCGE_INT j = 0; j < k; ++j) {
