@@ -28,9 +28,13 @@ pairs (the reference build does the same); --reference-locking restores it.
 The W block is B-orthonormalised with -gcge_w_orth_type svqb (the reference's
 projection against [X, P], then one or two SVQB passes instead of the
 recursive single-column leaves: a few host round trips instead of about 40,
-same span), and the projection runs two passes (-gcge_max_reorth_time 2;
-the reference's 3rd pass removes components of relative size 1e-16 and
-below); --reference-orth restores "multi" and three passes.
+same span); W leaves the inner solve as the correction to its starting
+vector, and the projection makes a second pass only where a column kept
+less than 1/sqrt(2) of its B-norm in the first (-gcge_svqb_dgks_eta, the
+Daniel-Gragg-Kaufman-Stewart criterion), at most two passes
+(-gcge_max_reorth_time 2; the reference's 3rd pass removes components of
+relative size 1e-16 and below); --reference-orth restores "multi", W as the
+reference forms it and three passes.
 
 The JSON line carries:
   value          mean seconds per solve over the timed steps (lower is better)
@@ -721,7 +725,10 @@ def solve_workload(args):
                    "rayleigh_ritz": "reference soft locking" if args.reference_locking else "full (-gcge_full_rr 1)",
                    "w_orth": "multi (reference)" if args.reference_orth else
                              "svqb (-gcge_w_orth_type svqb; one pass below Gram condition 1e4, else two)",
-                   "projection_passes": reorth if reorth is not None else 3,
+                   "projection_passes": ("1, a 2nd where a column kept < 1/sqrt(2) of its B-norm "
+                                         "(W as a correction, -gcge_svqb_dgks_eta)"
+                                         if not args.reference_orth and reorth == 2 else
+                                         reorth if reorth is not None else 3),
                    "n": n, "nnz_K": dK.nnz, "nnz_M": dM.nnz, "nev": nev,
                    "block_size": last["block_size"], "assembly": args.assembly,
                    "parallelism": "1 GPU"},
@@ -988,7 +995,7 @@ def permuted_workload(args):
                        "assembly": args.assembly, "setup_s": round(t_prep, 3),
                        "n": n, "nnz_K": dK.nnz, "nnz_M": dM.nnz, "nev": nev,
                        "rayleigh_ritz": "full (-gcge_full_rr 1)", "w_orth": "svqb",
-                       "projection_passes": 2},
+                       "projection_passes": "1, a 2nd where a column kept < 1/sqrt(2) of its B-norm"},
             "fp64": legs["fp64"], "mixed": legs["mixed"],
             "mixed_speedup_per_iteration": round(fp64 / mixed, 3),
             "eigenvalue_adjoint_all_elements_ms": round(t_adj * 1e3, 2),

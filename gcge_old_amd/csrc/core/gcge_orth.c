@@ -205,8 +205,9 @@ static void sub_orth_fused(void **V, GCGE_INT *start, GCGE_INT *end, void *B, vo
     ops->MultiVecLinearComb(V, V, mv_s, mv_e, subspace_dtmp, len1, 0, 1.0, 1.0, ops);
 }
 
-static void svqb_orth(void **V, GCGE_INT start, GCGE_INT *end, void *B, GCGE_OPS *ops,
-                      GCGE_PARA *para, GCGE_WORKSPACE *ws, int bw_ready);
+static int svqb_orth(void **V, GCGE_INT start, GCGE_INT *end, void *B, GCGE_OPS *ops,
+                     GCGE_PARA *para, GCGE_WORKSPACE *ws, int bw_ready, const GCGE_DOUBLE *dgks_d,
+                     GCGE_INT dgks_ld);
 
 void GCGE_StableMultiOrthonormalizationPartly(void **V, GCGE_INT start, GCGE_INT *end, void *B,
                                               GCGE_OPS *ops, GCGE_PARA *para,
@@ -219,14 +220,24 @@ void GCGE_StableMultiOrthonormalizationPartly(void **V, GCGE_INT start, GCGE_INT
         GCGE_INT iter = 0;
         const GCGE_DOUBLE tol = para->orth_para->scbgs_reorth_tol;
         const int fused = strcmp(para->w_orth_type, "svqb") == 0;
-        while (value_inner > tol) {
+        if (fused && para->svqb_dgks_eta > 0.0 && para->orth_para->max_reorth_time > 1) {
+            /* one pass, then the SVQB on its B-product: it runs to the end
+             * when every column kept at least eta of its B-norm in the pass
+             * (the norms come from the SVQB Gram and the pass's
+             * coefficients), and otherwise returns before changing W, and
+             * the remaining passes run as below */
+            iter = 1;
+            sub_orth_fused(V, mv_s, mv_e, B, ws->V_tmp, ws->subspace_dtmp, ops);
+            value_inner = frobenius(ws->subspace_dtmp, start * (*end - start));
+            if (svqb_orth(V, start, end, B, ops, para, ws, 1, ws->subspace_dtmp, start) == 0) return;
+        }
+        while (value_inner > tol && iter < para->orth_para->max_reorth_time) {
             iter++;
             if (fused)
                 sub_orth_fused(V, mv_s, mv_e, B, ws->V_tmp, ws->subspace_dtmp, ops);
             else
                 GCGE_SubOrthonormalization(V, mv_s, mv_e, B, ws->V_tmp, ws->subspace_dtmp, ops);
             value_inner = frobenius(ws->subspace_dtmp, start * (*end - start));
-            if (iter >= para->orth_para->max_reorth_time) break;
         }
         /* after a second pass (or a first one below the tolerance) V_tmp holds
          * B W' for the W' before the last correction W = W' - V d, and the
@@ -236,7 +247,7 @@ void GCGE_StableMultiOrthonormalizationPartly(void **V, GCGE_INT start, GCGE_INT
         bw_ready = fused && (iter >= 2 || value_inner <= tol);
     }
     if (strcmp(para->w_orth_type, "svqb") == 0)
-        svqb_orth(V, start, end, B, ops, para, ws, bw_ready);
+        svqb_orth(V, start, end, B, ops, para, ws, bw_ready, NULL, 0);
     else
         GCGE_MultiOrthonormalization(V, start, end, B, ops, para, ws);
 }
@@ -254,15 +265,22 @@ void GCGE_StableMultiOrthonormalizationPartly(void **V, GCGE_INT start, GCGE_INT
 void GCGE_SVQBOrthonormalization(void **V, GCGE_INT start, GCGE_INT *end, void *B, GCGE_OPS *ops,
                                  GCGE_PARA *para, GCGE_WORKSPACE *ws)
 {
-    svqb_orth(V, start, end, B, ops, para, ws, 0);
+    svqb_orth(V, start, end, B, ops, para, ws, 0, NULL, 0);
 }
 
-static void svqb_orth(void **V, GCGE_INT start, GCGE_INT *end, void *B, GCGE_OPS *ops,
-                      GCGE_PARA *para, GCGE_WORKSPACE *ws, int bw_ready)
+/* dgks_d (leading dimension dgks_ld): the coefficients d of the projection
+ * pass just made, W = W' - V d.  Column j then kept the fraction
+ * sqrt(G_jj / (G_jj + |d_j|^2)) of its B-norm; below svqb_dgks_eta for any
+ * column the call returns 1 before W is changed (another projection pass
+ * is due), else 0. */
+static int svqb_orth(void **V, GCGE_INT start, GCGE_INT *end, void *B, GCGE_OPS *ops,
+                     GCGE_PARA *para, GCGE_WORKSPACE *ws, int bw_ready, const GCGE_DOUBLE *dgks_d,
+                     GCGE_INT dgks_ld)
 {
     const GCGE_DOUBLE zero_tol = para->orth_para->orth_zero_tol;
     GCGE_INT k = *end - start;
-    if (k <= 0) return;
+    int rc = 0;
+    if (k <= 0) return 0;
     GCGE_DOUBLE *G = (GCGE_DOUBLE *)malloc((size_t)k * k * sizeof(GCGE_DOUBLE));
     GCGE_DOUBLE *S = (GCGE_DOUBLE *)malloc((size_t)k * k * sizeof(GCGE_DOUBLE));
     GCGE_DOUBLE *U = (GCGE_DOUBLE *)malloc((size_t)k * k * sizeof(GCGE_DOUBLE));
@@ -283,6 +301,17 @@ static void svqb_orth(void **V, GCGE_INT start, GCGE_INT *end, void *B, GCGE_OPS
                 ops->MatDotMultiVec(B, V, ws->V_tmp, mv_s, mv_e, ops);
         }
         ops->MultiVecInnerProd(V, ws->V_tmp, G, "ns", mv_s, mv_e, k, 0, ops);
+        if (pass == 0 && dgks_d) {
+            const GCGE_DOUBLE eta2 = para->svqb_dgks_eta * para->svqb_dgks_eta;
+            for (GCGE_INT j = 0; j < k && !rc; ++j) {
+                const GCGE_DOUBLE *dj = dgks_d + (size_t)j * dgks_ld;
+                GCGE_DOUBLE dn = 0.0;
+                for (GCGE_INT i = 0; i < dgks_ld; ++i) dn += dj[i] * dj[i];
+                const GCGE_DOUBLE gjj = G[(size_t)j * k + j];
+                if (!(gjj >= eta2 * (gjj + dn))) rc = 1;
+            }
+            if (rc) break;  /* W unchanged: another projection pass first */
+        }
         GCGE_INT m = 0;
         for (GCGE_INT j = 0; j < k; ++j) {
             const GCGE_DOUBLE gjj = G[(size_t)j * k + j];
@@ -343,6 +372,7 @@ static void svqb_orth(void **V, GCGE_INT start, GCGE_INT *end, void *B, GCGE_OPS
     free(dsc);
     free(work);
     free(keep);
+    return rc;
 }
 
 void GCGE_SubOrthonormalization(void **V, GCGE_INT *start, GCGE_INT *end, void *B,

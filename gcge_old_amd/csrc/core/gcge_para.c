@@ -116,6 +116,7 @@ void GCGE_PARA_Create(GCGE_PARA **para)
     p->use_ops_ext = 1;
     p->full_rr = 0;
     p->svqb_one_pass_cond = 1e4;
+    p->svqb_dgks_eta = 0.7071067811865476;
     p->bcg_by_column = 0;
     p->history = NULL;
     p->history_cap = 0;
@@ -194,6 +195,7 @@ static const flag_row flag_table[] = {
     PF("-gcge_full_rr", F_INT, full_rr),
     PF("-gcge_bcg_by_column", F_INT, bcg_by_column),
     PF("-gcge_svqb_one_pass_cond", F_DBL, svqb_one_pass_cond),
+    PF("-gcge_svqb_dgks_eta", F_DBL, svqb_dgks_eta),
     /* a GCGE_PARA field the reference sets only in code (example/csr_solver.c:51) */
     PF("-gcge_dirichlet_boundary", F_INT, dirichlet_boundary),
 };
@@ -216,6 +218,8 @@ static void print_usage(const char *prog)
     GCGE_Printf("  -gcge_full_rr <i>           keep converged vectors in the Rayleigh-Ritz step (default 0)\n");
     GCGE_Printf("  -gcge_bcg_by_column <i>     block CG scalars kept per column (default 0)\n");
     GCGE_Printf("  -gcge_svqb_one_pass_cond <d> one SVQB pass below this Gram condition (default 1e4; 0: two)\n");
+    GCGE_Printf("  -gcge_svqb_dgks_eta <d>    svqb: W as a correction, second projection pass below this\n"
+                "                             kept B-norm fraction (default 0.7071; 0: reference W, all passes)\n");
     GCGE_Printf("  -gcge_dirichlet_boundary <i> zero the Dirichlet rows of the initial vectors (default 0)\n");
     GCGE_Printf("  (all other reference -gcge_* flags are accepted with the same meaning)\n\n");
 }
@@ -387,5 +391,6 @@ void GCGE_PrintParaInfo(GCGE_PARA *para)
     GCGE_Printf("  use_ops_ext                : %8d\n", para->use_ops_ext);
     GCGE_Printf("  full_rr                    : %8d\n", para->full_rr);
     GCGE_Printf("  bcg_by_column              : %8d\n", para->bcg_by_column);
-    GCGE_Printf("  svqb_one_pass_cond         : %3.2e\n\n", para->svqb_one_pass_cond);
+    GCGE_Printf("  svqb_one_pass_cond         : %3.2e\n", para->svqb_one_pass_cond);
+    GCGE_Printf("  svqb_dgks_eta              : %3.2e\n\n", para->svqb_dgks_eta);
 }

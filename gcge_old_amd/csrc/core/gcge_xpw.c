@@ -31,8 +31,53 @@ void GCGE_ComputeX(void **V, GCGE_DOUBLE *subspace_evec, void **X, GCGE_INT size
                             1.0, 0.0, ops);
 }
 
+static void compute_w_solve(void *A, void *B, void **V, GCGE_DOUBLE *eval, GCGE_OPS *ops,
+                            GCGE_PARA *para, GCGE_WORKSPACE *ws);
+
+/* With the svqb W orthonormalisation and svqb_dgks_eta > 0, W leaves as the
+ * correction to its starting vector: w - a x, a = 1 where |lambda| <= 1 and
+ * 1 / (lambda + shift) elsewhere (the starting vector set below).  x is a
+ * column of V, so [X, P, W] spans the same space and the projected W is the
+ * same as with the reference's W; without the dominant x component (W is
+ * close to a x once the pair converges) the projection loses no digits to
+ * cancellation, which is what lets a single projection pass suffice. */
 void GCGE_ComputeW(void *A, void *B, void **V, GCGE_DOUBLE *eval, GCGE_OPS *ops,
                    GCGE_PARA *para, GCGE_WORKSPACE *ws)
+{
+    compute_w_solve(A, B, V, eval, ops, para, ws);
+    const GCGE_INT w_start = ws->dim_xp;
+    const GCGE_INT w_length = ws->unconv_bs;
+    if (!(para->svqb_dgks_eta > 0.0) || strcmp(para->w_orth_type, "svqb") != 0 || w_length <= 0)
+        return;
+    GCGE_OPS_EXT *ext = GCGE_OPS_GetExt(ops);
+    GCGE_DOUBLE *a = (GCGE_DOUBLE *)malloc(2 * (size_t)w_length * sizeof(GCGE_DOUBLE));
+    GCGE_INT *cols = (GCGE_INT *)malloc(2 * (size_t)w_length * sizeof(GCGE_INT));
+    GCGE_DOUBLE *b = a + w_length;
+    for (GCGE_INT idx = 0; idx < w_length; ++idx) {
+        const GCGE_INT xi = ws->unlock[idx];
+        a[idx] = -(fabs(eval[xi]) <= 1.0 ? 1.0 : 1.0 / (eval[xi] + para->shift));
+        b[idx] = 1.0;
+        cols[idx] = xi;
+        cols[w_length + idx] = w_start + idx;
+    }
+    if (para->use_ops_ext && ext && ext->MultiVecAxpbyCols) {
+        ext->MultiVecAxpbyCols(a, V, cols, b, V, cols + w_length, w_length, ops);
+    } else {
+        for (GCGE_INT idx = 0; idx < w_length; ++idx) {
+            void *xc, *wc;
+            ops->GetVecFromMultiVec(V, cols[idx], &xc, ops);
+            ops->GetVecFromMultiVec(V, cols[w_length + idx], &wc, ops);
+            ops->VecAxpby(a[idx], xc, 1.0, wc, ops);
+            ops->RestoreVecForMultiVec(V, cols[idx], &xc, ops);
+            ops->RestoreVecForMultiVec(V, cols[w_length + idx], &wc, ops);
+        }
+    }
+    free(a);
+    free(cols);
+}
+
+static void compute_w_solve(void *A, void *B, void **V, GCGE_DOUBLE *eval, GCGE_OPS *ops,
+                            GCGE_PARA *para, GCGE_WORKSPACE *ws)
 {
     const GCGE_INT w_start = ws->dim_xp;
     const GCGE_INT w_length = ws->unconv_bs;

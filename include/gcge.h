@@ -189,6 +189,13 @@ typedef struct GCGE_PARA_ {
      * lambda_max / lambda_min is below this (B-orthonormality error of one
      * pass ~ u times that number); 0 always runs two passes */
     GCGE_DOUBLE svqb_one_pass_cond;
+    /* added, w_orth_type "svqb" only: eta > 0 returns W as the correction to
+     * its starting vector (w - x or w - x / lambda, the same span once W is
+     * projected against [X, P]) and runs a second projection pass only when
+     * a column keeps less than eta of its B-norm in the first (Daniel,
+     * Gragg, Kaufman, Stewart 1976); 0 forms W as the reference does and
+     * runs up to max_reorth_time passes */
+    GCGE_DOUBLE svqb_dgks_eta;
 } GCGE_PARA;
 
 /* ---- workspace (reference src/gcge_workspace.h:23-61) ---- */

@@ -520,12 +520,13 @@ def test_two_projection_passes_with_svqb():
     passes, where the reference default runs a third whose coefficients are
     of relative size 1e-16 and below.  Same closed-form spectrum, about as
     many iterations, B-orthonormal eigenvectors; on a lognormal pencil the
-    eigenvalues agree with the three-pass solve to 1e-12."""
+    eigenvalues agree with the three-pass solve to 1e-12.  (W as the
+    reference forms it: -gcge_svqb_dgks_eta 0.)"""
     E, nev = 14, 30
     K, M = problems.q1_fem_3d(E)
     x0 = oracle.random_fill(4, 0, 0, nev, K.nrows)
     base = ["-gcge_conv_type", "RB", "-gcge_ev_tol", "1e-8", "-gcge_ev_max_it", "100",
-            "-gcge_full_rr", "1", "-gcge_w_orth_type", "svqb"]
+            "-gcge_full_rr", "1", "-gcge_w_orth_type", "svqb", "-gcge_svqb_dgks_eta", "0"]
     three = oracle.solve(K, M, nev, base, x0=x0)
     two = oracle.solve(K, M, nev, base + ["-gcge_max_reorth_time", "2"], x0=x0)
     assert three["num_conv"] == nev and two["num_conv"] == nev
@@ -539,3 +540,36 @@ def test_two_projection_passes_with_svqb():
     b = oracle.solve(Kl, Ml, 20, base + ["-gcge_max_reorth_time", "2"], x0=x0)
     assert a["num_conv"] == 20 and b["num_conv"] == 20
     np.testing.assert_allclose(b["eval"], a["eval"], rtol=1e-12)
+
+
+def test_svqb_w_as_correction_with_one_projection_pass():
+    """-gcge_svqb_dgks_eta (default 1/sqrt 2, svqb only): W leaves the inner
+    solve as the correction to its starting vector and the projection makes
+    a second pass only for columns that kept less than eta of their B-norm
+    in the first.  Against eta = 0 (W as the reference forms it, every
+    pass): the same closed-form spectrum in about as many iterations, and
+    on a lognormal pencil eigenvalues within 1e-12 and B-orthonormal
+    eigenvectors; eta = 1 forces the second pass on every column."""
+    E, nev = 14, 30
+    K, M = problems.q1_fem_3d(E)
+    x0 = oracle.random_fill(4, 0, 0, nev, K.nrows)
+    base = ["-gcge_conv_type", "RB", "-gcge_ev_tol", "1e-8", "-gcge_ev_max_it", "100",
+            "-gcge_full_rr", "1", "-gcge_w_orth_type", "svqb", "-gcge_max_reorth_time", "2"]
+    runs = {eta: oracle.solve(K, M, nev, base + ["-gcge_svqb_dgks_eta", eta], x0=x0)
+            for eta in ("0", "0.7071067811865476", "1")}
+    default = oracle.solve(K, M, nev, base, x0=x0)
+    np.testing.assert_array_equal(default["eval"], runs["0.7071067811865476"]["eval"])
+    for eta, out in runs.items():
+        assert out["num_conv"] == nev, eta
+        assert abs(out["num_iter"] - runs["0"]["num_iter"]) <= 2, eta
+        np.testing.assert_allclose(np.sort(out["eval"]), problems.q1_eigs(E, nev), rtol=1e-10)
+        X = out["evec"]
+        assert np.abs(X.T @ (M.to_scipy() @ X) - np.eye(nev)).max() < 1e-10, eta
+    Kl, Ml = problems.q1_fem_3d(12, problems.lognormal_kappa(12, 3))
+    x0 = oracle.random_fill(6, 0, 0, 20, Kl.nrows)
+    a = oracle.solve(Kl, Ml, 20, base + ["-gcge_svqb_dgks_eta", "0"], x0=x0)
+    b = oracle.solve(Kl, Ml, 20, base, x0=x0)
+    assert a["num_conv"] == 20 and b["num_conv"] == 20
+    np.testing.assert_allclose(b["eval"], a["eval"], rtol=1e-12)
+    X = b["evec"]
+    assert np.abs(X.T @ (Ml.to_scipy() @ X) - np.eye(20)).max() < 1e-10
