@@ -88,7 +88,7 @@ def build_examples() -> str:
 
 def build_oracle(with_reference: bool = True) -> None:
     """The checker: the CPU restatement, and the reference built from its own
-    sources when the reference tree is present (this container only)."""
+    sources where the reference tree is present."""
     odir = os.path.join(ROOT, "oracle")
     _run(["make", "-s", "-C", odir])
     if with_reference and os.path.isdir("/root/reference/src"):

@@ -44,7 +44,7 @@ static GCGE_DOUBLE tick(GCGE_OPS *ops)
 
 static void unsupported(const char *what, const char *name)
 {
-    fprintf(stderr, "gcge: %s \"%s\" is not available in this build\n", what, name);
+    fprintf(stderr, "gcge: %s \"%s\" is not implemented in this library\n", what, name);
     exit(1);
 }
 

@@ -170,7 +170,7 @@ static void compute_w_solve(void *A, void *B, void **V, GCGE_DOUBLE *eval, GCGE_
     if (para->if_use_bcg == 1) {
         if (para->if_shift == 1) para->use_bcg_continuous = 0;
         if (para->use_bcg_continuous) {
-            fprintf(stderr, "gcge: use_bcg_continuous is not available in this build\n");
+            fprintf(stderr, "gcge: use_bcg_continuous is not implemented in this library\n");
             exit(1);
         }
         GCGE_OPS_EXT *ext = GCGE_OPS_GetExt(ops);
@@ -229,7 +229,7 @@ void GCGE_ComputeP(GCGE_DOUBLE *subspace_evec, void **V, GCGE_OPS *ops, GCGE_PAR
         GCGE_StableMultiOrthonormalizationInSubspace(sub, ldc, orth_length, p_start, &p_end, NULL,
                                                      -1, para->orth_para, ws, ops);
     } else if (strcmp(para->p_orth_type, "scbgs") == 0 || strcmp(para->p_orth_type, "bgs") == 0) {
-        fprintf(stderr, "gcge: p_orth_type \"%s\" is not available in this build\n",
+        fprintf(stderr, "gcge: p_orth_type \"%s\" is not implemented in this library\n",
                 para->p_orth_type);
         exit(1);
     } else {

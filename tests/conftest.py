@@ -9,7 +9,7 @@ if ROOT not in sys.path:
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs an MI355X (run with -m gpu on the GPU box)")
+    config.addinivalue_line("markers", "gpu: needs an MI355X (run with -m gpu on a machine that has one)")
     config.addinivalue_line("markers", "slow: longer CPU test")
 
 

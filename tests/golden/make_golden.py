@@ -1,5 +1,5 @@
-"""Writes the committed fixtures under tests/golden/ (run in the container
-that holds the reference tree; the GPU box only reads the results).
+"""Writes the committed fixtures under tests/golden/ (run where the
+reference tree is present; the GPU tests only read the results).
 
 1. data/*.npz -- the CSR matrices the reference ships as test inputs
    (test/data/testA, testB, A_3.txt, M_3.txt, A_5.txt, M_5.txt), converted
