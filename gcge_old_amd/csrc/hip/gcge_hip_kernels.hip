@@ -1440,12 +1440,14 @@ __global__ __launch_bounds__(512) void k_gram_lds(int n, int rows_per_block,
     for (int rs = r0; rs < r1; rs += kGlStage) {
         const bool more = rs + kGlStage < r1;
         if (more) load(rs + kGlStage);
+        // NJ is the number of 16-column tiles of W (the launch picks the
+        // instance), so the products of a row tile run without branches
 #pragma unroll
         for (int kk = 0; kk < kGlStage / 4; ++kk) {
             const int qa = 4 * kk + lq;
             double bw[NJ];
 #pragma unroll
-            for (int j = 0; j < NJ; ++j) bw[j] = j < tj ? lds[b][qa][MP + 16 * j + li] : 0.0;
+            for (int j = 0; j < NJ; ++j) bw[j] = lds[b][qa][MP + 16 * j + li];
 #pragma unroll
             for (int i = 0; i < MI; ++i) {
                 const int it = wid + 8 * i;
@@ -1453,7 +1455,7 @@ __global__ __launch_bounds__(512) void k_gram_lds(int n, int rows_per_block,
                     const double a = lds[b][qa][16 * it + li];
 #pragma unroll
                     for (int j = 0; j < NJ; ++j)
-                        if (j < tj) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bw[j], acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bw[j], acc[i][j], 0, 0, 0);
                 }
             }
         }
@@ -1500,18 +1502,27 @@ static bool launch_gram_lds(int nbx, int n, int rows, const double *V, long ldv,
 #define GCGE_GL(MI, NJ)                                                                                  \
     hipLaunchKernelGGL((k_gram_lds<MI, NJ>), dim3(nbx), dim3(512), 0, st, n, rows, V, ldv, m, W, ldw, k, \
                        partial)
-    if (tj <= 7) {
-        if (mi == 1) GCGE_GL(1, 7);
-        else if (mi == 2) GCGE_GL(2, 7);
-        else if (mi == 3) GCGE_GL(3, 7);
-        else return false;
-    } else if (tj == 8) {
+#define GCGE_GL_MI(NJ)                       \
+    do {                                     \
+        if (mi == 1) GCGE_GL(1, NJ);         \
+        else if (mi == 2) GCGE_GL(2, NJ);    \
+        else if (mi == 3) GCGE_GL(3, NJ);    \
+        else return false;                   \
+    } while (0)
+    // one instance per W tile count (k from 49 up): no masked products
+    switch (tj) {
+    case 4: GCGE_GL_MI(4); break;
+    case 5: GCGE_GL_MI(5); break;
+    case 6: GCGE_GL_MI(6); break;
+    case 7: GCGE_GL_MI(7); break;
+    case 8:
         if (mi == 1) GCGE_GL(1, 8);
         else if (mi == 2) GCGE_GL(2, 8);
         else return false;
-    } else {
-        return false;
+        break;
+    default: return false;
     }
+#undef GCGE_GL_MI
 #undef GCGE_GL
     return true;
 }
@@ -2616,7 +2627,7 @@ int gram_impl(int n, const double *V, long ldv, int m, const double *W, long ldw
     }();
     const int gti = ceil_div(m, 16), gtj = ceil_div(k, 16);
     if (gram_mode == 0 && gram_lds_min > 0 && m >= gram_lds_min && k >= gram_lds_min &&
-        ((gtj <= 7 && gti <= 24) || (gtj == 8 && gti <= 16))) {
+        ((gtj >= 4 && gtj <= 7 && gti <= 24) || (gtj == 8 && gti <= 16))) {
         // one workgroup per CU (eight waves, up to 127 KiB of LDS), the whole
         // m x k block each, about 500 stages of 16 rows per workgroup at 128^3
         nbx = gram_ranges(n, std::min(ceil_div(n, 4 * kGlStage), num_cus()), mk, partial_cap, kGlStage,
