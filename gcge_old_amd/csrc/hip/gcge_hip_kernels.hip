@@ -1509,8 +1509,11 @@ static bool launch_gram_lds(int nbx, int n, int rows, const double *V, long ldv,
         else if (mi == 3) GCGE_GL(3, NJ);    \
         else return false;                   \
     } while (0)
-    // one instance per W tile count (k from 49 up): no masked products
+    // one instance per W tile count: no masked products
     switch (tj) {
+    case 1: GCGE_GL_MI(1); break;
+    case 2: GCGE_GL_MI(2); break;
+    case 3: GCGE_GL_MI(3); break;
     case 4: GCGE_GL_MI(4); break;
     case 5: GCGE_GL_MI(5); break;
     case 6: GCGE_GL_MI(6); break;
@@ -2619,15 +2622,19 @@ int gram_impl(int n, const double *V, long ldv, int m, const double *W, long ldw
         const char *e = getenv("GCGE_HIP_GRAM");  // "fma": the vector-FMA kernels (measurement)
         return (e && strcmp(e, "fma") == 0) ? 1 : 0;
     }();
-    // smallest m and k for the whole-block kernel (GCGE_HIP_GRAM_LDS; 0: the
-    // k_gram_mfma blocks only, for measurement)
+    // smallest m for the whole-block kernel, any k (GCGE_HIP_GRAM_LDS; 0: the
+    // k_gram_mfma blocks only, for measurement).  Measured at n = 2 048 383
+    // (wall time of the call): 300 x 48 1.81 against 2.81 ms, 300 x 16 1.89
+    // against 2.03, 200 x 8 0.87 against 1.39, 100 x 24 0.65 against 0.88;
+    // at n = 250 047, 60 x 10 took 68 against 57 us, so small m stay on the
+    // other kernels
     static const int gram_lds_min = [] {
         const char *e = getenv("GCGE_HIP_GRAM_LDS");
         return e ? atoi(e) : 64;
     }();
     const int gti = ceil_div(m, 16), gtj = ceil_div(k, 16);
-    if (gram_mode == 0 && gram_lds_min > 0 && m >= gram_lds_min && k >= gram_lds_min &&
-        ((gtj >= 4 && gtj <= 7 && gti <= 24) || (gtj == 8 && gti <= 16))) {
+    if (gram_mode == 0 && gram_lds_min > 0 && m >= gram_lds_min &&
+        ((gtj <= 7 && gti <= 24) || (gtj == 8 && gti <= 16))) {
         // one workgroup per CU (eight waves, up to 127 KiB of LDS), the whole
         // m x k block each, about 500 stages of 16 rows per workgroup at 128^3
         nbx = gram_ranges(n, std::min(ceil_div(n, 4 * kGlStage), num_cus()), mk, partial_cap, kGlStage,

@@ -153,12 +153,14 @@ def test_spmm_storages_agree(name):
                                    (2000, 100, 100), (1500, 256, 200), (999, 96, 96),
                                    (20000, 300, 100), (5000, 64, 128), (3001, 250, 128),
                                    (17, 384, 112), (100003, 200, 100), (7001, 150, 80),
-                                   (7001, 330, 90), (4099, 70, 64), (2051, 300, 65)])
+                                   (7001, 330, 90), (4099, 70, 64), (2051, 300, 65),
+                                   (7001, 300, 48), (5003, 200, 8), (3001, 100, 24),
+                                   (4097, 65, 1), (2049, 384, 17)])
 def test_gram_matches_oracle(n, m, k):
     """Small register blocks (m, k <= 8), column blocks, matrix-core 128 x 128
-    blocks and the whole-block LDS-staged kernel (64 <= m <= 384, 64 <= k <=
-    128: one to three row tiles per wave, an instance for each count of 4 to
-    8 column tiles, fewer rows than one stage, partial tiles)."""
+    blocks and the whole-block LDS-staged kernel (64 <= m <= 384, k <= 128:
+    one to three row tiles per wave, an instance for each count of 1 to 8
+    column tiles, fewer rows than one stage, partial tiles)."""
     rng = _rng(n + m + k)
     V = rng.standard_normal((n, m + 1))
     W = rng.standard_normal((n, k + 2))
