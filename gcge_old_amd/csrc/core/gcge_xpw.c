@@ -250,9 +250,18 @@ void GCGE_ComputeP(GCGE_DOUBLE *subspace_evec, void **V, GCGE_OPS *ops, GCGE_PAR
     mv_s[1] = 0; mv_e[1] = p_end_tmp;
     ops->MultiVecLinearComb(V, V_tmp, mv_s, mv_e, coef + (size_t)nev * ldc + orth_start, ldc, 0,
                             1.0, 0.0, ops);
-    mv_s[0] = nev; mv_e[0] = p_end;
-    mv_s[1] = 0; mv_e[1] = p_end_tmp;
-    ops->MultiVecSwap(V, V_tmp, mv_s, mv_e, ops);
+    if (strcmp(para->w_orth_type, "svqb") == 0) {
+        /* V_tmp is written before it is read again (the next right-hand
+         * sides, projections and SVQB products), so a copy does what the
+         * swap does for V with half the traffic on a device backend */
+        mv_s[0] = 0; mv_e[0] = p_end_tmp;
+        mv_s[1] = nev; mv_e[1] = p_end;
+        ops->MultiVecAxpby(1.0, V_tmp, 0.0, V, mv_s, mv_e, ops);
+    } else {
+        mv_s[0] = nev; mv_e[0] = p_end;
+        mv_s[1] = 0; mv_e[1] = p_end_tmp;
+        ops->MultiVecSwap(V, V_tmp, mv_s, mv_e, ops);
+    }
     ws->dim_xp = p_end;
 
     GCGE_OPS_EXT *ext = GCGE_OPS_GetExt(ops);
