@@ -46,6 +46,25 @@ void GCGE_ComputeSubspaceMatrixVTAW(void **V, void *A, GCGE_INT start_W, GCGE_IN
     ops->MultiVecInnerProd(V, work, subspace_mat + xp_length, "sym", mv_s, mv_e, end_V, 0, ops);
 }
 
+/* The full projection (full_rr, every row from 0): one inner-product block
+ * [X, P, W]^T (A W) instead of [X, P]^T (A W) and the "sym" W^T (A W) (one
+ * Gram launch and one transfer fewer, A W read once); the W^T A W part then
+ * takes its lower triangle from the upper one, as the "sym" block does. */
+static void vtaw_one_block(void **V, void *A, GCGE_INT start_W, GCGE_INT end_V,
+                           GCGE_DOUBLE *subspace_mat, GCGE_OPS *ops, void **work)
+{
+    const GCGE_INT w_length = end_V - start_W;
+    GCGE_INT mv_s[2] = {start_W, 0}, mv_e[2] = {end_V, w_length};
+    ops->MatDotMultiVec(A, V, work, mv_s, mv_e, ops);
+    mv_s[0] = 0; mv_e[0] = end_V;
+    mv_s[1] = 0; mv_e[1] = w_length;
+    ops->MultiVecInnerProd(V, work, subspace_mat, "nonsym", mv_s, mv_e, end_V, 0, ops);
+    for (GCGE_INT j = 0; j < w_length; ++j)
+        for (GCGE_INT i = j + 1; i < w_length; ++i)
+            subspace_mat[(size_t)j * end_V + start_W + i] =
+                subspace_mat[(size_t)i * end_V + start_W + j];
+}
+
 void GCGE_ComputeSubspaceMatrix(void *A, void **V, GCGE_OPS *ops, GCGE_PARA *para,
                                 GCGE_WORKSPACE *ws)
 {
@@ -69,8 +88,11 @@ void GCGE_ComputeSubspaceMatrix(void *A, void **V, GCGE_OPS *ops, GCGE_PARA *par
     }
     para->opt_rr_eig_partly = 1;
     const GCGE_INT start_V = para->full_rr ? 0 : ws->num_soft_locked_in_last_iter;
-    GCGE_ComputeSubspaceMatrixVTAW(V, A, dim_xp, start_V, ldm, H + (size_t)dim_xp * ldm, ops,
-                                   ws->evec);
+    if (para->full_rr && dim_xp > 0)
+        vtaw_one_block(V, A, dim_xp, ldm, H + (size_t)dim_xp * ldm, ops, ws->evec);
+    else
+        GCGE_ComputeSubspaceMatrixVTAW(V, A, dim_xp, start_V, ldm, H + (size_t)dim_xp * ldm, ops,
+                                       ws->evec);
     if (para->if_shift == 1)
         for (GCGE_INT i = 0; i < ldm; ++i) H[(size_t)i * ldm + i] += para->shift;
 }
