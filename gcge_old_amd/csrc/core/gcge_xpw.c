@@ -31,8 +31,8 @@ void GCGE_ComputeX(void **V, GCGE_DOUBLE *subspace_evec, void **X, GCGE_INT size
                             1.0, 0.0, ops);
 }
 
-static void compute_w_solve(void *A, void *B, void **V, GCGE_DOUBLE *eval, GCGE_OPS *ops,
-                            GCGE_PARA *para, GCGE_WORKSPACE *ws);
+static int compute_w_solve(void *A, void *B, void **V, GCGE_DOUBLE *eval, GCGE_OPS *ops,
+                           GCGE_PARA *para, GCGE_WORKSPACE *ws, int want_correction);
 
 /* With the svqb W orthonormalisation and svqb_dgks_eta > 0, W leaves as the
  * correction to its starting vector: w - a x, a = 1 where |lambda| <= 1 and
@@ -44,11 +44,11 @@ static void compute_w_solve(void *A, void *B, void **V, GCGE_DOUBLE *eval, GCGE_
 void GCGE_ComputeW(void *A, void *B, void **V, GCGE_DOUBLE *eval, GCGE_OPS *ops,
                    GCGE_PARA *para, GCGE_WORKSPACE *ws)
 {
-    compute_w_solve(A, B, V, eval, ops, para, ws);
+    const int want = para->svqb_dgks_eta > 0.0 && strcmp(para->w_orth_type, "svqb") == 0;
+    const int corrected = compute_w_solve(A, B, V, eval, ops, para, ws, want);
     const GCGE_INT w_start = ws->dim_xp;
     const GCGE_INT w_length = ws->unconv_bs;
-    if (!(para->svqb_dgks_eta > 0.0) || strcmp(para->w_orth_type, "svqb") != 0 || w_length <= 0)
-        return;
+    if (!want || corrected || w_length <= 0) return;
     GCGE_OPS_EXT *ext = GCGE_OPS_GetExt(ops);
     GCGE_DOUBLE *a = (GCGE_DOUBLE *)malloc(2 * (size_t)w_length * sizeof(GCGE_DOUBLE));
     GCGE_INT *cols = (GCGE_INT *)malloc(2 * (size_t)w_length * sizeof(GCGE_INT));
@@ -76,8 +76,10 @@ void GCGE_ComputeW(void *A, void *B, void **V, GCGE_DOUBLE *eval, GCGE_OPS *ops,
     free(cols);
 }
 
-static void compute_w_solve(void *A, void *B, void **V, GCGE_DOUBLE *eval, GCGE_OPS *ops,
-                            GCGE_PARA *para, GCGE_WORKSPACE *ws)
+/* returns 1 when the block CG already left W as the correction (the
+ * backend's BlockCGCorrection, asked for by want_correction) */
+static int compute_w_solve(void *A, void *B, void **V, GCGE_DOUBLE *eval, GCGE_OPS *ops,
+                           GCGE_PARA *para, GCGE_WORKSPACE *ws, int want_correction)
 {
     const GCGE_INT w_start = ws->dim_xp;
     const GCGE_INT w_length = ws->unconv_bs;
@@ -155,7 +157,7 @@ static void compute_w_solve(void *A, void *B, void **V, GCGE_DOUBLE *eval, GCGE_
 
     if (ops->MultiLinearSolver) {
         ops->MultiLinearSolver(A, V_tmp, V, mv_s, mv_e, ops);
-        return;
+        return 0;
     }
     if (ops->LinearSolver) {
         for (GCGE_INT idx = 0; idx < w_length; ++idx) {
@@ -165,7 +167,7 @@ static void compute_w_solve(void *A, void *B, void **V, GCGE_DOUBLE *eval, GCGE_
             ops->RestoreVecForMultiVec(V, w_start + idx, &wc, ops);
             ops->RestoreVecForMultiVec(V_tmp, idx, &rhs, ops);
         }
-        return;
+        return 0;
     }
     if (para->if_use_bcg == 1) {
         if (para->if_shift == 1) para->use_bcg_continuous = 0;
@@ -174,7 +176,12 @@ static void compute_w_solve(void *A, void *B, void **V, GCGE_DOUBLE *eval, GCGE_
             exit(1);
         }
         GCGE_OPS_EXT *ext = GCGE_OPS_GetExt(ops);
-        if (para->use_ops_ext && ext && ext->BlockCG) {
+        if (para->use_ops_ext && ext && want_correction && ext->BlockCGCorrection) {
+            ext->BlockCGCorrection(A, para->if_shift, para->shift, B, V_tmp, V, mv_s, mv_e,
+                                   para->cg_max_it, para->cg_rate, ops, ws->CG_p, ws->evec,
+                                   ws->subspace_dtmp, ws->subspace_itmp, para->bcg_by_column);
+            return 1;
+        } else if (para->use_ops_ext && ext && ext->BlockCG) {
             ext->BlockCG(A, para->if_shift, para->shift, B, V_tmp, V, mv_s, mv_e, para->cg_max_it,
                          para->cg_rate, ops, ws->CG_p, ws->evec, ws->subspace_dtmp,
                          ws->subspace_itmp, para->bcg_by_column);
@@ -192,6 +199,7 @@ static void compute_w_solve(void *A, void *B, void **V, GCGE_DOUBLE *eval, GCGE_
             ops->RestoreVecForMultiVec(V_tmp, idx, &rhs, ops);
         }
     }
+    return 0;
 }
 
 void GCGE_ComputeP(GCGE_DOUBLE *subspace_evec, void **V, GCGE_OPS *ops, GCGE_PARA *para,

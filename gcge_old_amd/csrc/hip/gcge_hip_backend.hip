@@ -1233,9 +1233,12 @@ static void bcg_iterate_dev(Mat *A, int n, int len, PT *Rd, long ldr, int r0, XT
         }
 }
 
+// correction: the solution columns leave as the correction to their starting
+// values (the float64 iteration starts from zero once r0 = b - A x0 is
+// formed; the float32 one returns its correction D without adding x0)
 static void block_cg(void *Amat, int if_shift, double shift, void *Bmat, void **RHS, void **V,
                      int *start, int *end, int max_it, double rate, void **CG_P, void **CG_W,
-                     double *dtmp, int *itmp, int by_column = 0)
+                     double *dtmp, int *itmp, int by_column = 0, bool correction = false)
 {
     Mat *A = as_mat(Amat);
     Mat *B = Bmat ? as_mat(Bmat) : nullptr;
@@ -1306,6 +1309,8 @@ static void block_cg(void *Amat, int if_shift, double shift, void *Bmat, void **
         }
     }
     axpby(n, -1.0, Wd + (long)w0 * ldw, ldw, 1.0, Rd + (long)start[0] * ldr, ldr, len);
+    if (correction && !f32)  // x = 0 (a = b = 0: y = 0 y)
+        axpby(n, 0.0, Xd + (long)start[1] * ldx, ldx, 0.0, Xd + (long)start[1] * ldx, ldx, len);
     // bookkeeping on the device for the unshifted solve (the GCG inner
     // solve); the shifted form keeps the host loop
     const bool dev = if_shift != 1 && len <= kBcgControlMax && !getenv_flag("GCGE_HIP_BCG_HOST");
@@ -1329,7 +1334,10 @@ static void block_cg(void *Amat, int if_shift, double shift, void *Bmat, void **
             bcg_iterate_dev<float, float>(A, n, len, Rf, lrf, 0, Df, ldf, 0, Pf, lpf, 0, Wf, lwf, 0,
                                           max_it, rate, rho1, rho2, err, last_err, unlock,
                                           &num_unlock, d_st, d_it, by_column);
-            add_float(n, Df, ldf, Xd + (long)start[1] * ldx, ldx, len);
+            if (correction)
+                block_to_double(Df, ldf, Xd + (long)start[1] * ldx, ldx, n, len);
+            else
+                add_float(n, Df, ldf, Xd + (long)start[1] * ldx, ldx, len);
             block_to_double(Rf, lrf, Rd + (long)start[0] * ldr, ldr, n, len);
         } else {
             bcg_iterate_dev<double, double>(A, n, len, Rd, ldr, start[0], Xd, ldx, start[1], Pd, ldp,
@@ -1364,7 +1372,10 @@ static void block_cg(void *Amat, int if_shift, double shift, void *Bmat, void **
         bcg_iterate<float, float>(A, B, if_shift, shift, n, len, Rf, lrf, 0, Df, ldf, 0, Pf, lpf,
                                   0, Wf, lwf, 0, Tf, ltf, max_it, rate, rho1, rho2, err, last_err,
                                   unlock, num_unlock, by_column);
-        add_float(n, Df, ldf, Xd + (long)start[1] * ldx, ldx, len);
+        if (correction)
+            block_to_double(Df, ldf, Xd + (long)start[1] * ldx, ldx, n, len);
+        else
+            add_float(n, Df, ldf, Xd + (long)start[1] * ldx, ldx, len);
         block_to_double(Rf, lrf, Rd + (long)start[0] * ldr, ldr, n, len);
     } else {
         bcg_iterate<double, double>(A, B, if_shift, shift, n, len, Rd, ldr, start[0], Xd, ldx,
@@ -1380,6 +1391,16 @@ static void ext_block_cg(void *A, GCGE_INT if_shift, GCGE_DOUBLE shift, void *B,
 {
     block_cg(A, if_shift, shift, B, RHS, V, start, end, max_it, rate, CG_P, CG_W, dtmp, itmp,
              by_column);
+}
+
+static void ext_block_cg_correction(void *A, GCGE_INT if_shift, GCGE_DOUBLE shift, void *B,
+                                    void **RHS, void **V, GCGE_INT *start, GCGE_INT *end,
+                                    GCGE_INT max_it, GCGE_DOUBLE rate, GCGE_OPS *, void **CG_P,
+                                    void **CG_W, GCGE_DOUBLE *dtmp, GCGE_INT *itmp,
+                                    GCGE_INT by_column)
+{
+    block_cg(A, if_shift, shift, B, RHS, V, start, end, max_it, rate, CG_P, CG_W, dtmp, itmp,
+             by_column, true);
 }
 
 static void op_multi_linear_solver(void *A, void **b, void **x, int *start, int *end, GCGE_OPS *)
@@ -1558,7 +1579,8 @@ static int ext_dense_eig(GCGE_DOUBLE *a, GCGE_INT lda, GCGE_INT n, GCGE_DOUBLE *
 }
 
 static GCGE_OPS_EXT g_ext = {GCGE_OPS_EXT_MAGIC, ext_residual_norms, ext_block_cg, ext_sync,
-                             ext_mat_dot_cols, ext_axpby_cols, ext_dense_eig, nullptr};
+                             ext_mat_dot_cols, ext_axpby_cols, ext_dense_eig, nullptr,
+                             ext_block_cg_correction};
 
 // ---- host copies ----------------------------------------------------------------------------------------------
 static void mv_set_host(void **mv, int start, int ncols, const double *host, int ld)

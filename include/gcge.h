@@ -62,6 +62,14 @@ typedef struct GCGE_OPS_EXT_ {
     int (*DenseMatEigenSolver)(GCGE_DOUBLE *a, GCGE_INT lda, GCGE_INT nrows, GCGE_DOUBLE *eval,
                                GCGE_DOUBLE *evec, GCGE_INT lde, GCGE_INT il, GCGE_INT iu);
     void *backend; /* backend private state */
+    /* BlockCG whose solution columns leave as the correction to the values
+     * they start from (w - w0 instead of w; the same iteration).  May be
+     * NULL. */
+    void (*BlockCGCorrection)(void *A, GCGE_INT if_shift, GCGE_DOUBLE shift, void *B,
+                              void **RHS, void **V, GCGE_INT *start, GCGE_INT *end,
+                              GCGE_INT max_it, GCGE_DOUBLE rate, GCGE_OPS *ops,
+                              void **CG_P, void **CG_W, GCGE_DOUBLE *dtmp, GCGE_INT *itmp,
+                              GCGE_INT by_column);
 } GCGE_OPS_EXT;
 
 /* ---- statistics (reference src/gcge_statistics.h:23-42) ---- */
