@@ -181,6 +181,28 @@ def test_svqb_full_rr_device_matches_oracle():
     assert np.all(_residuals(K, M, dev["eval"], dev["evec"]) <= 1.05e-8)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_svqb_correction_block_cg_host_bookkeeping(precision, monkeypatch):
+    """The SVQB path asks the device block CG for the correction to its
+    starting values (BlockCGCorrection): with the bookkeeping on the device
+    and, under GCGE_HIP_BCG_HOST=1, in the host loop, both in float64 and
+    with the float32 inner solve, the solve reaches the closed-form spectrum
+    and the two bookkeeping forms agree."""
+    E, nev = 14, 24
+    K, M = problems.q1_fem_3d(E)
+    x0 = oracle.random_fill(5, 0, 0, nev, K.nrows)
+    flags = ["-gcge_conv_type", "RB", "-gcge_ev_tol", "1e-8", "-gcge_ev_max_it", "150",
+             "-gcge_full_rr", "1", "-gcge_w_orth_type", "svqb", "-gcge_max_reorth_time", "2"]
+    dev = gcge.solve(K, M, nev, flags, x0=x0, precision=precision)
+    monkeypatch.setenv("GCGE_HIP_BCG_HOST", "1")
+    host = gcge.solve(K, M, nev, flags, x0=x0, precision=precision)
+    for out in (dev, host):
+        assert out["num_conv"] == nev
+        np.testing.assert_allclose(np.sort(out["eval"]), problems.q1_eigs(E, nev), rtol=1e-10)
+        assert np.all(_residuals(K, M, out["eval"], out["evec"]) <= 1.05e-8)
+    np.testing.assert_allclose(host["eval"], dev["eval"], rtol=1e-10)
+
+
 def test_c_program_through_solver_init():
     """A C program on the public headers only (gcge_old_amd/csrc/examples/
     q1_solve.c): device-assembled Q1 pencil, GCGE_HIP_Solver_Init (the
