@@ -203,6 +203,26 @@ def test_svqb_correction_block_cg_host_bookkeeping(precision, monkeypatch):
     np.testing.assert_allclose(host["eval"], dev["eval"], rtol=1e-10)
 
 
+@pytest.mark.parametrize("eta", ["0", "1"])
+def test_svqb_projection_pass_rules_on_device(eta):
+    """-gcge_svqb_dgks_eta 0 (W as the reference forms it, both projection
+    passes) and 1 (W as a correction, the second pass forced on every
+    block) on the device: the closed-form spectrum with B-orthonormal
+    eigenvectors, as with the default (the other tests)."""
+    E, nev = 14, 24
+    K, M = problems.q1_fem_3d(E)
+    x0 = oracle.random_fill(6, 0, 0, nev, K.nrows)
+    flags = ["-gcge_conv_type", "RB", "-gcge_ev_tol", "1e-8", "-gcge_ev_max_it", "150",
+             "-gcge_full_rr", "1", "-gcge_w_orth_type", "svqb", "-gcge_max_reorth_time", "2",
+             "-gcge_svqb_dgks_eta", eta]
+    out = gcge.solve(K, M, nev, flags, x0=x0, precision="mixed")
+    assert out["num_conv"] == nev
+    np.testing.assert_allclose(np.sort(out["eval"]), problems.q1_eigs(E, nev), rtol=1e-10)
+    assert np.all(_residuals(K, M, out["eval"], out["evec"]) <= 1.05e-8)
+    X = out["evec"]
+    assert np.abs(X.T @ (M.to_scipy() @ X) - np.eye(nev)).max() < 1e-10
+
+
 def test_c_program_through_solver_init():
     """A C program on the public headers only (gcge_old_amd/csrc/examples/
     q1_solve.c): device-assembled Q1 pencil, GCGE_HIP_Solver_Init (the
