@@ -89,6 +89,43 @@ static int compute_w_solve(void *A, void *B, void **V, GCGE_DOUBLE *eval, GCGE_O
     GCGE_INT mv_s[5], mv_e[5];
     GCGE_OPS_EXT *ext = GCGE_OPS_GetExt(ops);
 
+    /* ranges: [0] rhs in V_tmp, [1] solution in V, [2] CG_p, [3] scratch, [4] spare */
+    mv_s[0] = 0; mv_e[0] = w_length;
+    mv_s[1] = w_start; mv_e[1] = w_start + w_length;
+    mv_s[2] = 0; mv_e[2] = w_length;
+    mv_s[3] = 0; mv_e[3] = w_length;
+    mv_s[4] = 0; mv_e[4] = w_length;
+
+    /* the correction straight from the residual check's products: with
+     * rhs = s1 B x and w0 = s2 x (as set below), r0 = rhs - A w0 =
+     * s1 B x - s2 A x is formed from the B X and A X of the convergence check
+     * (X = evec, unchanged since), and the block CG starts from it */
+    if (want_correction && para->use_ops_ext && ext && ext->RhsFromResidual &&
+        ext->BlockCGFromResidual && w_length > 0 && para->if_use_bcg == 1 &&
+        para->if_shift != 1 && !para->use_bcg_continuous && !ops->MultiLinearSolver &&
+        !ops->LinearSolver) {
+        GCGE_INT *cols = (GCGE_INT *)malloc(2 * (size_t)w_length * sizeof(GCGE_INT));
+        GCGE_DOUBLE *s12 = (GCGE_DOUBLE *)malloc(2 * (size_t)w_length * sizeof(GCGE_DOUBLE));
+        for (GCGE_INT idx = 0; idx < w_length; ++idx) {
+            const GCGE_INT xi = unlock[idx];
+            const int small = fabs(eval[xi]) <= 1.0;
+            cols[idx] = xi;
+            cols[w_length + idx] = idx;
+            s12[idx] = small ? eval[xi] + para->shift : 1.0;
+            s12[w_length + idx] = small ? 1.0 : 1.0 / (eval[xi] + para->shift);
+        }
+        const int rc = ext->RhsFromResidual(A, B, ws->evec, cols, s12, s12 + w_length, V_tmp,
+                                            cols + w_length, w_length, ops);
+        free(cols);
+        free(s12);
+        if (rc == 0) {
+            ext->BlockCGFromResidual(A, para->if_shift, para->shift, B, V_tmp, V, mv_s, mv_e,
+                                     para->cg_max_it, para->cg_rate, ops, ws->CG_p, ws->evec,
+                                     ws->subspace_dtmp, ws->subspace_itmp, para->bcg_by_column);
+            return 1;
+        }
+    }
+
     if (para->use_ops_ext && ext && ext->MatDotMultiVecCols && ext->MultiVecAxpbyCols &&
         w_length > 0) {
         /* the same per-column operations as the loop below, one call each:
@@ -147,13 +184,6 @@ static int compute_w_solve(void *A, void *B, void **V, GCGE_DOUBLE *eval, GCGE_O
         ops->RestoreVecForMultiVec(V, wi, &wc, ops);
         ops->RestoreVecForMultiVec(V_tmp, idx, &rhs, ops);
     }
-
-    /* ranges: [0] rhs in V_tmp, [1] solution in V, [2] CG_p, [3] scratch, [4] spare */
-    mv_s[0] = 0; mv_e[0] = w_length;
-    mv_s[1] = w_start; mv_e[1] = w_start + w_length;
-    mv_s[2] = 0; mv_e[2] = w_length;
-    mv_s[3] = 0; mv_e[3] = w_length;
-    mv_s[4] = 0; mv_e[4] = w_length;
 
     if (ops->MultiLinearSolver) {
         ops->MultiLinearSolver(A, V_tmp, V, mv_s, mv_e, ops);

@@ -723,8 +723,13 @@ struct Scratch {
 constexpr int kScratchSlots = 9;
 static Scratch g_scratch[kScratchSlots];
 
+// bumped by every request of scratch slots 0 and 1 (the residual products
+// kept in them stay valid while it does not change)
+static unsigned g_scratch_gen01 = 0;
+
 static void *scratch_bytes(int which, size_t need)
 {
+    if (which == 0 || which == 1) ++g_scratch_gen01;
     Scratch &s = g_scratch[which];
     if (need > s.cap || !s.data) {
         if (s.data) {
@@ -1236,10 +1241,14 @@ static void bcg_iterate_dev(Mat *A, int n, int len, PT *Rd, long ldr, int r0, XT
 // correction: the solution columns leave as the correction to their starting
 // values (the float64 iteration starts from zero once r0 = b - A x0 is
 // formed; the float32 one returns its correction D without adding x0)
+// from_residual: RHS already holds r0 = b - A x0 (x0 is not read) and the
+// solution columns receive the correction
 static void block_cg(void *Amat, int if_shift, double shift, void *Bmat, void **RHS, void **V,
                      int *start, int *end, int max_it, double rate, void **CG_P, void **CG_W,
-                     double *dtmp, int *itmp, int by_column = 0, bool correction = false)
+                     double *dtmp, int *itmp, int by_column = 0, bool correction = false,
+                     bool from_residual = false)
 {
+    if (from_residual) correction = true;
     Mat *A = as_mat(Amat);
     Mat *B = Bmat ? as_mat(Bmat) : nullptr;
     MultiVec *R = as_mv(RHS), *X = as_mv(V);
@@ -1297,18 +1306,20 @@ static void block_cg(void *Amat, int if_shift, double shift, void *Bmat, void **
     const long ldr = R->ld, ldx = X->ld;
 
     // r = A x (+ shift B x) into W, b -= r, rho2 = b.b   (all columns, float64)
-    work_spmv(A, len);
-    work_vec(len, len, n);
-    spmm(A, Xd + (long)start[1] * ldx, ldx, Wd + (long)w0 * ldw, ldw, len);
-    if (if_shift == 1) {
-        if (!B) {
-            axpby(n, shift, Xd + (long)start[1] * ldx, ldx, 1.0, Wd + (long)w0 * ldw, ldw, len);
-        } else {
-            spmm(B, Xd + (long)start[1] * ldx, ldx, T, ldt, len);
-            axpby(n, shift, T, ldt, 1.0, Wd + (long)w0 * ldw, ldw, len);
+    if (!from_residual) {
+        work_spmv(A, len);
+        work_vec(len, len, n);
+        spmm(A, Xd + (long)start[1] * ldx, ldx, Wd + (long)w0 * ldw, ldw, len);
+        if (if_shift == 1) {
+            if (!B) {
+                axpby(n, shift, Xd + (long)start[1] * ldx, ldx, 1.0, Wd + (long)w0 * ldw, ldw, len);
+            } else {
+                spmm(B, Xd + (long)start[1] * ldx, ldx, T, ldt, len);
+                axpby(n, shift, T, ldt, 1.0, Wd + (long)w0 * ldw, ldw, len);
+            }
         }
+        axpby(n, -1.0, Wd + (long)w0 * ldw, ldw, 1.0, Rd + (long)start[0] * ldr, ldr, len);
     }
-    axpby(n, -1.0, Wd + (long)w0 * ldw, ldw, 1.0, Rd + (long)start[0] * ldr, ldr, len);
     if (correction && !f32)  // x = 0 (a = b = 0: y = 0 y)
         axpby(n, 0.0, Xd + (long)start[1] * ldx, ldx, 0.0, Xd + (long)start[1] * ldx, ldx, len);
     // bookkeeping on the device for the unshifted solve (the GCG inner
@@ -1403,6 +1414,16 @@ static void ext_block_cg_correction(void *A, GCGE_INT if_shift, GCGE_DOUBLE shif
              by_column, true);
 }
 
+static void ext_block_cg_from_residual(void *A, GCGE_INT if_shift, GCGE_DOUBLE shift, void *B,
+                                       void **RHS, void **V, GCGE_INT *start, GCGE_INT *end,
+                                       GCGE_INT max_it, GCGE_DOUBLE rate, GCGE_OPS *, void **CG_P,
+                                       void **CG_W, GCGE_DOUBLE *dtmp, GCGE_INT *itmp,
+                                       GCGE_INT by_column)
+{
+    block_cg(A, if_shift, shift, B, RHS, V, start, end, max_it, rate, CG_P, CG_W, dtmp, itmp,
+             by_column, true, true);
+}
+
 static void op_multi_linear_solver(void *A, void **b, void **x, int *start, int *end, GCGE_OPS *)
 {
     block_cg(A, g_cg.if_shift, g_cg.shift, g_cg.B, b, x, start, end, g_cg.max_it, g_cg.rate,
@@ -1410,6 +1431,18 @@ static void op_multi_linear_solver(void *A, void **b, void **x, int *start, int 
 }
 
 // ---- residual norms of a column block --------------------------------------------------------------------
+// A X and B X of the last residual-norm call, kept in scratch slots 0 and 1
+// for the right-hand sides of the inner solve that follows it
+struct ResidualProducts {
+    const Mat *A = nullptr, *B = nullptr;
+    const MultiVec *X = nullptr;
+    int start = 0, k = 0;
+    const double *AX = nullptr, *BX = nullptr;
+    long ldax = 0, ldbx = 0;
+    unsigned gen = 0;
+};
+static ResidualProducts g_resprod;
+
 static void ext_residual_norms(void *Amat, void *Bmat, void **Xh, GCGE_INT start, GCGE_INT end,
                                const GCGE_DOUBLE *eval, GCGE_DOUBLE *rnorm, GCGE_DOUBLE *xnorm,
                                GCGE_DOUBLE *bxnorm, GCGE_OPS *)
@@ -1434,6 +1467,8 @@ static void ext_residual_norms(void *Amat, void *Bmat, void **Xh, GCGE_INT start
         spmm(as_mat(Bmat), Xc, X->ld, t, ldbx, k);
         BX = t;
     }
+    g_resprod = ResidualProducts{A, Bmat ? as_mat(Bmat) : nullptr, X, (int)start, k, AX, BX, ldax,
+                                 ldbx, g_scratch_gen01};
     double *d = small_buffer2((size_t)4 * k);
     upload_async(d, eval, (size_t)k * sizeof(double));
     residual_sq(n, AX, ldax, BX, ldbx, Xc, X->ld, d, k, d + k);
@@ -1445,6 +1480,41 @@ static void ext_residual_norms(void *Amat, void *Bmat, void **Xh, GCGE_INT start
         xnorm[c] = std::sqrt(h[k + c]);
         if (bxnorm) bxnorm[c] = std::sqrt(h[2 * k + c]);
     }
+}
+
+static int *upload_maps(const GCGE_INT *xcols, const GCGE_INT *ycols, int k);
+
+static int ext_rhs_from_residual(void *Amat, void *Bmat, void **Xh, const GCGE_INT *xcols,
+                                 const GCGE_DOUBLE *s1, const GCGE_DOUBLE *s2, void **Rh,
+                                 const GCGE_INT *rcols, GCGE_INT k, GCGE_OPS *)
+{
+    const ResidualProducts &p = g_resprod;
+    if (k <= 0) return 0;
+    if (!p.X || p.gen != g_scratch_gen01 || p.A != as_mat(Amat) ||
+        p.B != (Bmat ? as_mat(Bmat) : nullptr) || p.X != as_mv(Xh))
+        return 1;
+    MultiVec *R = as_mv(Rh);
+    if (R->n != p.X->n) return 1;
+    std::vector<int> pc(k);
+    for (int c = 0; c < k; ++c) {
+        if (xcols[c] < p.start || xcols[c] >= p.start + p.k) return 1;
+        check_range(R, rcols[c], 1, "RhsFromResidual");
+        pc[c] = xcols[c] - p.start;
+    }
+    work_vec(0, 2.0 * k, R->n);
+    int *d = upload_maps(pc.data(), rcols, k);
+    std::vector<double> coef((size_t)4 * k);
+    for (int c = 0; c < k; ++c) {
+        coef[c] = s1[c];           // R = s1 BX      (b = 0: R not read)
+        coef[k + c] = 0.0;
+        coef[2 * k + c] = -s2[c];  // R = -s2 AX + R
+        coef[3 * k + c] = 1.0;
+    }
+    double *dc = small_buffer(coef.size());
+    upload_async(dc, coef.data(), coef.size() * sizeof(double));
+    axpby_list(R->n, dc, p.BX, p.ldbx, d, dc + k, R->data, R->ld, d + k, k);
+    axpby_list(R->n, dc + 2 * k, p.AX, p.ldax, d, dc + 3 * k, R->data, R->ld, d + k, k);
+    return 0;
 }
 
 static void ext_sync(GCGE_OPS *) { HIP_CHECK(hipStreamSynchronize(state().stream)); }
@@ -1580,7 +1650,8 @@ static int ext_dense_eig(GCGE_DOUBLE *a, GCGE_INT lda, GCGE_INT n, GCGE_DOUBLE *
 
 static GCGE_OPS_EXT g_ext = {GCGE_OPS_EXT_MAGIC, ext_residual_norms, ext_block_cg, ext_sync,
                              ext_mat_dot_cols, ext_axpby_cols, ext_dense_eig, nullptr,
-                             ext_block_cg_correction};
+                             ext_block_cg_correction, ext_rhs_from_residual,
+                             ext_block_cg_from_residual};
 
 // ---- host copies ----------------------------------------------------------------------------------------------
 static void mv_set_host(void **mv, int start, int ncols, const double *host, int ld)

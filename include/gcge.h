@@ -70,6 +70,22 @@ typedef struct GCGE_OPS_EXT_ {
                               GCGE_INT max_it, GCGE_DOUBLE rate, GCGE_OPS *ops,
                               void **CG_P, void **CG_W, GCGE_DOUBLE *dtmp, GCGE_INT *itmp,
                               GCGE_INT by_column);
+    /* R(:, rcols[c]) = s1[c] B x - s2[c] A x for x = X(:, xcols[c]) (B x = x
+     * when B is NULL), from the products of the last MultiVecResidualNorms
+     * call on X, whose columns the caller has left unchanged since; returns
+     * nonzero, doing nothing, when that call did not cover these columns of
+     * this X with these A and B.  May be NULL. */
+    int (*RhsFromResidual)(void *A, void *B, void **X, const GCGE_INT *xcols,
+                           const GCGE_DOUBLE *s1, const GCGE_DOUBLE *s2, void **R,
+                           const GCGE_INT *rcols, GCGE_INT k, GCGE_OPS *ops);
+    /* BlockCG from the initial residual: RHS holds b - A x0 on entry and the
+     * solution columns receive the correction x - x0 (x0 is not read).  May
+     * be NULL. */
+    void (*BlockCGFromResidual)(void *A, GCGE_INT if_shift, GCGE_DOUBLE shift, void *B,
+                                void **RHS, void **V, GCGE_INT *start, GCGE_INT *end,
+                                GCGE_INT max_it, GCGE_DOUBLE rate, GCGE_OPS *ops,
+                                void **CG_P, void **CG_W, GCGE_DOUBLE *dtmp, GCGE_INT *itmp,
+                                GCGE_INT by_column);
 } GCGE_OPS_EXT;
 
 /* ---- statistics (reference src/gcge_statistics.h:23-42) ---- */
