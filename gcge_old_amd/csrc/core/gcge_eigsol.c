@@ -316,11 +316,13 @@ void GCGE_CheckConvergence(void *A, void *B, GCGE_DOUBLE *eval, void **evec, GCG
     /* block path: all candidate columns at once */
     GCGE_DOUBLE *rn = NULL, *xn = NULL, *bxn = NULL;
     int have_block = 0;
+    ws->residual_products_fresh = 0;
     if (nev > start) {
         rn = (GCGE_DOUBLE *)malloc(3 * (size_t)(nev - start) * sizeof(GCGE_DOUBLE));
         xn = rn + (nev - start);
         bxn = xn + (nev - start);
         have_block = block_residuals(A, B, eval, evec, ops, para, start, nev, rn, xn, bxn);
+        ws->residual_products_fresh = have_block && evec == ws->evec;
     }
 
     void *tmp1 = NULL, *tmp2 = NULL;

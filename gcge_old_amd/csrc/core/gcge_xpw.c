@@ -27,6 +27,7 @@ void GCGE_ComputeX(void **V, GCGE_DOUBLE *subspace_evec, void **X, GCGE_INT size
     GCGE_INT sx = ws->num_soft_locked_in_last_iter;
     GCGE_INT start[2] = {sx, sx};
     GCGE_INT end[2] = {size_V, size_X};
+    ws->residual_products_fresh = 0;  /* X changes */
     ops->MultiVecLinearComb(V, X, start, end, subspace_evec + (size_t)sx * size_V + sx, size_V, 0,
                             1.0, 0.0, ops);
 }
@@ -46,6 +47,7 @@ void GCGE_ComputeW(void *A, void *B, void **V, GCGE_DOUBLE *eval, GCGE_OPS *ops,
 {
     const int want = para->svqb_dgks_eta > 0.0 && strcmp(para->w_orth_type, "svqb") == 0;
     const int corrected = compute_w_solve(A, B, V, eval, ops, para, ws, want);
+    ws->residual_products_fresh = 0;  /* the inner solve uses evec as scratch */
     const GCGE_INT w_start = ws->dim_xp;
     const GCGE_INT w_length = ws->unconv_bs;
     if (!want || corrected || w_length <= 0) return;
@@ -100,7 +102,8 @@ static int compute_w_solve(void *A, void *B, void **V, GCGE_DOUBLE *eval, GCGE_O
      * rhs = s1 B x and w0 = s2 x (as set below), r0 = rhs - A w0 =
      * s1 B x - s2 A x is formed from the B X and A X of the convergence check
      * (X = evec, unchanged since), and the block CG starts from it */
-    if (want_correction && para->use_ops_ext && ext && ext->RhsFromResidual &&
+    if (want_correction && ws->residual_products_fresh && para->use_ops_ext && ext &&
+        ext->RhsFromResidual &&
         ext->BlockCGFromResidual && w_length > 0 && para->if_use_bcg == 1 &&
         para->if_shift != 1 && !para->use_bcg_continuous && !ops->MultiLinearSolver &&
         !ops->LinearSolver) {

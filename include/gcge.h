@@ -252,6 +252,9 @@ typedef struct GCGE_WORKSPACE_ {
     GCGE_INT *subspace_itmp;
     GCGE_DOUBLE *eval;
     GCGE_INT *orth_ind;
+    /* added: 1 from a convergence check that formed the residual products
+     * of evec on the backend until the next inner solve reads them */
+    GCGE_INT residual_products_fresh;
 } GCGE_WORKSPACE;
 
 typedef struct GCGE_SOLVER_ {
