@@ -932,6 +932,26 @@ static void op_mv_lincomb(void **x, void **y, GCGE_INT *start, GCGE_INT *end, do
     lincomb_dev(Y->n, col_ptr(X, start[0]), X->ld, m, d_C, ldct, beta, col_ptr(Y, start[1]), Y->ld, k);
 }
 
+static int ext_lincomb_in_place(void **v, GCGE_INT start, GCGE_INT m, const GCGE_DOUBLE *a,
+                                GCGE_INT lda, GCGE_INT k, GCGE_OPS *)
+{
+    MultiVec *X = as_mv(v);
+    if (k <= 0) return 0;
+    if (!lincomb_in_place_ok(m, k)) return 1;
+    check_range(X, start, m, "MultiVecLinearCombInPlace");
+    // the reference's work for the combination and the copy back
+    work_vec(0, (double)m * k + k, X->n);
+    const int g = lincomb_group(k);
+    const int ldct = (k + g - 1) / g * g;
+    std::vector<double> packed((size_t)m * ldct, 0.0);
+    for (int j = 0; j < k; ++j)
+        for (int i = 0; i < m; ++i) packed[(size_t)i * ldct + j] = a[(size_t)j * lda + i];
+    double *d_C = small_buffer(packed.size());
+    upload_async(d_C, packed.data(), packed.size() * sizeof(double));
+    lincomb_in_place(X->n, col_ptr(X, start), X->ld, m, d_C, ldct, k);
+    return 0;
+}
+
 static void op_mv_inner(void **V, void **W, double *a, char *is_sym, GCGE_INT *start,
                         GCGE_INT *end, GCGE_INT lda, GCGE_INT if_Vec, GCGE_OPS *)
 {
@@ -1651,7 +1671,7 @@ static int ext_dense_eig(GCGE_DOUBLE *a, GCGE_INT lda, GCGE_INT n, GCGE_DOUBLE *
 static GCGE_OPS_EXT g_ext = {GCGE_OPS_EXT_MAGIC, ext_residual_norms, ext_block_cg, ext_sync,
                              ext_mat_dot_cols, ext_axpby_cols, ext_dense_eig, nullptr,
                              ext_block_cg_correction, ext_rhs_from_residual,
-                             ext_block_cg_from_residual};
+                             ext_block_cg_from_residual, ext_lincomb_in_place};
 
 // ---- host copies ----------------------------------------------------------------------------------------------
 static void mv_set_host(void **mv, int start, int ncols, const double *host, int ld)

@@ -2752,6 +2752,50 @@ void gram(int n, const double *V, long ldv, int m, const double *W, long ldw, in
 
 int lincomb_group(int k) { return k <= 8 ? 8 : k <= 16 ? 16 : k <= 32 ? 32 : 64; }
 
+// the kernel switches of lincomb_dev (measurement): GCGE_HIP_LINCOMB=fma
+// for the vector kernel, GCGE_HIP_LINCOMB_LDS=0 for the wavefront kernels
+static int lincomb_mode()
+{
+    static const int mode = [] {
+        const char *e = getenv("GCGE_HIP_LINCOMB");
+        return (e && strcmp(e, "fma") == 0) ? 1 : 0;
+    }();
+    return mode;
+}
+static int lincomb_lds_mode()
+{
+    static const int v = [] {
+        const char *e = getenv("GCGE_HIP_LINCOMB_LDS");
+        return e ? atoi(e) : 1;
+    }();
+    return v;
+}
+
+bool lincomb_in_place_ok(int m, int k)
+{
+    return k > 32 && k <= 128 && m >= 32 && k <= m && !use_blas() && lincomb_mode() == 0 &&
+           lincomb_lds_mode() != 0;
+}
+
+// V(:, 0:k) = V(:, 0:m) C in place (k <= m).  The LDS-staged kernel loads all
+// m input columns of a workgroup's 128 rows in its stage loop, which ends in
+// a workgroup barrier, and writes those rows only after it; with k <= 128 a
+// single workgroup owns each row block, so the output may overwrite the
+// input (the same kernel, the same arithmetic as the out-of-place call).
+void lincomb_in_place(int n, double *V, long ldv, int m, const double *d_Ct, int ldct, int k)
+{
+    if (n <= 0 || k <= 0) return;
+    if (!lincomb_in_place_ok(m, k)) fail_msg("lincomb_in_place: shape not supported");
+    ProfScope prof("lincomb", 8.0 * (double)n * (m + k));
+    const dim3 g(ceil_div(n, 128), 1), block(kBlock);
+    hipStream_t st = state().stream;
+    if (k > 64)
+        hipLaunchKernelGGL((k_lincomb_lds<2, 8>), g, block, 0, st, n, V, ldv, m, d_Ct, ldct, 0.0, V, ldv, k);
+    else
+        hipLaunchKernelGGL((k_lincomb_lds<1, 8>), g, block, 0, st, n, V, ldv, m, d_Ct, ldct, 0.0, V, ldv, k);
+    HIP_CHECK(hipGetLastError());
+}
+
 void lincomb_dev(int n, const double *V, long ldv, int m, const double *d_Ct, int ldct,
                  double beta, double *Y, long ldy, int k)
 {
@@ -2771,19 +2815,13 @@ void lincomb_dev(int n, const double *V, long ldv, int m, const double *d_Ct, in
     }
     const dim3 grid(ceil_div(n, kBlock)), block(kBlock);
     hipStream_t st = state().stream;
-    static const int mode = [] {
-        const char *e = getenv("GCGE_HIP_LINCOMB");  // "fma": the vector kernel (measurement)
-        return (e && strcmp(e, "fma") == 0) ? 1 : 0;
-    }();
+    const int mode = lincomb_mode();
     // matrix cores for 9..16 and more than 32 columns: 64 rows x 16 columns
     // or 32 rows x 64 columns per wavefront; measured on MI355X (n = 250047):
     // m = 70, k = 10: 27.7 us against 34.1 (vector kernel); m = 90, k = 50:
     // 113.7 against 136.3; m = 80, k = 30: 57.5 against 47.9, so the vector
     // kernel keeps 17..32
-    static const int lds_mode = [] {
-        const char *e = getenv("GCGE_HIP_LINCOMB_LDS");  // 0: the wavefront kernels (measurement)
-        return e ? atoi(e) : 1;
-    }();
+    const int lds_mode = lincomb_lds_mode();
     if (mode == 0 && lds_mode && k > 32 && m >= 32) {
         // V staged through LDS: 128 rows x up to 128 columns per workgroup
         const int tjw = std::min(k, 128) > 64 ? 2 : 1;
