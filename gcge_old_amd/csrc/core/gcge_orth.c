@@ -346,7 +346,7 @@ static int svqb_orth(void **V, GCGE_INT start, GCGE_INT *end, void *B, GCGE_OPS 
         }
         GCGE_OPS_EXT *ext = GCGE_OPS_GetExt(ops);
         if (r > 0 && !(para->use_ops_ext && ext && ext->MultiVecLinearCombInPlace &&
-                       ext->MultiVecLinearCombInPlace(V, start, k, C, k, r, ops) == 0)) {
+                       ext->MultiVecLinearCombInPlace(V, start, k, C, k, start, r, ops) == 0)) {
             /* V_tmp(:, 0:r) = W C; W(:, 0:r) = V_tmp(:, 0:r) */
             mv_s[0] = start; mv_e[0] = *end;
             mv_s[1] = 0; mv_e[1] = r;

@@ -287,18 +287,29 @@ void GCGE_ComputeP(GCGE_DOUBLE *subspace_evec, void **V, GCGE_OPS *ops, GCGE_PAR
      * then swap into V(:, nev:p_end) */
     const GCGE_INT nev = para->nev;
     const GCGE_INT p_end_tmp = p_end - nev;
-    mv_s[0] = orth_start; mv_e[0] = ldc;
-    mv_s[1] = 0; mv_e[1] = p_end_tmp;
-    ops->MultiVecLinearComb(V, V_tmp, mv_s, mv_e, coef + (size_t)nev * ldc + orth_start, ldc, 0,
-                            1.0, 0.0, ops);
-    if (strcmp(para->w_orth_type, "svqb") == 0) {
+    GCGE_OPS_EXT *ext_ip = GCGE_OPS_GetExt(ops);
+    if (strcmp(para->w_orth_type, "svqb") == 0 && para->use_ops_ext && ext_ip &&
+        ext_ip->MultiVecLinearCombInPlace &&
+        ext_ip->MultiVecLinearCombInPlace(V, orth_start, ldc - orth_start,
+                                          coef + (size_t)nev * ldc + orth_start, ldc, nev,
+                                          p_end_tmp, ops) == 0) {
+        /* V(:, nev:p_end) formed in place: V_tmp is not needed */
+    } else if (strcmp(para->w_orth_type, "svqb") == 0) {
         /* V_tmp is written before it is read again (the next right-hand
          * sides, projections and SVQB products), so a copy does what the
          * swap does for V with half the traffic on a device backend */
+        mv_s[0] = orth_start; mv_e[0] = ldc;
+        mv_s[1] = 0; mv_e[1] = p_end_tmp;
+        ops->MultiVecLinearComb(V, V_tmp, mv_s, mv_e, coef + (size_t)nev * ldc + orth_start, ldc, 0,
+                                1.0, 0.0, ops);
         mv_s[0] = 0; mv_e[0] = p_end_tmp;
         mv_s[1] = nev; mv_e[1] = p_end;
         ops->MultiVecAxpby(1.0, V_tmp, 0.0, V, mv_s, mv_e, ops);
     } else {
+        mv_s[0] = orth_start; mv_e[0] = ldc;
+        mv_s[1] = 0; mv_e[1] = p_end_tmp;
+        ops->MultiVecLinearComb(V, V_tmp, mv_s, mv_e, coef + (size_t)nev * ldc + orth_start, ldc, 0,
+                                1.0, 0.0, ops);
         mv_s[0] = nev; mv_e[0] = p_end;
         mv_s[1] = 0; mv_e[1] = p_end_tmp;
         ops->MultiVecSwap(V, V_tmp, mv_s, mv_e, ops);

@@ -932,13 +932,14 @@ static void op_mv_lincomb(void **x, void **y, GCGE_INT *start, GCGE_INT *end, do
     lincomb_dev(Y->n, col_ptr(X, start[0]), X->ld, m, d_C, ldct, beta, col_ptr(Y, start[1]), Y->ld, k);
 }
 
-static int ext_lincomb_in_place(void **v, GCGE_INT start, GCGE_INT m, const GCGE_DOUBLE *a,
-                                GCGE_INT lda, GCGE_INT k, GCGE_OPS *)
+static int ext_lincomb_in_place(void **v, GCGE_INT in, GCGE_INT m, const GCGE_DOUBLE *a,
+                                GCGE_INT lda, GCGE_INT out, GCGE_INT k, GCGE_OPS *)
 {
     MultiVec *X = as_mv(v);
     if (k <= 0) return 0;
     if (!lincomb_in_place_ok(m, k)) return 1;
-    check_range(X, start, m, "MultiVecLinearCombInPlace");
+    check_range(X, in, m, "MultiVecLinearCombInPlace");
+    check_range(X, out, k, "MultiVecLinearCombInPlace");
     // the reference's work for the combination and the copy back
     work_vec(0, (double)m * k + k, X->n);
     const int g = lincomb_group(k);
@@ -948,7 +949,7 @@ static int ext_lincomb_in_place(void **v, GCGE_INT start, GCGE_INT m, const GCGE
         for (int i = 0; i < m; ++i) packed[(size_t)i * ldct + j] = a[(size_t)j * lda + i];
     double *d_C = small_buffer(packed.size());
     upload_async(d_C, packed.data(), packed.size() * sizeof(double));
-    lincomb_in_place(X->n, col_ptr(X, start), X->ld, m, d_C, ldct, k);
+    lincomb_in_place(X->n, col_ptr(X, in), col_ptr(X, out), X->ld, m, d_C, ldct, k);
     return 0;
 }
 

@@ -199,9 +199,11 @@ void gram(int n, const double *V, long ldv, int m, const double *W, long ldw, in
 int lincomb_group(int k);
 void lincomb_dev(int n, const double *V, long ldv, int m, const double *d_Ct, int ldct,
                  double beta, double *Y, long ldy, int k);
-// V(:, 0:k) = V(:, 0:m) C in place, for the shapes lincomb_in_place_ok accepts
+// Y = V C with Y's columns allowed to overlap V's (same block of columns),
+// for the shapes lincomb_in_place_ok accepts
 bool lincomb_in_place_ok(int m, int k);
-void lincomb_in_place(int n, double *V, long ldv, int m, const double *d_Ct, int ldct, int k);
+void lincomb_in_place(int n, const double *V, double *Y, long ld, int m, const double *d_Ct, int ldct,
+                      int k);
 void axpby(int n, double a, const double *X, long ldx, double b, double *Y, long ldy, int k);
 void swap_cols(int n, double *X, long ldx, double *Y, long ldy, int k);
 // Y(:, ymap[c]) = a[c] X(:, xmap[c]) + b[c] Y(:, ymap[c]), scalars and maps on the device

@@ -2773,16 +2773,19 @@ static int lincomb_lds_mode()
 
 bool lincomb_in_place_ok(int m, int k)
 {
-    return k > 32 && k <= 128 && m >= 32 && k <= m && !use_blas() && lincomb_mode() == 0 &&
+    return k > 32 && k <= 128 && m >= 32 && !use_blas() && lincomb_mode() == 0 &&
            lincomb_lds_mode() != 0;
 }
 
-// V(:, 0:k) = V(:, 0:m) C in place (k <= m).  The LDS-staged kernel loads all
-// m input columns of a workgroup's 128 rows in its stage loop, which ends in
-// a workgroup barrier, and writes those rows only after it; with k <= 128 a
-// single workgroup owns each row block, so the output may overwrite the
-// input (the same kernel, the same arithmetic as the out-of-place call).
-void lincomb_in_place(int n, double *V, long ldv, int m, const double *d_Ct, int ldct, int k)
+// Y = V C where the columns of Y may overlap those of V (one block of
+// columns, same rows and leading dimension).  The LDS-staged kernel loads
+// all m input columns of a workgroup's 128 rows in its stage loop, which
+// ends in a workgroup barrier, and writes those rows only after it; with
+// k <= 128 a single workgroup owns each row block, so the output may
+// overwrite the input (the same kernel, the same arithmetic as the
+// out-of-place call).
+void lincomb_in_place(int n, const double *V, double *Y, long ld, int m, const double *d_Ct, int ldct,
+                      int k)
 {
     if (n <= 0 || k <= 0) return;
     if (!lincomb_in_place_ok(m, k)) fail_msg("lincomb_in_place: shape not supported");
@@ -2790,9 +2793,9 @@ void lincomb_in_place(int n, double *V, long ldv, int m, const double *d_Ct, int
     const dim3 g(ceil_div(n, 128), 1), block(kBlock);
     hipStream_t st = state().stream;
     if (k > 64)
-        hipLaunchKernelGGL((k_lincomb_lds<2, 8>), g, block, 0, st, n, V, ldv, m, d_Ct, ldct, 0.0, V, ldv, k);
+        hipLaunchKernelGGL((k_lincomb_lds<2, 8>), g, block, 0, st, n, V, ld, m, d_Ct, ldct, 0.0, Y, ld, k);
     else
-        hipLaunchKernelGGL((k_lincomb_lds<1, 8>), g, block, 0, st, n, V, ldv, m, d_Ct, ldct, 0.0, V, ldv, k);
+        hipLaunchKernelGGL((k_lincomb_lds<1, 8>), g, block, 0, st, n, V, ld, m, d_Ct, ldct, 0.0, Y, ld, k);
     HIP_CHECK(hipGetLastError());
 }
 

@@ -86,11 +86,12 @@ typedef struct GCGE_OPS_EXT_ {
                                 GCGE_INT max_it, GCGE_DOUBLE rate, GCGE_OPS *ops,
                                 void **CG_P, void **CG_W, GCGE_DOUBLE *dtmp, GCGE_INT *itmp,
                                 GCGE_INT by_column);
-    /* V(:, start:start+k) = V(:, start:start+m) C in place, C m x k (leading
-     * dimension ldc), k <= m; returns nonzero, doing nothing, for shapes the
-     * backend cannot combine in place.  May be NULL. */
-    int (*MultiVecLinearCombInPlace)(void **V, GCGE_INT start, GCGE_INT m, const GCGE_DOUBLE *C,
-                                     GCGE_INT ldc, GCGE_INT k, GCGE_OPS *ops);
+    /* V(:, out:out+k) = V(:, in:in+m) C within one multivector, C m x k
+     * (leading dimension ldc); the output columns may overlap the input
+     * ones.  Returns nonzero, doing nothing, for shapes the backend cannot
+     * combine in place.  May be NULL. */
+    int (*MultiVecLinearCombInPlace)(void **V, GCGE_INT in, GCGE_INT m, const GCGE_DOUBLE *C,
+                                     GCGE_INT ldc, GCGE_INT out, GCGE_INT k, GCGE_OPS *ops);
 } GCGE_OPS_EXT;
 
 /* ---- statistics (reference src/gcge_statistics.h:23-42) ---- */
